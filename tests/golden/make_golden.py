@@ -1,0 +1,116 @@
+"""Writes the golden fixtures under tests/golden/ by running the compiled reference.
+
+Run in the development container (where the reference sources compile):
+
+    make -C oracle ref N="16 64"
+    python tests/golden/make_golden.py
+
+Each fixture is an .npz of inputs and the reference's outputs (complex128), plus a
+JSON sidecar with the parameters and scalar results.  The reference itself never
+travels; only these vectors do.
+
+Fixtures:
+  g16_dirac   D_phi, D_dagger_phi, Level::D_operator (level 0) on a 16^2 Metropolis field
+  g16_setup2  2-level hierarchy (Nv=4, 4x4 aggregates, 4x4 SAP blocks): test vectors,
+              level-1 links, P^dagger b, D_1 P^dagger b, P P^dagger b, SAP(nu=1) b
+  g16_setup3  3-level hierarchy (2x2 aggregates, Nv=4): test vectors of levels 0 and 1,
+              level-2 links, transfer ops, D_1, SAP at levels 0 and 1
+  g64_solve   cfg1 shape (64^2, m0=-0.1, 2 levels, Nv=8, nu2=1, FGMRES(20)): K- and
+              V-cycle solutions, iteration counts and residuals
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from oracle import ddamg_oracle as orc  # noqa: E402
+from oracle import run_ref  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+RANKS = (2, 2)
+
+
+def save(name, meta, **arrays):
+    np.savez(os.path.join(OUT, name + '.npz'), **arrays)
+    with open(os.path.join(OUT, name + '.json'), 'w') as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print('wrote', name, {k: v.shape for k, v in arrays.items()})
+
+
+def field(N, seed, sweeps=2000):
+    theta = orc.gauge_metropolis(N, N, 2.0, sweeps, seed=seed)
+    return theta, np.exp(1j * theta)
+
+
+def gaussian(N, seed, dof=2):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((N, N, dof)) + 1j * rng.standard_normal((N, N, dof))
+
+
+def level_meta(levels):
+    return [dict(agg=list(s.agg), sap_block=list(s.sap_block), nv=s.nv) for s in levels]
+
+
+def main():
+    # ---- 16^2 operator fixtures -------------------------------------------------------
+    N, m0 = 16, -0.1
+    theta, U = field(N, seed=1601)
+    b = gaussian(N, 1602)
+    levels = [orc.LevelSpec((4, 4), (4, 4), 4)]
+    wd, _ = run_ref.run('dirac', U, b, levels, m0, ranks=RANKS)
+    save('g16_dirac', dict(N=N, m0=m0, beta=2.0, sweeps=2000, seed=1601, ranks=list(RANKS)),
+         theta=theta, rhs=b, Dphi=run_ref.load(wd, 'Dphi', RANKS),
+         Ddagphi=run_ref.load(wd, 'Ddagphi', RANKS), Dop0=run_ref.load(wd, 'Dop0', RANKS))
+    run_ref.cleanup(wd)
+
+    wd, _ = run_ref.run('setup', U, b, levels, m0, ranks=RANKS, nu2=2, sap_nu=1)
+    arr = dict(theta=theta, rhs=b,
+               tvec_l0=np.stack([run_ref.load(wd, f'tvec_l0_{c}', RANKS) for c in range(4)]),
+               G1_l1=run_ref.load(wd, 'G1_l1', RANKS), G2_l1=run_ref.load(wd, 'G2_l1', RANKS),
+               G3_l1=run_ref.load(wd, 'G3_l1', RANKS), Pdagg_rhs=run_ref.load(wd, 'Pdagg_rhs', RANKS),
+               Dop1_Pdagg_rhs=run_ref.load(wd, 'Dop1_Pdagg_rhs', RANKS),
+               P_Pdagg_rhs=run_ref.load(wd, 'P_Pdagg_rhs', RANKS), SAP0_rhs=run_ref.load(wd, 'SAP0_rhs', RANKS))
+    save('g16_setup2', dict(N=N, m0=m0, levels=level_meta(levels), ranks=list(RANKS), sap_nu=1,
+                            setup_sap_iters=4), **arr)
+    run_ref.cleanup(wd)
+
+    levels3 = [orc.LevelSpec((2, 2), (4, 4), 4), orc.LevelSpec((2, 2), (2, 2), 4)]
+    wd, _ = run_ref.run('setup', U, b, levels3, m0, ranks=RANKS, nu2=2, sap_nu=1)
+    arr = dict(theta=theta, rhs=b,
+               tvec_l0=np.stack([run_ref.load(wd, f'tvec_l0_{c}', RANKS) for c in range(4)]),
+               tvec_l1=np.stack([run_ref.load(wd, f'tvec_l1_{c}', RANKS) for c in range(4)]),
+               G1_l2=run_ref.load(wd, 'G1_l2', RANKS), G2_l2=run_ref.load(wd, 'G2_l2', RANKS),
+               G3_l2=run_ref.load(wd, 'G3_l2', RANKS), Pdagg_rhs=run_ref.load(wd, 'Pdagg_rhs', RANKS),
+               Dop1_Pdagg_rhs=run_ref.load(wd, 'Dop1_Pdagg_rhs', RANKS),
+               P_Pdagg_rhs=run_ref.load(wd, 'P_Pdagg_rhs', RANKS), SAP0_rhs=run_ref.load(wd, 'SAP0_rhs', RANKS),
+               SAP1_Pdagg_rhs=run_ref.load(wd, 'SAP1_Pdagg_rhs', RANKS))
+    save('g16_setup3', dict(N=N, m0=m0, levels=level_meta(levels3), ranks=list(RANKS), sap_nu=1,
+                            setup_sap_iters=4), **arr)
+    run_ref.cleanup(wd)
+
+    # ---- 64^2 full solve (cfg1 shape) ---------------------------------------------------
+    N, m0 = 64, -0.1
+    theta, U = field(N, seed=6401)
+    b = gaussian(N, 6402)
+    levels = [orc.LevelSpec((4, 4), (4, 4), 8)]
+    meta = dict(N=N, m0=m0, beta=2.0, sweeps=2000, seed=6401, levels=level_meta(levels), ranks=list(RANKS),
+                nu1=0, nu2=1, fgmres_m=20, fgmres_restarts=50, tol=1e-10,
+                plaquette=float(np.mean(np.cos(orc.plaquette_angle(theta)))))
+    xs = {}
+    for cyc in ('K', 'V'):
+        wd, _ = run_ref.run('solve', U, b, levels, m0, ranks=RANKS, nu2=1, fgmres_m=20, cycle=cyc)
+        res = run_ref.result(wd)
+        meta[f'{cyc}_iterations'] = res['iterations']
+        meta[f'{cyc}_relres'] = res['relres']
+        xs[cyc] = run_ref.load(wd, 'x', RANKS)
+        run_ref.cleanup(wd)
+    save('g64_solve', meta, theta=theta, rhs=b, x_K=xs['K'])
+
+
+if __name__ == '__main__':
+    main()
