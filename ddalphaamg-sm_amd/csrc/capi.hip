@@ -1,0 +1,436 @@
+// extern "C" boundary of libddamg.so (declarations and reference citations in
+// include/ddamg.h).  Converts between the boundary format (complex128, reference
+// interior order) and the device layouts, and maps exceptions to status codes.
+#include <cstring>
+#include <string>
+
+#include "solver.h"
+
+struct dd_ctx {
+    std::unique_ptr<dd::Solver> s;
+};
+
+namespace {
+std::string g_err;
+
+void set_err(const std::string& e) { g_err = e; }
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const dd::HipFailure& e) {
+        set_err(std::string("HIP: ") + hipGetErrorString(e.err));
+        return DD_ERR_HIP;
+    } catch (const dd::BadShape& e) {
+        set_err(e.what);
+        return DD_ERR_BAD_SHAPE;
+    } catch (const std::exception& e) {
+        set_err(e.what());
+        return DD_ERR_HIP;
+    }
+}
+
+using dd::cplx;
+using dd::DevMem;
+
+size_t vsize(dd_ctx* c) { return (size_t)c->s->Nx * c->s->Nt * 2; }
+
+// host <-> device staging of complex128 data
+struct Staged {
+    DevMem d;
+    Staged(const double* h, size_t ncplx, hipStream_t s) : d(sizeof(cplx<double>) * ncplx) {
+        if (h) DD_HIP(hipMemcpyAsync(d.as<void>(), h, sizeof(cplx<double>) * ncplx, hipMemcpyHostToDevice, s));
+    }
+    cplx<double>* p() { return d.as<cplx<double>>(); }
+};
+
+void download(double* h, const cplx<double>* d, size_t ncplx, hipStream_t s) {
+    DD_HIP(hipMemcpyAsync(h, d, sizeof(cplx<double>) * ncplx, hipMemcpyDeviceToHost, s));
+    DD_HIP(hipStreamSynchronize(s));
+}
+
+int check(dd_ctx* c) {
+    if (!c || !c->s) {
+        set_err("null context");
+        return DD_ERR_ARG;
+    }
+    return DD_OK;
+}
+
+template <typename R>
+dd::Multigrid<R>* mg(dd_ctx* c);
+template <>
+dd::Multigrid<double>* mg<double>(dd_ctx* c) {
+    if (!c->s->mg64) throw dd::BadShape("no FP64 hierarchy (setup with precision 0 first)");
+    return c->s->mg64.get();
+}
+template <>
+dd::Multigrid<float>* mg<float>(dd_ctx* c) {
+    if (!c->s->mg32) throw dd::BadShape("no FP32 hierarchy (setup with precision 1 first)");
+    return c->s->mg32.get();
+}
+
+// runs f(in_R, out_R) on level-shaped data converted to precision R
+template <typename R, typename F>
+void with_level_io(dd_ctx* c, const double* in, double* out, size_t n_in, size_t n_out, int nrhs, F&& f,
+                   const double* inout_init = nullptr) {
+    hipStream_t s = c->s->s;
+    Staged a(in, n_in * nrhs, s);
+    DevMem ar(sizeof(cplx<R>) * n_in * nrhs), br(sizeof(cplx<R>) * n_out * nrhs);
+    dd::convert<R, double>(s, ar.as<cplx<R>>(), a.p(), n_in * nrhs, 0, 0, 1, 1);
+    if (inout_init) {
+        Staged b0(inout_init, n_out * nrhs, s);
+        dd::convert<R, double>(s, br.as<cplx<R>>(), b0.p(), n_out * nrhs, 0, 0, 1, 1);
+        DD_HIP(hipStreamSynchronize(s));
+    } else {
+        dd::zero<R>(s, br.as<cplx<R>>(), n_out * nrhs, 0, 1, 1);
+    }
+    f(ar.as<cplx<R>>(), br.as<cplx<R>>());
+    DevMem o(sizeof(cplx<double>) * n_out * nrhs);
+    dd::convert<double, R>(s, o.as<cplx<double>>(), br.as<cplx<R>>(), n_out * nrhs, 0, 0, 1, 1);
+    download(out, o.as<cplx<double>>(), n_out * nrhs, s);
+}
+}  // namespace
+
+extern "C" {
+
+const char* dd_last_error(void) { return g_err.c_str(); }
+
+int dd_default_params(dd_params* p, int levels) {
+    if (!p || levels < 2 || levels > DD_MAX_LEVELS) return DD_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->levels = levels;
+    for (int l = 0; l < DD_MAX_LEVELS; l++) {
+        p->agg_x[l] = p->agg_t[l] = 4;
+        p->sap_x[l] = p->sap_t[l] = 4;
+        p->nv[l] = 8;
+    }
+    p->m0 = -0.1;
+    p->nu1 = 0;
+    p->nu2 = 2;
+    p->setup_sap_iters = 4;
+    p->sap_m = 5;
+    p->sap_restarts = 5;
+    p->sap_tol = 1e-3;
+    p->coarse_m = 400;
+    p->coarse_restarts = 10;
+    p->coarse_tol = 0.1;
+    p->kcycle_m = 5;
+    p->kcycle_restarts = 2;
+    p->kcycle_tol = 0.1;
+    p->fgmres_m = 20;
+    p->fgmres_restarts = 50;
+    p->tol = 1e-10;
+    p->cycle = 1;
+    p->precision = 0;
+    p->rng_ranks_x = 2;
+    p->rng_ranks_t = 2;
+    p->ortho_outer = 0;
+    p->ortho_inner = 0;
+    p->ortho_coarse = 1;
+    p->max_rhs = 1;
+    return DD_OK;
+}
+
+int dd_create(dd_ctx** ctx, int Nx, int Nt, int device) {
+    if (!ctx) return DD_ERR_ARG;
+    return guard([&] {
+        *ctx = nullptr;
+        auto* c = new dd_ctx;
+        try {
+            c->s.reset(new dd::Solver(Nx, Nt, device));
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *ctx = c;
+        return DD_OK;
+    });
+}
+
+int dd_destroy(dd_ctx* ctx) {
+    if (!ctx) return DD_OK;
+    return guard([&] {
+        delete ctx;
+        return DD_OK;
+    });
+}
+
+int dd_set_gauge(dd_ctx* c, const double* U) {
+    if (check(c) || !U) return DD_ERR_ARG;
+    return guard([&] {
+        Staged u(U, vsize(c), c->s->s);
+        c->s->set_gauge(u.p());
+        return DD_OK;
+    });
+}
+
+int dd_set_gauge_theta(dd_ctx* c, const double* theta) {
+    if (check(c) || !theta) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c);
+        DD_HIP(hipMemcpyAsync(c->s->theta.as<void>(), theta, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        DevMem u(sizeof(cplx<double>) * n);
+        dd::theta_to_u<double>(s, c->s->theta.as<double>(), u.as<cplx<double>>(), n);
+        c->s->set_gauge(u.as<cplx<double>>());
+        return DD_OK;
+    });
+}
+
+int dd_generate_gauge(dd_ctx* c, double beta, int sweeps, unsigned long long seed, double step, double* theta_out) {
+    if (check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c);
+        dd::metropolis_u1(s, c->s->theta.as<double>(), c->s->Nx, c->s->Nt, beta, sweeps, seed, step);
+        DevMem u(sizeof(cplx<double>) * n);
+        dd::theta_to_u<double>(s, c->s->theta.as<double>(), u.as<cplx<double>>(), n);
+        c->s->set_gauge(u.as<cplx<double>>());
+        if (theta_out) {
+            DD_HIP(hipMemcpyAsync(theta_out, c->s->theta.as<void>(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            DD_HIP(hipStreamSynchronize(s));
+        }
+        return DD_OK;
+    });
+}
+
+int dd_get_gauge(dd_ctx* c, double* U) {
+    if (check(c) || !U) return DD_ERR_ARG;
+    return guard([&] {
+        download(U, c->s->U64.as<cplx<double>>(), vsize(c), c->s->s);
+        return DD_OK;
+    });
+}
+
+int dd_apply_D_dev(dd_ctx* c, const double* in, double* out, int nrhs, double m0, int dagger, int precision) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_gauge) throw dd::BadShape("no gauge field");
+        c->s->apply_D((const cplx<double>*)in, (cplx<double>*)out, nrhs, m0, dagger != 0, precision);
+        DD_HIP(hipStreamSynchronize(c->s->s));
+        return DD_OK;
+    });
+}
+
+int dd_apply_D(dd_ctx* c, const double* in, double* out, int nrhs, double m0, int dagger, int precision) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_gauge) throw dd::BadShape("no gauge field");
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c) * nrhs;
+        Staged a(in, n, s), b(nullptr, n, s);
+        c->s->apply_D(a.p(), b.p(), nrhs, m0, dagger != 0, precision);
+        download(out, b.p(), n, s);
+        return DD_OK;
+    });
+}
+
+int dd_setup(dd_ctx* c, const dd_params* p) {
+    if (check(c) || !p) return DD_ERR_ARG;
+    return guard([&] {
+        c->s->setup(*p);
+        return DD_OK;
+    });
+}
+
+double dd_setup_seconds(dd_ctx* c) { return (c && c->s) ? c->s->setup_seconds : -1.0; }
+
+int dd_solve_dev(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
+    if (check(c) || !b || !x) return DD_ERR_ARG;
+    return guard([&] { return c->s->solve(nrhs, (const cplx<double>*)b, (cplx<double>*)x, iters, info); });
+}
+
+int dd_solve(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
+    if (check(c) || !b || !x || nrhs < 1) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c) * nrhs;
+        Staged bb(b, n, s), xx(nullptr, n, s);
+        const int st = c->s->solve(nrhs, bb.p(), xx.p(), iters, info);
+        download(x, xx.p(), n, s);
+        return st;
+    });
+}
+
+int dd_solve_adjoint_dev(dd_ctx* c, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info) {
+    if (check(c) || !eta || !y || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c);
+        DevMem t(sizeof(cplx<double>) * n * nrhs);
+        dd::sigma3<double>(s, t.as<cplx<double>>(), (const cplx<double>*)eta, n, n, nrhs, dd::full_mask(nrhs));
+        const int st = c->s->solve(nrhs, t.as<cplx<double>>(), (cplx<double>*)y, iters, info);
+        dd::sigma3<double>(s, (cplx<double>*)y, (const cplx<double>*)y, n, n, nrhs, dd::full_mask(nrhs));
+        DD_HIP(hipStreamSynchronize(s));
+        return st;
+    });
+}
+
+int dd_solve_adjoint(dd_ctx* c, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info) {
+    if (check(c) || !eta || !y || nrhs < 1) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c) * nrhs;
+        Staged e(eta, n, s), yy(nullptr, n, s);
+        const int st = dd_solve_adjoint_dev(c, nrhs, (const double*)e.p(), (double*)yy.p(), iters, info);
+        if (st != DD_OK && st != DD_ERR_NOT_CONVERGED) return st;
+        download(y, yy.p(), n, s);
+        return st;
+    });
+}
+
+int dd_dD_dtheta(dd_ctx* c, int nrhs, const double* y, const double* x, double* grad, double scale) {
+    if (check(c) || !y || !x || !grad || nrhs < 1) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c);
+        Staged yy(y, n * nrhs, s), xx(x, n * nrhs, s);
+        DevMem g(sizeof(double) * n);
+        DD_HIP(hipMemcpyAsync(g.as<void>(), grad, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        // the kernel's bracket omits the -1/2 of the hopping term
+        dd::dtheta_contract(s, c->s->U64.as<cplx<double>>(), yy.p(), xx.p(), g.as<double>(), c->s->Nx, c->s->Nt,
+                            nrhs, n, -0.5 * scale);
+        DD_HIP(hipMemcpyAsync(grad, g.as<void>(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        DD_HIP(hipStreamSynchronize(s));
+        return DD_OK;
+    });
+}
+
+int dd_level_shape(dd_ctx* c, int level, int* Nx, int* Nt, int* dof) {
+    if (check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
+        std::vector<dd::LevelGeom> g = dd::make_geometry(c->s->Nx, c->s->Nt, c->s->prm);
+        if (level < 0 || level >= (int)g.size()) return (int)DD_ERR_ARG;
+        if (Nx) *Nx = g[level].Nx;
+        if (Nt) *Nt = g[level].Nt;
+        if (dof) *dof = g[level].dof;
+        return (int)DD_OK;
+    });
+}
+
+int dd_level_apply(dd_ctx* c, int level, const double* in, double* out, int nrhs, int precision) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels()) throw dd::BadShape("bad level");
+            if (nrhs > m->cap(level)) throw dd::BadShape("nrhs above the level's batch capacity");
+            const size_t n = m->geom(level).n();
+            with_level_io<R>(c, in, out, n, n, nrhs, [&](const cplx<R>* a, cplx<R>* b) {
+                m->apply(level, a, b, nullptr, dd::full_mask(nrhs));
+            });
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_restrict(dd_ctx* c, int level, const double* in, double* out, int nrhs, int precision) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels() - 1) throw dd::BadShape("bad level");
+            with_level_io<R>(c, in, out, m->geom(level).n(), m->geom(level + 1).n(), nrhs,
+                             [&](const cplx<R>* a, cplx<R>* b) { m->restrict_(level, a, b, dd::full_mask(nrhs)); });
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_prolong(dd_ctx* c, int level, const double* in, double* out, int nrhs, int precision) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels() - 1) throw dd::BadShape("bad level");
+            with_level_io<R>(c, in, out, m->geom(level + 1).n(), m->geom(level).n(), nrhs,
+                             [&](const cplx<R>* a, cplx<R>* b) {
+                                 m->prolong(level, a, nullptr, b, dd::full_mask(nrhs));
+                             });
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_sap(dd_ctx* c, int level, const double* v, double* x, int nu, double tol, int nrhs, int precision) {
+    if (check(c) || !v || !x || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels() - 1) throw dd::BadShape("no SAP on the coarsest level");
+            if (nrhs > m->cap(level)) throw dd::BadShape("nrhs above the level's batch capacity");
+            const size_t n = m->geom(level).n();
+            with_level_io<R>(
+                c, v, x, n, n, nrhs,
+                [&](const cplx<R>* a, cplx<R>* b) { m->sap(level, a, b, nu, tol, dd::full_mask(nrhs)); }, x);
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_get_test_vectors(dd_ctx* c, int level, double* out, int precision) {
+    if (check(c) || !out) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels() - 1) throw dd::BadShape("bad level");
+            const dd::LevelGeom& G = m->geom(level);
+            hipStream_t s = c->s->s;
+            const size_t tot = (size_t)G.V() * G.nv * G.dof;
+            DevMem t(sizeof(cplx<R>) * tot), o(sizeof(cplx<double>) * tot);
+            dd::transpose_tv<R>(s, m->W(level), t.as<cplx<R>>(), G.V(), G.nv, G.dof, false);
+            dd::convert<double, R>(s, o.as<cplx<double>>(), t.as<cplx<R>>(), tot, 0, 0, 1, 1);
+            download(out, o.as<cplx<double>>(), tot, s);
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_get_links(dd_ctx* c, int level, double* out, int precision) {
+    if (check(c) || !out) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 1 || level >= m->levels()) throw dd::BadShape("links exist on levels >= 1");
+            const dd::LevelGeom& G = m->geom(level);
+            hipStream_t s = c->s->s;
+            const size_t tot = (size_t)G.V() * 5 * G.dof * G.dof;
+            DevMem o(sizeof(cplx<double>) * tot);
+            dd::convert<double, R>(s, o.as<cplx<double>>(), m->links(level), tot, 0, 0, 1, 1);
+            std::vector<double> h(2 * tot);
+            download(h.data(), o.as<cplx<double>>(), tot, s);
+            const int d = G.dof;
+            for (size_t blk = 0; blk < (size_t)G.V() * 5; blk++)
+                for (int row = 0; row < d; row++)
+                    for (int col = 0; col < d; col++) {
+                        const size_t src = (blk * d + col) * d + row, dst = (blk * d + row) * d + col;
+                        out[2 * dst] = h[2 * src];
+                        out[2 * dst + 1] = h[2 * src + 1];
+                    }
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+}  // extern "C"

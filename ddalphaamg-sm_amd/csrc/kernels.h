@@ -1,0 +1,127 @@
+// Launchers of the gfx950 kernels (kernels.hip).  All launches go to the given stream
+// and check the launch status; shapes are validated by the callers (solver.hip).
+#pragma once
+#include "common.h"
+
+namespace dd {
+
+// Maximum block-GMRES restart length compiled into the SAP block kernels.
+constexpr int kSapM = 5;
+
+struct RedBuf {            // scratch for two-pass reductions
+    void* partial = nullptr;   // [nrhs][nblk] complex<double>
+    int nblk = 0;
+};
+
+// ---- level-0 Wilson-Dirac ---------------------------------------------------------
+// out = D in  (b == nullptr)   or   out = b - D in.   dagger selects D^dagger.
+template <typename R>
+void wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, const cplx<R>* b,
+            int Nx, int Nt, double m0, bool dagger, int nrhs, uint64_t mask, size_t ld);
+
+// Level-0 link matrices in the generic coarse-link layout [V][5][2][2] (col-major).
+template <typename R>
+void dirac_links(hipStream_t s, const cplx<R>* U, cplx<R>* L, int Nx, int Nt);
+
+// ---- generic level stencil ----------------------------------------------------------
+template <typename R>
+void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* out, const cplx<R>* b,
+                  int Nx, int Nt, int dof, double m0, int nrhs, uint64_t mask, size_t ld);
+
+// ---- SAP block solves (one colour, x += D_B^-1 r on the blocks of that colour) -------
+template <typename R>
+void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
+                 int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
+                 const int* gate);
+template <typename R>
+void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
+                 int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask,
+                 size_t ld, const int* gate);
+
+// ---- transfers ----------------------------------------------------------------------
+// out(coarse) = P^dagger v (fine);  W: [V][nv][dof]
+template <typename R>
+void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, int Nx, int Nt, int dof,
+               int ax, int at, int nv, int nrhs, uint64_t mask, size_t ld_f, size_t ld_c);
+// out(fine) = y + P vc;  y may alias out, y == nullptr means 0
+template <typename R>
+void prolong(hipStream_t s, const cplx<R>* W, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, int Nx, int Nt,
+             int dof, int ax, int at, int nv, int nrhs, uint64_t mask, size_t ld_f, size_t ld_c);
+// block-local modified Gram-Schmidt of W [V][nv][dof] per (aggregate, spin)
+template <typename R>
+void orthonormalize(hipStream_t s, cplx<R>* W, int Nx, int Nt, int dof, int ax, int at, int nv);
+// Galerkin links of the next level from links Lf of this level and test vectors W
+template <typename R>
+void galerkin(hipStream_t s, const cplx<R>* Lf, const cplx<R>* W, cplx<R>* Lc, int Nx, int Nt, int dof, int ax,
+              int at, int nv, bool has_A);
+// [nv][V][dof] <-> [V][nv][dof]
+template <typename R>
+void transpose_tv(hipStream_t s, const cplx<R>* in, cplx<R>* out, int V, int nv, int dof, bool to_site_major);
+
+// ---- BLAS-1 -----------------------------------------------------------------------------
+// partial sums of a . conj(b) per system into rb.partial; returns nothing
+template <typename R>
+void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, size_t ld, int nrhs, uint64_t mask,
+                 RedBuf& rb);
+// out[r*ostride] = sum of partials (complex, precision R)
+template <typename R>
+void dot_final(hipStream_t s, const RedBuf& rb, cplx<double>* out, int ostride, int nrhs, uint64_t mask);
+// h = sum partials -> H[r*hstride]; w -= h * v
+template <typename R>
+void mgs_update(hipStream_t s, cplx<R>* w, const cplx<R>* v, size_t n, size_t ld, int nrhs, uint64_t mask,
+                const RedBuf& rb, cplx<double>* H, int hstride);
+// h = sqrt(sum partials.re) -> H[r*hstride]; if h > 0: out = (1/h) w
+template <typename R>
+void norm_scale(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
+                const RedBuf& rb, cplx<double>* H, int hstride);
+// out = scale[r] * w, scale from device (1/beta computed in-kernel from beta[r*bstride].x)
+template <typename R>
+void scale_inv(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
+               const cplx<double>* beta, int bstride);
+// out = x + alpha*y, alpha host scalar (real)
+template <typename R>
+void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, double alpha, size_t n, size_t ld,
+          int nrhs, uint64_t mask);
+// x += sum_{j < nit[r]} eta[r*m + j] * Z_j   (Z_j = Z + j*zstride)
+template <typename R>
+void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const cplx<double>* eta,
+              const int* nit, int m, size_t n, size_t ld, int nrhs, uint64_t mask);
+template <typename R>
+void copy(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld_out, size_t ld_in, int nrhs,
+          uint64_t mask);
+template <typename R>
+void zero(hipStream_t s, cplx<R>* out, size_t n, size_t ld, int nrhs, uint64_t mask);
+template <typename To, typename From>
+void convert(hipStream_t s, cplx<To>* out, const cplx<From>* in, size_t n, size_t ld_out, size_t ld_in, int nrhs,
+             uint64_t mask);
+// multiply the spin-1 components by -1 (sigma_3 on dof = colour*2 + spin)
+template <typename R>
+void sigma3(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, int nrhs, uint64_t mask);
+// gate[r] = 0 where sqrt(err2[r*stride].x) < tol * sqrt(v2[r*stride].x)
+void gate_update(hipStream_t s, int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
+                 int nrhs, uint64_t mask);
+
+// ---- gauge field ------------------------------------------------------------------------
+void metropolis_u1(hipStream_t s, double* theta, int Nx, int Nt, double beta, int sweeps, uint64_t seed,
+                   double step);
+template <typename R>
+void theta_to_u(hipStream_t s, const double* theta, cplx<R>* U, size_t n);
+
+// ---- derivative of the stencil --------------------------------------------------------
+// grad[x][mu] += sign * Re( y(x)^H dD/dtheta_mu(x) x )  summed over systems, see kernels.hip
+void dtheta_contract(hipStream_t s, const cplx<double>* U, const cplx<double>* y, const cplx<double>* x,
+                     double* grad, int Nx, int Nt, int nrhs, size_t ld, double scale);
+
+}  // namespace dd
+
+namespace dd {
+// Classical Gram-Schmidt pass against V_0..V_{nvec-1} (V_i = V + i*vstride):
+// partials of <V_i, w> per block, then h_i = sum, w -= sum_i h_i V_i and
+// H[r*hstride + i] (=|+=) h_i.
+template <typename R>
+void multidot(hipStream_t s, const cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld,
+              int nrhs, uint64_t mask, RedBuf& rb);
+template <typename R>
+void multi_axpy(hipStream_t s, cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld, int nrhs,
+                uint64_t mask, const RedBuf& rb, cplx<double>* H, int hstride, bool accumulate);
+}  // namespace dd

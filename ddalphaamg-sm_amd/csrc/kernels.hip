@@ -1,0 +1,1232 @@
+// gfx950 kernels of the DD-alphaAMG solve path.  See common.h for the layouts.
+//
+// Reference behaviour restated here (paths relative to the reference root):
+//   k_wilson        D_phi / D_dagger_phi          src/dirac_operator.cpp:24-85
+//   k_dirac_links   Level::makeDirac              src/level.cpp:286-331
+//   k_coarse        Level::D_operator             src/level.cpp:394-426
+//   k_sap0/k_sapc   SAP_C::I_D_B_1_It + the block GMRES (FGMRES::fgmres with
+//                   SAP_C::GMRES_D_B hooks) and SAP_level_l::D_local
+//                   src/sap.cpp:48-77, src/fgmres.cpp:5-127, include/sap.h:14-83,
+//                   src/level.cpp:607-653
+//   k_restrict      Level::Pdagg_v                src/level.cpp:145-189
+//   k_prolong       Level::P_vc                   src/level.cpp:92-141
+//   k_ortho         Level::orthonormalize         src/level.cpp:193-284
+//   k_galerkin      Level::makeCoarseLinks        src/level.cpp:431-542
+#include "kernels.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace dd {
+
+#define DD_LAUNCH_CHECK() DD_HIP(hipGetLastError())
+
+static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ======================================================================================
+// Level-0 Wilson-Dirac stencil.  One thread per site and system.
+// ======================================================================================
+template <typename R, bool DAG, bool RES>
+__global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
+                                                cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
+                                                int Nt, R m0, uint64_t mask, size_t ld) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Nx * Nt) return;
+    const int x = n / Nt, t = n - x * Nt;
+    const cplx<R>* v = in + r * ld;
+    const int ntp = (t == Nt - 1) ? n - (Nt - 1) : n + 1;
+    const int ntm = (t == 0) ? n + (Nt - 1) : n - 1;
+    const int nxp = (x == Nx - 1) ? t : n + Nt;
+    const int nxm = (x == 0) ? (Nx - 1) * Nt + t : n - Nt;
+    const R sp = (t == Nt - 1) ? R(-1) : R(1);
+    const R sm = (t == 0) ? R(-1) : R(1);
+    const cplx<R> U0 = U[2 * n], U1 = U[2 * n + 1];
+    const cplx<R> U0m = conj(U[2 * ntm]), U1m = conj(U[2 * nxm + 1]);
+    const cplx<R> p0 = v[2 * n], p1 = v[2 * n + 1];
+    const cplx<R> tp0 = v[2 * ntp], tp1 = v[2 * ntp + 1];
+    const cplx<R> xp0 = v[2 * nxp], xp1 = v[2 * nxp + 1];
+    const cplx<R> tm0 = v[2 * ntm], tm1 = v[2 * ntm + 1];
+    const cplx<R> xm0 = v[2 * nxm], xm1 = v[2 * nxm + 1];
+    cplx<R> a0, a1;
+    if (!DAG) {
+        a0 = U0 * (sp * (tp0 - tp1)) + U1 * (xp0 + times_i(xp1)) + U0m * (sm * (tm0 + tm1)) +
+             U1m * (xm0 - times_i(xm1));
+        a1 = U0 * (sp * (tp1 - tp0)) + U1 * (xp1 - times_i(xp0)) + U0m * (sm * (tm0 + tm1)) +
+             U1m * (times_i(xm0) + xm1);
+    } else {
+        a0 = U0m * (sm * (tm0 - tm1)) + U1m * (xm0 + times_i(xm1)) + U0 * (sp * (tp0 + tp1)) +
+             U1 * (xp0 - times_i(xp1));
+        a1 = U0m * (sm * (tm1 - tm0)) + U1m * (xm1 - times_i(xm0)) + U0 * (sp * (tp0 + tp1)) +
+             U1 * (times_i(xp0) + xp1);
+    }
+    const R mass = m0 + R(2);
+    cplx<R> o0 = mass * p0 - R(0.5) * a0;
+    cplx<R> o1 = mass * p1 - R(0.5) * a1;
+    cplx<R>* o = out + r * ld;
+    if (RES) {
+        const cplx<R>* bb = b + r * ld;
+        o0 = bb[2 * n] - o0;
+        o1 = bb[2 * n + 1] - o1;
+    }
+    o[2 * n] = o0;
+    o[2 * n + 1] = o1;
+}
+
+template <typename R>
+void wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx, int Nt,
+            double m0, bool dagger, int nrhs, uint64_t mask, size_t ld) {
+    dim3 grid(cdivu((size_t)Nx * Nt, 256), nrhs);
+    if (dagger) {
+        if (b) k_wilson<R, true, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+        else k_wilson<R, true, false><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+    } else {
+        if (b) k_wilson<R, false, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+        else k_wilson<R, false, false><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+    }
+    DD_LAUNCH_CHECK();
+}
+
+// Level-0 links as generic stencil matrices (A = 0):
+//   B_mu = 1/2 (1 - sigma_mu) U_mu(x),   C_mu = 1/2 (1 + sigma_mu) U_mu(x - mu)^*
+template <typename R>
+__global__ void k_dirac_links(const cplx<R>* __restrict__ U, cplx<R>* __restrict__ L, int Nx, int Nt) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Nx * Nt) return;
+    const int x = n / Nt, t = n - x * Nt;
+    const int ntm = (t == 0) ? n + (Nt - 1) : n - 1;
+    const int nxm = (x == 0) ? (Nx - 1) * Nt + t : n - Nt;
+    const cplx<R> u[2] = {U[2 * n], U[2 * n + 1]};
+    const cplx<R> um[2] = {conj(U[2 * ntm]), conj(U[2 * nxm + 1])};
+    const cplx<R> one = mk<R>(1, 0), I = mk<R>(0, 1);
+    // (1 - sigma_mu)[row][col] and (1 + sigma_mu)[row][col]
+    const cplx<R> Mm[2][2][2] = {{{one, -one}, {-one, one}}, {{one, I}, {-I, one}}};
+    const cplx<R> Pm[2][2][2] = {{{one, one}, {one, one}}, {{one, -I}, {I, one}}};
+    cplx<R>* o = L + (size_t)n * 5 * 4;
+    for (int e = 0; e < 4; e++) o[e] = czero<R>();
+    for (int mu = 0; mu < 2; mu++)
+        for (int col = 0; col < 2; col++)
+            for (int row = 0; row < 2; row++) {
+                o[(1 + mu) * 4 + col * 2 + row] = R(0.5) * (Mm[mu][row][col] * u[mu]);
+                o[(3 + mu) * 4 + col * 2 + row] = R(0.5) * (Pm[mu][row][col] * um[mu]);
+            }
+}
+
+template <typename R>
+void dirac_links(hipStream_t s, const cplx<R>* U, cplx<R>* L, int Nx, int Nt) {
+    k_dirac_links<R><<<cdivu((size_t)Nx * Nt, 256), 256, 0, s>>>(U, L, Nx, Nt);
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// Generic level stencil  out = (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu)).
+// Work-group of SPB sites x DOF rows; one thread per (site, row).  The five input
+// spinors of each site are staged in LDS, the links stream from HBM (coalesced on
+// rows thanks to the column-major matrix layout).
+// ======================================================================================
+template <typename R, int DOF, bool RES>
+__global__ void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                         const cplx<R>* __restrict__ b, int Nx, int Nt, R m0, uint64_t mask, size_t ld, int spb) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* sv = reinterpret_cast<cplx<R>*>(smem_raw);
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const int tid = threadIdx.x;
+    const int sl = tid / DOF, row = tid - sl * DOF;
+    const int V = Nx * Nt;
+    const int n = blockIdx.x * spb + sl;
+    const bool ok = n < V;
+    const cplx<R>* v = in + r * ld;
+    cplx<R> self = czero<R>();
+    if (ok) {
+        const int x = n / Nt, t = n - x * Nt;
+        const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
+                           (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
+        const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+#pragma unroll
+        for (int k = 0; k < 5; k++) sv[(sl * 5 + k) * DOF + row] = sg[k] * v[(size_t)nb[k] * DOF + row];
+        self = v[(size_t)n * DOF + row];
+    }
+    __syncthreads();
+    if (!ok) return;
+    cplx<R> acc = czero<R>();
+    const cplx<R>* Ln = L + (size_t)n * 5 * DOF * DOF + row;
+    const cplx<R>* s0 = sv + sl * 5 * DOF;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+#pragma unroll 8
+        for (int col = 0; col < DOF; col++) acc = acc + Ln[(k * DOF + col) * DOF] * s0[k * DOF + col];
+    }
+    cplx<R> o = (m0 + R(2)) * self - acc;
+    if (RES) o = b[r * ld + (size_t)n * DOF + row] - o;
+    out[r * ld + (size_t)n * DOF + row] = o;
+}
+
+static int lcm64(int dof) {
+    int a = dof, b = 64;
+    while (b) {
+        int t = a % b;
+        a = b;
+        b = t;
+    }
+    return dof / a * 64;
+}
+
+#define DD_DOF_LIST(X) X(4) X(8) X(12) X(16) X(20) X(24) X(32)
+
+template <typename R>
+void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx,
+                  int Nt, int dof, double m0, int nrhs, uint64_t mask, size_t ld) {
+    const int threads = lcm64(dof);
+    const int spb = threads / dof;
+    dim3 grid(cdivu((size_t)Nx * Nt, spb), nrhs);
+    const size_t sh = sizeof(cplx<R>) * spb * 5 * dof;
+    switch (dof) {
+#define DD_CASE(D)                                                                                        \
+    case D:                                                                                               \
+        if (b) k_coarse<R, D, true><<<grid, threads, sh, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, spb); \
+        else k_coarse<R, D, false><<<grid, threads, sh, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, spb); \
+        break;
+        DD_DOF_LIST(DD_CASE)
+#undef DD_CASE
+        default:
+            throw BadShape("coarse_apply: unsupported dof");
+    }
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// Block GMRES of the SAP smoother: FGMRES::fgmres with the identity preconditioner,
+// initial guess = block right-hand side, restart length M, the break on
+// |g_{j+1}| < tol ||phi||, maxIt kept across restarts and the true-residual test after
+// each cycle (src/fgmres.cpp:5-97, src/sap.cpp:65).  Every thread of a block holds one
+// element of each Krylov vector and the (uniform) Hessenberg scalars.
+// ======================================================================================
+template <int M>
+struct Tri {
+    static constexpr int at(int i, int j) { return j * (j + 1) / 2 + i; }   // i <= j
+    static constexpr int size = M * (M + 1) / 2;
+};
+
+template <typename R, int M, typename E, typename Op, typename Red>
+__device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int restarts, R tol) {
+    E x = phi;
+    E r = phi - A(x);
+    const R nphi = sqrt(red(phi.norm2()).x);
+    R err = sqrt(red(r.norm2()).x);
+    int maxIt = M;
+    E V[M + 1];
+#pragma unroll
+    for (int i = 0; i <= M; i++) V[i] = E::zero();
+    cplx<R> H[Tri<M>::size];
+    cplx<R> g[M + 1], cn[M], sn[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        cn[i] = czero<R>();
+        sn[i] = czero<R>();
+    }
+    for (int k = 0; k < restarts; k++) {
+        V[0] = (R(1) / err) * r;
+        g[0] = mk<R>(err, 0);
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            E w = A(V[j]);
+#pragma unroll
+            for (int i = 0; i <= j; i++) {
+                const cplx<R> h = red(w.dotc(V[i]));
+                H[Tri<M>::at(i, j)] = h;
+                w = w - h * V[i];
+            }
+            const R hn = sqrt(red(w.norm2()).x);
+            if (j + 1 <= M && hn > R(0)) V[j + 1] = (R(1) / hn) * w;
+            // Givens rotations of column j (src/fgmres.cpp:99-115)
+#pragma unroll
+            for (int i = 0; i < j; i++) {
+                const cplx<R> a = H[Tri<M>::at(i, j)], bb = H[Tri<M>::at(i + 1, j)];
+                H[Tri<M>::at(i, j)] = cmul(cn[i], a) + cmul(sn[i], bb);
+                H[Tri<M>::at(i + 1, j)] = cn[i] * bb - sn[i] * a;
+            }
+            const cplx<R> hjj = H[Tri<M>::at(j, j)];
+            const cplx<R> hs = mk<R>(hn, 0);
+            const R den = sqrt(norm2(hjj) + norm2(hs));
+            sn[j] = mk<R>(hs.x / den, hs.y / den);
+            cn[j] = mk<R>(hjj.x / den, hjj.y / den);
+            H[Tri<M>::at(j, j)] = cmul(cn[j], hjj) + cmul(sn[j], hs);
+            g[j + 1] = -(sn[j] * g[j]);
+            g[j] = cmul(cn[j], g[j]);
+            if (sqrt(norm2(g[j + 1])) < tol * nphi) {
+                maxIt = j + 1;
+                break;
+            }
+        }
+        // back substitution (src/fgmres.cpp:118-127) and x += sum eta_j Z_j
+        cplx<R> eta[M];
+#pragma unroll
+        for (int i = M - 1; i >= 0; i--) {
+            if (i < maxIt) {
+                cplx<R> e = g[i];
+#pragma unroll
+                for (int jj = i + 1; jj < M; jj++)
+                    if (jj < maxIt) e = e - H[Tri<M>::at(i, jj)] * eta[jj];
+                eta[i] = cdiv(e, H[Tri<M>::at(i, i)]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            if (j < maxIt) x = x + eta[j] * V[j];
+        r = phi - A(x);
+        err = sqrt(red(r.norm2()).x);
+        if (err < tol * nphi) break;
+    }
+    return x;
+}
+
+// Colour-c block number k -> block coordinates (red iff BX + BT even).
+__device__ __forceinline__ void colour_block(int k, int colour, int nbx, int nbt, int& BX, int& BT) {
+    if ((nbt & 1) == 0) {
+        const int h = nbt >> 1;
+        BX = k / h;
+        BT = 2 * (k - BX * h) + ((BX + colour) & 1);
+    } else {
+        const int bidx = 2 * k + colour;
+        BX = bidx / nbt;
+        BT = bidx - BX * nbt;
+    }
+}
+
+// ---- level 0: one site (two spin components) per lane; S = bx*bt lanes per block,
+// 64/S blocks per wave; neighbours through cross-lane permutes, sums through
+// butterflies inside the S-lane segment.  No LDS and no barriers.
+template <typename R>
+struct Spin2 {
+    cplx<R> a, b;
+    __device__ static Spin2 zero() { return {czero<R>(), czero<R>()}; }
+    __device__ Spin2 operator+(const Spin2& o) const { return {a + o.a, b + o.b}; }
+    __device__ Spin2 operator-(const Spin2& o) const { return {a - o.a, b - o.b}; }
+    __device__ cplx<R> dotc(const Spin2& o) const { return mulc(a, o.a) + mulc(b, o.b); }
+    __device__ cplx<R> norm2() const { return mk<R>(dd::norm2(a) + dd::norm2(b), R(0)); }
+};
+template <typename R>
+__device__ __forceinline__ Spin2<R> operator*(cplx<R> s, const Spin2<R>& v) { return {s * v.a, s * v.b}; }
+template <typename R>
+__device__ __forceinline__ Spin2<R> operator*(R s, const Spin2<R>& v) { return {s * v.a, s * v.b}; }
+
+template <typename R>
+__device__ __forceinline__ cplx<R> shfl_c(cplx<R> v, int src) {
+    return mk<R>(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+template <typename R, int M>
+__global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ rr,
+                                             cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt, int colour,
+                                             R m0, int restarts, R tol, uint64_t mask, size_t ld,
+                                             const int* __restrict__ gate, int nblk_colour) {
+    const int rh = blockIdx.y;
+    if (!rhs_on(mask, rh)) return;
+    if (gate && gate[rh] == 0) return;
+    const int S = bx * bt;
+    const int lane = threadIdx.x;
+    const int seg = lane / S, k = lane - seg * S;
+    const int blk = blockIdx.x * (64 / S) + seg;
+    if (blk >= nblk_colour) return;
+    const int nbx = Nx / bx, nbt = Nt / bt;
+    int BX, BT;
+    colour_block(blk, colour, nbx, nbt, BX, BT);
+    const int lx = k / bt, lt = k - lx * bt;
+    const int X = BX * bx + lx, T = BT * bt + lt;
+    const int n = X * Nt + T;
+    const int base = seg * S;
+    const bool hasTp = lt < bt - 1, hasTm = lt > 0, hasXp = lx < bx - 1, hasXm = lx > 0;
+    const int sTp = hasTp ? lane + 1 : lane, sTm = hasTm ? lane - 1 : lane;
+    const int sXp = hasXp ? lane + bt : lane, sXm = hasXm ? lane - bt : lane;
+    const cplx<R> U0 = U[2 * n], U1 = U[2 * n + 1];
+    const cplx<R> U0m = hasTm ? conj(U[2 * (n - 1)]) : czero<R>();
+    const cplx<R> U1m = hasXm ? conj(U[2 * (n - Nt) + 1]) : czero<R>();
+    const R mass = m0 + R(2);
+    (void)base;
+    // D_B restricted to the block, zero outside, no boundary sign
+    auto A = [&](const Spin2<R>& v) -> Spin2<R> {
+        cplx<R> tp0 = shfl_c(v.a, sTp), tp1 = shfl_c(v.b, sTp);
+        cplx<R> xp0 = shfl_c(v.a, sXp), xp1 = shfl_c(v.b, sXp);
+        cplx<R> tm0 = shfl_c(v.a, sTm), tm1 = shfl_c(v.b, sTm);
+        cplx<R> xm0 = shfl_c(v.a, sXm), xm1 = shfl_c(v.b, sXm);
+        if (!hasTp) tp0 = tp1 = czero<R>();
+        if (!hasXp) xp0 = xp1 = czero<R>();
+        if (!hasTm) tm0 = tm1 = czero<R>();
+        if (!hasXm) xm0 = xm1 = czero<R>();
+        const cplx<R> a0 =
+            U0 * (tp0 - tp1) + U1 * (xp0 + times_i(xp1)) + U0m * (tm0 + tm1) + U1m * (xm0 - times_i(xm1));
+        const cplx<R> a1 =
+            U0 * (tp1 - tp0) + U1 * (xp1 - times_i(xp0)) + U0m * (tm0 + tm1) + U1m * (times_i(xm0) + xm1);
+        return {mass * v.a - R(0.5) * a0, mass * v.b - R(0.5) * a1};
+    };
+    auto red = [&](cplx<R> v) -> cplx<R> { return wave_sum(v, S); };
+    const cplx<R>* rv = rr + rh * ld;
+    const Spin2<R> phi = {rv[2 * n], rv[2 * n + 1]};
+    const Spin2<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
+    cplx<R>* xv = xx + rh * ld;
+    xv[2 * n] = xv[2 * n] + e.a;
+    xv[2 * n + 1] = xv[2 * n + 1] + e.b;
+}
+
+template <typename R>
+void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
+                 int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
+                 const int* gate) {
+    const int S = bx * bt;
+    if (S > 64 || (S & (S - 1)) != 0) throw BadShape("level-0 SAP block must have a power-of-two site count <= 64");
+    if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
+    const int nblk = (Nx / bx) * (Nt / bt) / 2;
+    dim3 grid(cdivu(nblk, 64 / S), nrhs);
+    k_sap0<R, kSapM><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld, gate,
+                                         nblk);
+    DD_LAUNCH_CHECK();
+}
+
+// ---- coarse levels: one work-group per block, one thread per (site, row)
+template <typename R>
+struct Elem1 {
+    cplx<R> a;
+    __device__ static Elem1 zero() { return {czero<R>()}; }
+    __device__ Elem1 operator+(const Elem1& o) const { return {a + o.a}; }
+    __device__ Elem1 operator-(const Elem1& o) const { return {a - o.a}; }
+    __device__ cplx<R> dotc(const Elem1& o) const { return mulc(a, o.a); }
+    __device__ cplx<R> norm2() const { return mk<R>(dd::norm2(a), R(0)); }
+};
+template <typename R>
+__device__ __forceinline__ Elem1<R> operator*(cplx<R> s, const Elem1<R>& v) { return {s * v.a}; }
+template <typename R>
+__device__ __forceinline__ Elem1<R> operator*(R s, const Elem1<R>& v) { return {s * v.a}; }
+
+template <typename R, int DOF, int M>
+__global__ __launch_bounds__(512) void k_sapc(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ rr,
+                                               cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
+                                               int colour, R m0, int restarts, R tol, uint64_t mask, size_t ld,
+                                               const int* __restrict__ gate) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* sv = reinterpret_cast<cplx<R>*>(smem_raw);
+    R* scratch = reinterpret_cast<R*>(sv + bx * bt * DOF);
+    const int rh = blockIdx.y;
+    if (!rhs_on(mask, rh)) return;
+    if (gate && gate[rh] == 0) return;
+    const int S = bx * bt;
+    const int tid = threadIdx.x;
+    const bool act = tid < S * DOF;
+    const int k = act ? tid / DOF : 0, row = act ? tid - (tid / DOF) * DOF : 0;
+    const int nbx = Nx / bx, nbt = Nt / bt;
+    int BX, BT;
+    colour_block(blockIdx.x, colour, nbx, nbt, BX, BT);
+    const int lx = k / bt, lt = k - lx * bt;
+    const int n = (BX * bx + lx) * Nt + BT * bt + lt;
+    const bool hasTp = lt < bt - 1, hasTm = lt > 0, hasXp = lx < bx - 1, hasXm = lx > 0;
+    const cplx<R>* Ln = L + (size_t)n * 5 * DOF * DOF + row;
+    const R mass = m0 + R(2);
+    auto A = [&](const Elem1<R>& v) -> Elem1<R> {
+        __syncthreads();
+        if (act) sv[tid] = v.a;
+        __syncthreads();
+        if (!act) return Elem1<R>::zero();
+        cplx<R> acc = czero<R>();
+        const cplx<R>* s0 = sv + k * DOF;
+#pragma unroll 8
+        for (int col = 0; col < DOF; col++) acc = acc + Ln[col * DOF] * s0[col];
+        if (hasTp) {
+            const cplx<R>* s1 = sv + (k + 1) * DOF;
+#pragma unroll 8
+            for (int col = 0; col < DOF; col++) acc = acc + Ln[(1 * DOF + col) * DOF] * s1[col];
+        }
+        if (hasXp) {
+            const cplx<R>* s1 = sv + (k + bt) * DOF;
+#pragma unroll 8
+            for (int col = 0; col < DOF; col++) acc = acc + Ln[(2 * DOF + col) * DOF] * s1[col];
+        }
+        if (hasTm) {
+            const cplx<R>* s1 = sv + (k - 1) * DOF;
+#pragma unroll 8
+            for (int col = 0; col < DOF; col++) acc = acc + Ln[(3 * DOF + col) * DOF] * s1[col];
+        }
+        if (hasXm) {
+            const cplx<R>* s1 = sv + (k - bt) * DOF;
+#pragma unroll 8
+            for (int col = 0; col < DOF; col++) acc = acc + Ln[(4 * DOF + col) * DOF] * s1[col];
+        }
+        return {mass * v.a - acc};
+    };
+    auto red = [&](cplx<R> v) -> cplx<R> { return block_sum(v, scratch); };
+    const cplx<R>* rv = rr + rh * ld;
+    Elem1<R> phi = {act ? rv[(size_t)n * DOF + row] : czero<R>()};
+    const Elem1<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
+    if (act) {
+        cplx<R>* xv = xx + rh * ld + (size_t)n * DOF + row;
+        *xv = *xv + e.a;
+    }
+}
+
+template <typename R>
+void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
+                 int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
+                 const int* gate) {
+    const int S = bx * bt;
+    if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
+    const int threads = ((S * dof + 63) / 64) * 64;
+    if (threads > 512) throw BadShape("coarse SAP block too large (sites*dof > 512)");
+    const int nblk = (Nx / bx) * (Nt / bt) / 2;
+    dim3 grid(nblk, nrhs);
+    const size_t sh = sizeof(cplx<R>) * S * dof + sizeof(R) * 32;
+    switch (dof) {
+#define DD_CASE(D)                                                                                               \
+    case D:                                                                                                      \
+        k_sapc<R, D, kSapM><<<grid, threads, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, \
+                                                     mask, ld, gate);                                           \
+        break;
+        DD_DOF_LIST(DD_CASE)
+#undef DD_CASE
+        default:
+            throw BadShape("sap_blocksc: unsupported dof");
+    }
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// Transfers
+// ======================================================================================
+template <typename R>
+__global__ void k_restrict(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
+                           int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf, size_t ldc) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const int Ntc = Nt / at, Vc = (Nx / ax) * Ntc;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Vc * 2 * nv) return;
+    const int Xc = idx / (2 * nv), q = idx - Xc * 2 * nv, p = q >> 1, sp = q & 1;
+    const int X = Xc / Ntc, Tc = Xc - X * Ntc;
+    const int col = dof / 2;
+    const cplx<R>* vr = v + r * ldf;
+    cplx<R> sum = czero<R>();
+    for (int lx = 0; lx < ax; lx++)
+        for (int lt = 0; lt < at; lt++) {
+            const size_t n = (size_t)(X * ax + lx) * Nt + Tc * at + lt;
+            for (int c = 0; c < col; c++) {
+                const int d = c * 2 + sp;
+                sum = sum + cmul(W[(n * nv + p) * dof + d], vr[n * dof + d]);
+            }
+        }
+    out[r * ldc + (size_t)Xc * 2 * nv + q] = sum;
+}
+
+template <typename R>
+void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, int Nx, int Nt, int dof, int ax,
+               int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
+    const size_t tot = (size_t)(Nx / ax) * (Nt / at) * 2 * nv;
+    dim3 grid(cdivu(tot, 256), nrhs);
+    k_restrict<R><<<grid, 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ void k_prolong(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ vc, const cplx<R>* y,
+                          cplx<R>* out, int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf,
+                          size_t ldc) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)Nx * Nt * dof) return;
+    const int n = (int)(idx / dof), d = (int)(idx - (size_t)n * dof), sp = d & 1;
+    const int x = n / Nt, t = n - x * Nt;
+    const int Xc = (x / ax) * (Nt / at) + t / at;
+    const cplx<R>* vcr = vc + r * ldc + (size_t)Xc * 2 * nv + sp;
+    const cplx<R>* Wn = W + (size_t)n * nv * dof + d;
+    cplx<R> sum = czero<R>();
+    for (int p = 0; p < nv; p++) sum = sum + Wn[p * dof] * vcr[2 * p];
+    if (y) sum = y[r * ldf + idx] + sum;
+    out[r * ldf + idx] = sum;
+}
+
+template <typename R>
+void prolong(hipStream_t s, const cplx<R>* W, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, int Nx, int Nt,
+             int dof, int ax, int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
+    dim3 grid(cdivu((size_t)Nx * Nt * dof, 256), nrhs);
+    k_prolong<R><<<grid, 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    DD_LAUNCH_CHECK();
+}
+
+// Modified Gram-Schmidt per (aggregate, spin); one work-group, one thread per
+// (site, colour) entry; the aggregate's vectors live in LDS.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_ortho(cplx<R>* __restrict__ W, int Nx, int Nt, int dof, int ax, int at,
+                                                int nv) {
+    extern __shared__ unsigned char smem_raw[];
+    const int col = dof / 2, E = ax * at * col;
+    cplx<R>* w = reinterpret_cast<cplx<R>*>(smem_raw);   // [nv][E]
+    R* scratch = reinterpret_cast<R*>(w + nv * E);
+    const int Ntc = Nt / at;
+    const int agg = blockIdx.x >> 1, sp = blockIdx.x & 1;
+    const int X = agg / Ntc, Tc = agg - X * Ntc;
+    const int e = threadIdx.x;
+    const bool act = e < E;
+    size_t gidx = 0;
+    if (act) {
+        const int site = e / col, c = e - site * col;
+        const int lx = site / at, lt = site - lx * at;
+        const size_t n = (size_t)(X * ax + lx) * Nt + Tc * at + lt;
+        gidx = n * nv * dof + c * 2 + sp;
+        for (int p = 0; p < nv; p++) w[p * E + e] = W[gidx + (size_t)p * dof];
+    }
+    __syncthreads();
+    for (int nt = 0; nt < nv; nt++) {
+        for (int ntt = 0; ntt < nt; ntt++) {
+            const cplx<R> part = act ? mulc(w[nt * E + e], w[ntt * E + e]) : czero<R>();
+            const cplx<R> proj = block_sum(part, scratch);
+            if (act) w[nt * E + e] = w[nt * E + e] - proj * w[ntt * E + e];
+        }
+        const cplx<R> part = act ? mk<R>(norm2(w[nt * E + e]), R(0)) : czero<R>();
+        const R nrm = sqrt(block_sum(part, scratch).x);
+        if (act && nrm > R(1e-8)) {
+            const cplx<R> v = w[nt * E + e];
+            w[nt * E + e] = mk<R>(v.x / nrm, v.y / nrm);
+        }
+    }
+    if (act)
+        for (int p = 0; p < nv; p++) W[gidx + (size_t)p * dof] = w[p * E + e];
+}
+
+template <typename R>
+void orthonormalize(hipStream_t s, cplx<R>* W, int Nx, int Nt, int dof, int ax, int at, int nv) {
+    const int E = ax * at * (dof / 2);
+    const int threads = ((E + 63) / 64) * 64;
+    if (threads > 1024) throw BadShape("orthonormalize: aggregate too large");
+    const size_t sh = sizeof(cplx<R>) * nv * E + sizeof(R) * 32;
+    const int nagg = (Nx / ax) * (Nt / at);
+    k_ortho<R><<<2 * nagg, threads, sh, s>>>(W, Nx, Nt, dof, ax, at, nv);
+    DD_LAUNCH_CHECK();
+}
+
+// Galerkin coarse links.  One work-group per coarse site; thread (i, j) of the
+// dc x dc output (dc = 2 nv) accumulates all five matrices.  For each fine site x of
+// the aggregate and each link k of the fine stencil, T = L_k(x) P_y is formed in LDS
+// (y the neighbour the link reaches) and then P_x^dagger T is added to A_c, or to
+// B_c,mu / C_c,mu when the hop leaves the aggregate through its +mu / -mu face.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_galerkin(const cplx<R>* __restrict__ Lf, const cplx<R>* __restrict__ W,
+                                                   cplx<R>* __restrict__ Lc, int Nx, int Nt, int df, int ax, int at,
+                                                   int nv, int has_A) {
+    extern __shared__ unsigned char smem_raw[];
+    const int dc = 2 * nv, col = df / 2;
+    cplx<R>* wx = reinterpret_cast<cplx<R>*>(smem_raw);   // [nv][df]
+    cplx<R>* wy = wx + nv * df;                           // [nv][df]
+    cplx<R>* T = wy + nv * df;                            // [df][dc]
+    const int Ntc = Nt / at;
+    const int Xc = blockIdx.x, X = Xc / Ntc, Tc = Xc - X * Ntc;
+    const int tid = threadIdx.x;
+    const bool own = tid < dc * dc;
+    const int i = own ? tid % dc : 0, j = own ? tid / dc : 0;   // output row i=(p,alf), col j=(s,bet)
+    const int p = i >> 1, alf = i & 1;
+    cplx<R> acc[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = czero<R>();
+    for (int lx = 0; lx < ax; lx++)
+        for (int lt = 0; lt < at; lt++) {
+            const int x = X * ax + lx, t = Tc * at + lt;
+            const int n = x * Nt + t;
+            __syncthreads();
+            for (int e = tid; e < nv * df; e += blockDim.x) wx[e] = W[(size_t)n * nv * df + e];
+            for (int kk = has_A ? 0 : 1; kk < 5; kk++) {
+                int y, target;
+                switch (kk) {
+                    case 0: y = n; target = 0; break;
+                    case 1: y = x * Nt + (t == Nt - 1 ? 0 : t + 1); target = (lt == at - 1) ? 1 : 0; break;
+                    case 2: y = (x == Nx - 1 ? 0 : x + 1) * Nt + t; target = (lx == ax - 1) ? 2 : 0; break;
+                    case 3: y = x * Nt + (t == 0 ? Nt - 1 : t - 1); target = (lt == 0) ? 3 : 0; break;
+                    default: y = (x == 0 ? Nx - 1 : x - 1) * Nt + t; target = (lx == 0) ? 4 : 0; break;
+                }
+                __syncthreads();
+                for (int e = tid; e < nv * df; e += blockDim.x) wy[e] = W[(size_t)y * nv * df + e];
+                __syncthreads();
+                // T[row=(c,a)][jj=(s,b)] = sum_b' L_k(x)[row][(b',b)] w_s(y, b', b)
+                const cplx<R>* Lk = Lf + ((size_t)n * 5 + kk) * df * df;
+                for (int e = tid; e < df * dc; e += blockDim.x) {
+                    const int row = e / dc, jj = e - row * dc, ss = jj >> 1, bet = jj & 1;
+                    cplx<R> sum = czero<R>();
+                    for (int bb = 0; bb < col; bb++) {
+                        const int cidx = bb * 2 + bet;
+                        sum = sum + Lk[cidx * df + row] * wy[ss * df + cidx];
+                    }
+                    T[e] = sum;
+                }
+                __syncthreads();
+                if (own) {
+                    cplx<R> sum = czero<R>();
+                    for (int c = 0; c < col; c++) {
+                        const int rr = c * 2 + alf;
+                        sum = sum + cmul(wx[p * df + rr], T[rr * dc + j]);
+                    }
+                    // static indexing keeps the accumulators in registers
+                    if (target == 0) acc[0] = acc[0] + sum;
+                    else if (target == 1) acc[1] = acc[1] + sum;
+                    else if (target == 2) acc[2] = acc[2] + sum;
+                    else if (target == 3) acc[3] = acc[3] + sum;
+                    else acc[4] = acc[4] + sum;
+                }
+            }
+        }
+    if (own) {
+        cplx<R>* o = Lc + (size_t)Xc * 5 * dc * dc;
+#pragma unroll
+        for (int q = 0; q < 5; q++) o[(q * dc + j) * dc + i] = acc[q];
+    }
+}
+
+template <typename R>
+void galerkin(hipStream_t s, const cplx<R>* Lf, const cplx<R>* W, cplx<R>* Lc, int Nx, int Nt, int df, int ax, int at,
+              int nv, bool has_A) {
+    const int dc = 2 * nv;
+    if (dc * dc > 1024) throw BadShape("galerkin: too many test vectors");
+    const int threads = std::max(64, ((dc * dc + 63) / 64) * 64);
+    const size_t sh = sizeof(cplx<R>) * (2 * nv * df + df * dc);
+    const int Vc = (Nx / ax) * (Nt / at);
+    k_galerkin<R><<<Vc, threads, sh, s>>>(Lf, W, Lc, Nx, Nt, df, ax, at, nv, has_A ? 1 : 0);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ void k_transpose_tv(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int V, int nv, int dof,
+                               int to_site) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)V * nv * dof;
+    if (idx >= tot) return;
+    // idx enumerates the site-major layout [V][nv][dof]
+    const int d = (int)(idx % dof);
+    const size_t q = idx / dof;
+    const int p = (int)(q % nv);
+    const size_t n = q / nv;
+    const size_t vm = ((size_t)p * V + n) * dof + d;   // vector-major [nv][V][dof]
+    if (to_site) out[idx] = in[vm];
+    else out[vm] = in[idx];
+}
+
+template <typename R>
+void transpose_tv(hipStream_t s, const cplx<R>* in, cplx<R>* out, int V, int nv, int dof, bool to_site_major) {
+    k_transpose_tv<R><<<cdivu((size_t)V * nv * dof, 256), 256, 0, s>>>(in, out, V, nv, dof, to_site_major ? 1 : 0);
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// BLAS-1 with deterministic two-pass reductions (partials summed in a fixed order).
+// ======================================================================================
+template <typename R>
+__global__ __launch_bounds__(256) void k_dot_partial(const cplx<R>* __restrict__ a, const cplx<R>* __restrict__ b,
+                                                     size_t n, size_t ld, uint64_t mask,
+                                                     cplx<double>* __restrict__ partial) {
+    __shared__ R scratch[32];
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const cplx<R>* ar = a + r * ld;
+    const cplx<R>* br = b + r * ld;
+    cplx<R> s = czero<R>();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s = s + mulc(ar[i], br[i]);
+    s = block_sum(s, scratch);
+    if (threadIdx.x == 0) partial[(size_t)r * gridDim.x + blockIdx.x] = mk<double>(s.x, s.y);
+}
+
+static inline int red_blocks(size_t n) { return (int)std::min<size_t>(256, std::max<size_t>(1, (n + 2047) / 2048)); }
+
+template <typename R>
+void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, size_t ld, int nrhs, uint64_t mask,
+                 RedBuf& rb) {
+    rb.nblk = red_blocks(n);
+    dim3 grid(rb.nblk, nrhs);
+    k_dot_partial<R><<<grid, 256, 0, s>>>(a, b, n, ld, mask, (cplx<double>*)rb.partial);
+    DD_LAUNCH_CHECK();
+}
+
+// sum of the nblk partials of system r, in a fixed order (lane-strided then butterfly),
+// by one wave; every lane returns the same bits
+__device__ __forceinline__ cplx<double> sum_partials(const cplx<double>* p, int nblk) {
+    const int lane = threadIdx.x & 63;
+    cplx<double> s = czero<double>();
+    for (int i = lane; i < nblk; i += 64) s = s + p[i];
+    return wave_sum(s, 64);
+}
+
+__global__ void k_dot_final(const cplx<double>* __restrict__ partial, int nblk, cplx<double>* out, int ostride,
+                            uint64_t mask) {
+    const int r = blockIdx.x;
+    if (!rhs_on(mask, r)) return;
+    const cplx<double> s = sum_partials(partial + (size_t)r * nblk, nblk);
+    if (threadIdx.x == 0) out[(size_t)r * ostride] = s;
+}
+
+template <typename R>
+void dot_final(hipStream_t s, const RedBuf& rb, cplx<double>* out, int ostride, int nrhs, uint64_t mask) {
+    k_dot_final<<<nrhs, 64, 0, s>>>((const cplx<double>*)rb.partial, rb.nblk, out, ostride, mask);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_mgs_update(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ v, size_t n,
+                                                    size_t ld, uint64_t mask, const cplx<double>* __restrict__ partial,
+                                                    int nblk, cplx<double>* H, int hstride) {
+    __shared__ cplx<double> hs;
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    if (threadIdx.x < 64) {
+        const cplx<double> h = sum_partials(partial + (size_t)r * nblk, nblk);
+        if (threadIdx.x == 0) {
+            hs = h;
+            if (blockIdx.x == 0) H[(size_t)r * hstride] = h;
+        }
+    }
+    __syncthreads();
+    const cplx<R> h = mk<R>((R)hs.x, (R)hs.y);
+    cplx<R>* wr = w + r * ld;
+    const cplx<R>* vr = v + r * ld;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        wr[i] = wr[i] - h * vr[i];
+}
+
+static inline unsigned ew_blocks(size_t n) { return (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (n + 255) / 256)); }
+
+template <typename R>
+void mgs_update(hipStream_t s, cplx<R>* w, const cplx<R>* v, size_t n, size_t ld, int nrhs, uint64_t mask,
+                const RedBuf& rb, cplx<double>* H, int hstride) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_mgs_update<R><<<grid, 256, 0, s>>>(w, v, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_norm_scale(cplx<R>* __restrict__ out, const cplx<R>* __restrict__ w, size_t n,
+                                                    size_t ld, uint64_t mask, const cplx<double>* __restrict__ partial,
+                                                    int nblk, cplx<double>* H, int hstride) {
+    __shared__ double hs;
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    if (threadIdx.x < 64) {
+        const cplx<double> h = sum_partials(partial + (size_t)r * nblk, nblk);
+        if (threadIdx.x == 0) {
+            // partial sums are accumulated in R; the norm is formed in R as well
+            const R hn = sqrt((R)h.x);
+            hs = (double)hn;
+            if (blockIdx.x == 0) H[(size_t)r * hstride] = mk<double>((double)hn, 0.0);
+        }
+    }
+    __syncthreads();
+    const R hn = (R)hs;
+    if (!(hn > R(0))) return;
+    const R inv = R(1) / hn;
+    cplx<R>* o = out + r * ld;
+    const cplx<R>* wr = w + r * ld;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = inv * wr[i];
+}
+
+template <typename R>
+void norm_scale(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
+                const RedBuf& rb, cplx<double>* H, int hstride) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_norm_scale<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scale_inv(cplx<R>* __restrict__ out, const cplx<R>* __restrict__ w, size_t n,
+                                                   size_t ld, uint64_t mask, const cplx<double>* __restrict__ beta,
+                                                   int bstride) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const R bt = (R)beta[(size_t)r * bstride].x;
+    const R inv = R(1) / bt;
+    cplx<R>* o = out + r * ld;
+    const cplx<R>* wr = w + r * ld;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = inv * wr[i];
+}
+
+template <typename R>
+void scale_inv(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
+               const cplx<double>* beta, int bstride) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_scale_inv<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, beta, bstride);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_axpy(cplx<R>* out, const cplx<R>* x, const cplx<R>* y, R alpha, size_t n,
+                                              size_t ld, uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[r * ld + i] = x[r * ld + i] + alpha * y[r * ld + i];
+}
+
+template <typename R>
+void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, double alpha, size_t n, size_t ld, int nrhs,
+          uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_axpy<R><<<grid, 256, 0, s>>>(out, x, y, (R)alpha, n, ld, mask);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_update_x(cplx<R>* __restrict__ x, const cplx<R>* __restrict__ Z,
+                                                  size_t zstride, const cplx<double>* __restrict__ eta,
+                                                  const int* __restrict__ nit, int m, size_t n, size_t ld,
+                                                  uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const int k = nit[r];
+    cplx<R>* xr = x + r * ld;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        cplx<R> v = xr[i];
+        for (int j = 0; j < k; j++) {
+            const cplx<double> e = eta[(size_t)r * m + j];
+            v = v + mk<R>((R)e.x, (R)e.y) * Z[j * zstride + r * ld + i];
+        }
+        xr[i] = v;
+    }
+}
+
+template <typename R>
+void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const cplx<double>* eta, const int* nit,
+              int m, size_t n, size_t ld, int nrhs, uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_update_x<R><<<grid, 256, 0, s>>>(x, Z, zstride, eta, nit, m, n, ld, mask);
+    DD_LAUNCH_CHECK();
+}
+
+
+// ---- classical Gram-Schmidt passes (used by CGS2 orthogonalisation) --------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_multidot(const cplx<R>* __restrict__ w, const cplx<R>* __restrict__ V,
+                                                  size_t vstride, int nvec, size_t n, size_t ld, uint64_t mask,
+                                                  cplx<double>* __restrict__ partial) {
+    __shared__ R scratch[32];
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const cplx<R>* wr = w + r * ld;
+    for (int i = 0; i < nvec; i++) {
+        const cplx<R>* vr = V + i * vstride + r * ld;
+        cplx<R> s = czero<R>();
+        for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+            s = s + mulc(wr[e], vr[e]);
+        s = block_sum(s, scratch);
+        if (threadIdx.x == 0) partial[((size_t)r * nvec + i) * gridDim.x + blockIdx.x] = mk<double>(s.x, s.y);
+    }
+}
+
+template <typename R>
+void multidot(hipStream_t s, const cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld,
+              int nrhs, uint64_t mask, RedBuf& rb) {
+    rb.nblk = red_blocks(n);
+    dim3 grid(rb.nblk, nrhs);
+    k_multidot<R><<<grid, 256, 0, s>>>(w, V, vstride, nvec, n, ld, mask, (cplx<double>*)rb.partial);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_multi_axpy(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ V,
+                                                    size_t vstride, int nvec, size_t n, size_t ld, uint64_t mask,
+                                                    const cplx<double>* __restrict__ partial, int nblk,
+                                                    cplx<double>* H, int hstride, int accumulate) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<double>* hs = reinterpret_cast<cplx<double>*>(smem_raw);
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int i = wid; i < nvec; i += nw) {
+        const cplx<double> h = sum_partials(partial + ((size_t)r * nvec + i) * nblk, nblk);
+        if ((threadIdx.x & 63) == 0) hs[i] = h;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < nvec; i += blockDim.x) {
+            cplx<double>* hp = H + (size_t)r * hstride + i;
+            *hp = accumulate ? (*hp + hs[i]) : hs[i];
+        }
+    cplx<R>* wr = w + r * ld;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        cplx<R> v = wr[e];
+        for (int i = 0; i < nvec; i++) v = v - mk<R>((R)hs[i].x, (R)hs[i].y) * V[i * vstride + r * ld + e];
+        wr[e] = v;
+    }
+}
+
+template <typename R>
+void multi_axpy(hipStream_t s, cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld, int nrhs,
+                uint64_t mask, const RedBuf& rb, cplx<double>* H, int hstride, bool accumulate) {
+    dim3 grid(ew_blocks(n), nrhs);
+    const size_t sh = sizeof(cplx<double>) * nvec;
+    k_multi_axpy<R><<<grid, 256, sh, s>>>(w, V, vstride, nvec, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H,
+                                          hstride, accumulate ? 1 : 0);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename To, typename From>
+__global__ __launch_bounds__(256) void k_convert(cplx<To>* __restrict__ out, const cplx<From>* __restrict__ in, size_t n,
+                                                 size_t ldo, size_t ldi, uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[r * ldo + i] = cast<To>(in[r * ldi + i]);
+}
+
+template <typename R>
+void copy(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ldo, size_t ldi, int nrhs, uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_convert<R, R><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename To, typename From>
+void convert(hipStream_t s, cplx<To>* out, const cplx<From>* in, size_t n, size_t ldo, size_t ldi, int nrhs,
+             uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_convert<To, From><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_zero(cplx<R>* out, size_t n, size_t ld, uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[r * ld + i] = czero<R>();
+}
+
+template <typename R>
+void zero(hipStream_t s, cplx<R>* out, size_t n, size_t ld, int nrhs, uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_zero<R><<<grid, 256, 0, s>>>(out, n, ld, mask);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_sigma3(cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const cplx<R> v = in[r * ld + i];
+        out[r * ld + i] = (i & 1) ? -v : v;
+    }
+}
+
+template <typename R>
+void sigma3(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, int nrhs, uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_sigma3<R><<<grid, 256, 0, s>>>(out, in, n, ld, mask);
+    DD_LAUNCH_CHECK();
+}
+
+__global__ void k_gate(int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
+                       uint64_t mask, int nrhs) {
+    const int r = threadIdx.x;
+    if (r >= nrhs || !rhs_on(mask, r)) return;
+    if (sqrt(err2[r * stride].x) < tol * sqrt(v2[r * stride].x)) gate[r] = 0;
+}
+
+void gate_update(hipStream_t s, int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
+                 int nrhs, uint64_t mask) {
+    k_gate<<<1, 64, 0, s>>>(gate, err2, v2, stride, tol, mask, nrhs);
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// Quenched U(1) Metropolis (seeded, counter-based random numbers)
+// ======================================================================================
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+// One sub-sweep: links of direction mu with parity par (of x for mu = 0, of t for mu = 1)
+// share no plaquette and are updated together.  theta_P(x) = th0(x) + th1(x+0) - th0(x+1) - th1(x).
+__global__ void k_metro(double* th, int Nx, int Nt, double beta, uint64_t key, int mu, int par, double step) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Nx * Nt / 2) return;
+    int x, t;
+    if (mu == 0) {
+        const int hx = Nx / 2;
+        t = idx / hx;
+        x = 2 * (idx - t * hx) + par;
+    } else {
+        const int ht = Nt / 2;
+        x = idx / ht;
+        t = 2 * (idx - x * ht) + par;
+    }
+    auto T0 = [&](int xx, int tt) -> double {
+        xx = (xx + Nx) % Nx;
+        tt = (tt + Nt) % Nt;
+        return th[(xx * Nt + tt) * 2];
+    };
+    auto T1 = [&](int xx, int tt) -> double {
+        xx = (xx + Nx) % Nx;
+        tt = (tt + Nt) % Nt;
+        return th[(xx * Nt + tt) * 2 + 1];
+    };
+    auto P = [&](int xx, int tt) -> double { return T0(xx, tt) + T1(xx, tt + 1) - T0(xx + 1, tt) - T1(xx, tt); };
+    const uint64_t h1 = mix64(key ^ ((uint64_t)(x * Nt + t) * 2 + 0));
+    const uint64_t h2 = mix64(h1 ^ 0x5851f42d4c957f2dull);
+    const double d = step * (2.0 * u01(h1) - 1.0);
+    double pa, pb, na, nb;
+    if (mu == 0) {   // in P(x,t) with +, in P(x-1,t) with -
+        pa = P(x, t);
+        pb = P(x - 1, t);
+        na = pa + d;
+        nb = pb - d;
+    } else {         // in P(x,t) with -, in P(x,t-1) with +
+        pa = P(x, t);
+        pb = P(x, t - 1);
+        na = pa - d;
+        nb = pb + d;
+    }
+    const double dS = -beta * (cos(na) + cos(nb) - cos(pa) - cos(pb));
+    if (dS <= 0.0 || u01(h2) < exp(-dS)) th[(x * Nt + t) * 2 + mu] += d;
+}
+
+__global__ void k_wrap(double* th, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pi = 3.141592653589793;
+    double v = fmod(th[i] + pi, 2.0 * pi);
+    if (v < 0) v += 2.0 * pi;
+    th[i] = v - pi;
+}
+
+void metropolis_u1(hipStream_t s, double* theta, int Nx, int Nt, double beta, int sweeps, uint64_t seed, double step) {
+    if ((Nx & 1) || (Nt & 1)) throw BadShape("metropolis: lattice extents must be even");
+    DD_HIP(hipMemsetAsync(theta, 0, sizeof(double) * Nx * Nt * 2, s));
+    const unsigned blocks = cdivu((size_t)Nx * Nt / 2, 256);
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int mu = 0; mu < 2; mu++)
+            for (int par = 0; par < 2; par++) {
+                const uint64_t key = (seed * 0x100000001b3ull) ^ ((uint64_t)sw << 20) ^ ((uint64_t)(mu * 2 + par) << 52);
+                k_metro<<<blocks, 256, 0, s>>>(theta, Nx, Nt, beta, key, mu, par, step);
+            }
+    k_wrap<<<cdivu((size_t)Nx * Nt * 2, 256), 256, 0, s>>>(theta, (size_t)Nx * Nt * 2);
+    DD_LAUNCH_CHECK();
+}
+
+template <typename R>
+__global__ void k_theta_to_u(const double* th, cplx<R>* U, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    U[i] = mk<R>((R)cos(th[i]), (R)sin(th[i]));
+}
+
+template <typename R>
+void theta_to_u(hipStream_t s, const double* theta, cplx<R>* U, size_t n) {
+    k_theta_to_u<R><<<cdivu(n, 256), 256, 0, s>>>(theta, U, n);
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// dD/dtheta contraction.  With U = e^{i theta}: the link U_mu(n) enters row n through
+// -1/2 (1 - sigma_mu) s+ U x(n+mu) and row n+mu through -1/2 (1 + sigma_mu) s- U^* x(n).
+// grad[n][mu] += scale * Re[ y(n)^H (1-s_mu) s i U x(n+mu) + y(n+mu)^H (1+s_mu) s (-i) U^* x(n) ]
+// (scale = 1/2 gives dL/dtheta for L = Re<eta, D^-1 b>, y = D^-dagger eta, x = D^-1 b).
+// ======================================================================================
+__global__ void k_dtheta(const cplx<double>* __restrict__ U, const cplx<double>* __restrict__ y,
+                         const cplx<double>* __restrict__ xv, double* grad, int Nx, int Nt, int nrhs, size_t ld,
+                         double scale) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Nx * Nt) return;
+    const int x = n / Nt, t = n - x * Nt;
+    const cplx<double> I = mk(0.0, 1.0);
+    for (int mu = 0; mu < 2; mu++) {
+        const int m = (mu == 0) ? x * Nt + (t == Nt - 1 ? 0 : t + 1) : (x == Nx - 1 ? 0 : x + 1) * Nt + t;
+        const double sg = (mu == 0 && t == Nt - 1) ? -1.0 : 1.0;
+        const cplx<double> u = U[2 * n + mu];
+        double acc = 0.0;
+        for (int r = 0; r < nrhs; r++) {
+            const cplx<double>* yr = y + r * ld;
+            const cplx<double>* xr = xv + r * ld;
+            const cplx<double> iu = sg * (I * u);
+            const cplx<double> f0 = iu * xr[2 * m], f1 = iu * xr[2 * m + 1];
+            const cplx<double> mi = sg * (-(I * conj(u)));
+            const cplx<double> b0 = mi * xr[2 * n], b1 = mi * xr[2 * n + 1];
+            cplx<double> fm0, fm1, bp0, bp1;   // (1 - sigma) f, (1 + sigma) b
+            if (mu == 0) {
+                fm0 = f0 - f1;
+                fm1 = f1 - f0;
+                bp0 = b0 + b1;
+                bp1 = b0 + b1;
+            } else {
+                fm0 = f0 + times_i(f1);
+                fm1 = f1 - times_i(f0);
+                bp0 = b0 - times_i(b1);
+                bp1 = times_i(b0) + b1;
+            }
+            const cplx<double> tv = cmul(yr[2 * n], fm0) + cmul(yr[2 * n + 1], fm1) + cmul(yr[2 * m], bp0) +
+                                    cmul(yr[2 * m + 1], bp1);
+            acc += tv.x;
+        }
+        grad[2 * n + mu] += scale * acc;
+    }
+}
+
+void dtheta_contract(hipStream_t s, const cplx<double>* U, const cplx<double>* y, const cplx<double>* x, double* grad,
+                     int Nx, int Nt, int nrhs, size_t ld, double scale) {
+    k_dtheta<<<cdivu((size_t)Nx * Nt, 256), 256, 0, s>>>(U, y, x, grad, Nx, Nt, nrhs, ld, scale);
+    DD_LAUNCH_CHECK();
+}
+
+// ======================================================================================
+// Explicit instantiations
+// ======================================================================================
+#define DD_INST(R)                                                                                                  \
+    template void wilson<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, const cplx<R>*, int, int, double, \
+                            bool, int, uint64_t, size_t);                                                           \
+    template void dirac_links<R>(hipStream_t, const cplx<R>*, cplx<R>*, int, int);                                  \
+    template void coarse_apply<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, const cplx<R>*, int, int,  \
+                                  int, double, int, uint64_t, size_t);                                              \
+    template void sap_blocks0<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int,    \
+                                 double, int, int, double, int, uint64_t, size_t, const int*);                      \
+    template void sap_blocksc<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int,    \
+                                 int, double, int, int, double, int, uint64_t, size_t, const int*);                 \
+    template void restrict_<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int, int, \
+                               int, uint64_t, size_t, size_t);                                                      \
+    template void prolong<R>(hipStream_t, const cplx<R>*, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int,  \
+                             int, int, int, int, uint64_t, size_t, size_t);                                         \
+    template void orthonormalize<R>(hipStream_t, cplx<R>*, int, int, int, int, int, int);                          \
+    template void galerkin<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int, int,  \
+                              bool);                                                                                \
+    template void transpose_tv<R>(hipStream_t, const cplx<R>*, cplx<R>*, int, int, int, bool);                      \
+    template void dot_partial<R>(hipStream_t, const cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,        \
+                                 RedBuf&);                                                                          \
+    template void dot_final<R>(hipStream_t, const RedBuf&, cplx<double>*, int, int, uint64_t);                      \
+    template void mgs_update<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,               \
+                                const RedBuf&, cplx<double>*, int);                                                 \
+    template void norm_scale<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,               \
+                                const RedBuf&, cplx<double>*, int);                                                 \
+    template void scale_inv<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,                \
+                               const cplx<double>*, int);                                                           \
+    template void axpy<R>(hipStream_t, cplx<R>*, const cplx<R>*, const cplx<R>*, double, size_t, size_t, int,       \
+                          uint64_t);                                                                                \
+    template void update_x<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, const cplx<double>*, const int*, int,  \
+                              size_t, size_t, int, uint64_t);                                                       \
+    template void copy<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, size_t, int, uint64_t);            \
+    template void zero<R>(hipStream_t, cplx<R>*, size_t, size_t, int, uint64_t);                                    \
+    template void sigma3<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t);                  \
+    template void multidot<R>(hipStream_t, const cplx<R>*, const cplx<R>*, size_t, int, size_t, size_t, int,        \
+                              uint64_t, RedBuf&);                                                                   \
+    template void multi_axpy<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, int, size_t, size_t, int, uint64_t,  \
+                                const RedBuf&, cplx<double>*, int, bool);                                           \
+    template void theta_to_u<R>(hipStream_t, const double*, cplx<R>*, size_t);
+
+DD_INST(float)
+DD_INST(double)
+template void convert<float, double>(hipStream_t, cplx<float>*, const cplx<double>*, size_t, size_t, size_t, int,
+                                     uint64_t);
+template void convert<double, float>(hipStream_t, cplx<double>*, const cplx<float>*, size_t, size_t, size_t, int,
+                                     uint64_t);
+template void convert<double, double>(hipStream_t, cplx<double>*, const cplx<double>*, size_t, size_t, size_t, int,
+                                      uint64_t);
+template void convert<float, float>(hipStream_t, cplx<float>*, const cplx<float>*, size_t, size_t, size_t, int,
+                                    uint64_t);
+
+}  // namespace dd

@@ -1,0 +1,185 @@
+// Host-side orchestration of the DD-alphaAMG solve on one GCD: device buffers, the
+// Krylov engine, the multigrid hierarchy with its setup and cycles, and the top-level
+// solver object behind the C ABI.
+#pragma once
+#include <complex>
+#include <functional>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "../../include/ddamg.h"
+#include "kernels.h"
+
+namespace dd {
+
+// ---- memory ------------------------------------------------------------------------
+class DevMem {
+   public:
+    DevMem() = default;
+    explicit DevMem(size_t bytes) { alloc(bytes); }
+    ~DevMem() { release(); }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevMem& operator=(DevMem&& o) noexcept {
+        if (this != &o) {
+            release();
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void alloc(size_t bytes);
+    void release();
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p_); }
+    size_t bytes() const { return bytes_; }
+
+   private:
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+class PinMem {
+   public:
+    PinMem() = default;
+    explicit PinMem(size_t bytes) { alloc(bytes); }
+    ~PinMem() { release(); }
+    PinMem(const PinMem&) = delete;
+    PinMem& operator=(const PinMem&) = delete;
+    void alloc(size_t bytes);
+    void release();
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p_); }
+
+   private:
+    void* p_ = nullptr;
+};
+
+inline uint64_t full_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+inline int mask_count(uint64_t m) { return m ? 64 - __builtin_clzll(m) : 0; }   // highest set bit + 1
+
+// ---- Krylov engine -------------------------------------------------------------------
+// Restates FGMRES::fgmres (src/fgmres.cpp:5-97) for a batch of independent systems:
+// each system follows the reference's control flow (break on |g_{j+1}| < tol ||phi||,
+// maxIt kept across restarts, true residual after each cycle, k*m + maxIt count),
+// with Z_j persisting across restarts of one call.  Vectors of system r live at
+// base + r*n.  Orthogonalisation: modified Gram-Schmidt as the reference, or
+// classical Gram-Schmidt with one re-orthogonalisation (CGS2, two reductions per step).
+template <typename R>
+class Krylov {
+   public:
+    using Func = std::function<void(const cplx<R>* in, cplx<R>* out, uint64_t mask)>;
+    using Prec = std::function<void(const cplx<R>* in, cplx<R>* inout, uint64_t mask)>;
+
+    Krylov(hipStream_t s, size_t n, int m, int restarts, double tol, int cap, bool flexible, int ortho);
+    // Returns a mask of the systems that converged; iters[r] (cap entries) per system.
+    uint64_t solve(const cplx<R>* phi, const cplx<R>* x0, cplx<R>* x, uint64_t mask, const Func& A,
+                   const Prec* M, int* iters, double* relres);
+    size_t n() const { return n_; }
+    long long steps = 0;   // Arnoldi steps taken (max over the batch per step), cumulative
+
+   private:
+    void ortho(int j, cplx<R>* w, uint64_t run, int nr);
+    hipStream_t s_;
+    size_t n_;
+    int m_, restarts_, cap_, ortho_;
+    double tol_;
+    bool flex_;
+    DevMem V_, Z_, r_, w_, H_, nrm_, eta_, nit_, part_;
+    PinMem Hh_, nrmh_, etah_, nith_;
+    RedBuf rb_;
+};
+
+// ---- hierarchy -------------------------------------------------------------------------
+struct LevelGeom {
+    int Nx = 0, Nt = 0, dof = 0;
+    int ax = 1, at = 1, nv = 0;   // aggregation to the next level
+    int bx = 1, bt = 1;           // SAP blocks
+    size_t n() const { return (size_t)Nx * Nt * dof; }
+    int V() const { return Nx * Nt; }
+};
+
+std::vector<LevelGeom> make_geometry(int Nx, int Nt, const dd_params& p);
+
+// Level-0 test vectors drawn as AlgebraicMG::setUpPhase does (src/amg.cpp:6-21, 33-39):
+// one std::mt19937(rank) per virtual rank of an Rx x Rt grid, loop cc -> local x ->
+// local t -> dof, each entry taking two uniform(-1, 1) draws.  The reference writes
+// `d(g) + I*d(g)`, whose operands g++ evaluates right to left: the first draw is the
+// imaginary part.
+class RankStreams {
+   public:
+    RankStreams(int Rx, int Rt);
+    void fill(std::complex<double>* out, int first, int count, int Nx, int Nt, int dof);   // out[cc][V][dof]
+
+   private:
+    int Rx_, Rt_;
+    std::vector<std::mt19937> g_;
+};
+
+template <typename R>
+class Multigrid {
+   public:
+    Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const cplx<R>* U);
+    // setup phase from raw level-0 test vectors [nv0][V0][2] (device, precision R)
+    void setup(const cplx<R>* tv0, RankStreams& rng);
+    void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    void apply(int l, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mask);
+    void restrict_(int l, const cplx<R>* in, cplx<R>* out, uint64_t mask);
+    void prolong(int l, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, uint64_t mask);
+    void sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, uint64_t mask);
+    const LevelGeom& geom(int l) const { return g_[l]; }
+    int levels() const { return (int)g_.size(); }
+    const cplx<R>* links(int l) const { return links_[l].template as<cplx<R>>(); }
+    const cplx<R>* W(int l) const { return W_[l].template as<cplx<R>>(); }
+    int cap(int l) const { return cap_[l]; }
+    std::vector<long long> coarse_iters;   // coarsest GMRES iterations per call (stats)
+
+   private:
+    hipStream_t s_;
+    dd_params p_;
+    std::vector<LevelGeom> g_;
+    std::vector<int> cap_;
+    const cplx<R>* U_;
+    std::vector<DevMem> links_, W_;
+    // per-level work: residual, restricted rhs / coarse solution of the next level,
+    // prolongated correction, SAP residual and gates
+    std::vector<DevMem> res_, eta_, psi_, sapr_, gate_, nrm_;
+    std::vector<RedBuf> rb_;
+    std::vector<DevMem> part_;
+    std::vector<std::unique_ptr<Krylov<R>>> kry_;   // K-cycle FGMRES of level l (index l)
+    std::unique_ptr<Krylov<R>> coarse_;
+    std::vector<int> iters_scratch_;
+};
+
+// ---- top-level solver --------------------------------------------------------------------
+class Solver {
+   public:
+    Solver(int Nx, int Nt, int device);
+    ~Solver();
+    void set_gauge(const cplx<double>* U_dev);
+    void setup(const dd_params& p);
+    // b, x: device pointers, nrhs systems of V*2 entries back to back
+    int solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info);
+    void apply_D(const cplx<double>* in, cplx<double>* out, int nrhs, double m0, bool dagger, int precision);
+    int Nx, Nt, device;
+    hipStream_t s = nullptr;
+    DevMem U64, U32, theta;
+    dd_params prm{};
+    bool have_gauge = false, have_setup = false;
+    double setup_seconds = 0;
+    std::unique_ptr<Multigrid<double>> mg64;
+    std::unique_ptr<Multigrid<float>> mg32;
+
+   private:
+    std::unique_ptr<Krylov<double>> outer_;
+    DevMem x32_, y32_;
+    int outer_cap_ = 0;
+};
+
+}  // namespace dd

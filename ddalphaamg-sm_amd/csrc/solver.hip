@@ -1,0 +1,600 @@
+// Host orchestration of the DD-alphaAMG solve (see solver.h).
+//
+// Reference correspondence (paths relative to the reference root):
+//   Krylov::solve            FGMRES::fgmres / rotation / solve_upper_triangular,
+//                            src/fgmres.cpp:5-127; hooks include/fgmres.h:86-102
+//   Multigrid::sap           SAP_C::SAP, src/sap.cpp:81-165
+//   Multigrid::kcycle/vcycle AlgebraicMG::k_cycle / v_cycle, src/amg.cpp:64-174
+//   Multigrid::coarsest      GMRES_level_l, include/level.h:70-140
+//   Multigrid::setup         AlgebraicMG::setUpPhase, src/amg.cpp:4-62
+//   Solver::solve            FGMRES_AMG_{k,v}_cycle, include/amg.h:147-270
+#include "solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace dd {
+
+// ---------------------------------------------------------------------------------------
+void DevMem::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    DD_HIP(hipMalloc(&p_, bytes));
+    bytes_ = bytes;
+}
+void DevMem::release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+}
+void PinMem::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return;
+    DD_HIP(hipHostMalloc(&p_, bytes, hipHostMallocDefault));
+}
+void PinMem::release() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+}
+
+__global__ void k_fill_int(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Krylov engine
+// ---------------------------------------------------------------------------------------
+template <typename R>
+Krylov<R>::Krylov(hipStream_t s, size_t n, int m, int restarts, double tol, int cap, bool flexible, int ortho)
+    : s_(s), n_(n), m_(m), restarts_(restarts), cap_(cap), ortho_(ortho), tol_(tol), flex_(flexible) {
+    if (m < 1 || restarts < 1 || cap < 1 || cap > 64) throw BadShape("Krylov: bad m/restarts/batch");
+    const size_t vb = sizeof(cplx<R>) * n * cap;
+    V_.alloc(vb * (m + 1));
+    if (flex_) Z_.alloc(vb * m);
+    r_.alloc(vb);
+    w_.alloc(vb);
+    H_.alloc(sizeof(cplx<double>) * (size_t)m * cap * (m + 1));
+    nrm_.alloc(sizeof(cplx<double>) * cap * 4);
+    eta_.alloc(sizeof(cplx<double>) * cap * m);
+    nit_.alloc(sizeof(int) * cap);
+    const size_t nblk = std::min<size_t>(256, std::max<size_t>(1, (n + 2047) / 2048));
+    part_.alloc(sizeof(cplx<double>) * cap * (m + 1) * nblk);
+    rb_.partial = part_.as<void>();
+    Hh_.alloc(sizeof(cplx<double>) * cap * (m + 1));
+    nrmh_.alloc(sizeof(cplx<double>) * cap * 4);
+    etah_.alloc(sizeof(cplx<double>) * cap * m);
+    nith_.alloc(sizeof(int) * cap);
+}
+
+template <typename R>
+void Krylov<R>::ortho(int j, cplx<R>* w, uint64_t run, int nr) {
+    const size_t vs = n_ * cap_;
+    cplx<double>* Hj = H_.as<cplx<double>>() + (size_t)j * cap_ * (m_ + 1);
+    const cplx<R>* V = V_.as<cplx<R>>();
+    if (ortho_ == 0) {
+        for (int i = 0; i <= j; i++) {
+            dot_partial<R>(s_, w, V + i * vs, n_, n_, nr, run, rb_);
+            mgs_update<R>(s_, w, V + i * vs, n_, n_, nr, run, rb_, Hj + i, m_ + 1);
+        }
+    } else {
+        multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
+        multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, false);
+        multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
+        multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, true);
+    }
+}
+
+template <typename R>
+uint64_t Krylov<R>::solve(const cplx<R>* phi, const cplx<R>* x0, cplx<R>* x, uint64_t mask, const Func& A,
+                          const Prec* M, int* iters, double* relres) {
+    using C = std::complex<double>;
+    const int nr = mask_count(mask);
+    if (nr > cap_) throw BadShape("Krylov: batch larger than capacity");
+    const size_t n = n_, vs = n_ * cap_;
+    cplx<R>* V = V_.as<cplx<R>>();
+    cplx<R>* Z = flex_ ? Z_.as<cplx<R>>() : V;
+    cplx<R>* r = r_.as<cplx<R>>();
+    cplx<R>* w = w_.as<cplx<R>>();
+    cplx<double>* nrm = nrm_.as<cplx<double>>();
+    cplx<double>* nrmh = nrmh_.as<cplx<double>>();
+    // setZeros (include/fgmres.h:116-146): the basis and the Z vectors start at zero
+    for (int i = 0; i <= m_; i++) zero<R>(s_, V + i * vs, n, n, nr, mask);
+    if (flex_)
+        for (int i = 0; i < m_; i++) zero<R>(s_, Z + i * vs, n, n, nr, mask);
+    if (x != x0) copy<R>(s_, x, x0, n, n, n, nr, mask);
+    A(x, w, mask);
+    axpy<R>(s_, r, phi, w, -1.0, n, n, nr, mask);
+    dot_partial<R>(s_, phi, phi, n, n, nr, mask, rb_);
+    dot_final<R>(s_, rb_, nrm + 0, 4, nr, mask);
+    dot_partial<R>(s_, r, r, n, n, nr, mask, rb_);
+    dot_final<R>(s_, rb_, nrm + 1, 4, nr, mask);
+    DD_HIP(hipMemcpyAsync(nrmh, nrm, sizeof(cplx<double>) * 4 * nr, hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipStreamSynchronize(s_));
+    std::vector<double> nphi(nr), err(nr);
+    for (int q = 0; q < nr; q++) {
+        nphi[q] = std::sqrt(nrmh[4 * q].x);
+        err[q] = std::sqrt(nrmh[4 * q + 1].x);
+    }
+    std::vector<int> maxIt(nr, m_);
+    std::vector<std::vector<C>> H(nr, std::vector<C>((size_t)(m_ + 1) * m_));   // H[i*m + j]
+    std::vector<std::vector<C>> g(nr, std::vector<C>(m_ + 1)), sn(nr, std::vector<C>(m_)),
+        cn(nr, std::vector<C>(m_));
+    for (int q = 0; q < nr; q++)
+        if ((mask >> q) & 1) iters[q] = restarts_ * m_;
+    uint64_t done = ~mask;
+    // a zero initial residual is a solution already
+    for (int q = 0; q < nr; q++)
+        if (((mask >> q) & 1) && err[q] == 0.0) {
+            iters[q] = 0;
+            done |= 1ull << q;
+        }
+    cplx<double>* Hh = Hh_.as<cplx<double>>();
+    cplx<double>* etah = etah_.as<cplx<double>>();
+    int* nith = nith_.as<int>();
+    for (int k = 0; k < restarts_; k++) {
+        const uint64_t live = mask & ~done;
+        if (!live) break;
+        // V_0 = r / ||r|| from the partial sums of the last residual norm
+        norm_scale<R>(s_, V, r, n, n, nr, live, rb_, nrm + 2, 4);
+        for (int q = 0; q < nr; q++)
+            if ((live >> q) & 1) g[q][0] = C(err[q], 0.0);
+        uint64_t run = live;
+        for (int j = 0; j < m_ && run; j++) {
+            steps++;
+            cplx<R>* Vj = V + j * vs;
+            cplx<R>* Zj = Z + j * vs;
+            if (flex_) (*M)(Vj, Zj, run);
+            A(Zj, w, run);
+            ortho(j, w, run, nr);
+            cplx<double>* Hj = H_.as<cplx<double>>() + (size_t)j * cap_ * (m_ + 1);
+            dot_partial<R>(s_, w, w, n, n, nr, run, rb_);
+            norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, run, rb_, Hj + (j + 1), m_ + 1);
+            DD_HIP(hipMemcpyAsync(Hh, Hj, sizeof(cplx<double>) * (m_ + 1) * nr, hipMemcpyDeviceToHost, s_));
+            DD_HIP(hipStreamSynchronize(s_));
+            for (int q = 0; q < nr; q++) {
+                if (!((run >> q) & 1)) continue;
+                std::vector<C>& h = H[q];
+                for (int i = 0; i <= j + 1; i++) h[(size_t)i * m_ + j] = C(Hh[q * (m_ + 1) + i].x, Hh[q * (m_ + 1) + i].y);
+                // rotation (src/fgmres.cpp:99-115)
+                for (int i = 0; i < j; i++) {
+                    const C temp = std::conj(cn[q][i]) * h[i * m_ + j] + std::conj(sn[q][i]) * h[(i + 1) * m_ + j];
+                    h[(i + 1) * m_ + j] = -sn[q][i] * h[i * m_ + j] + cn[q][i] * h[(i + 1) * m_ + j];
+                    h[i * m_ + j] = temp;
+                }
+                const C hjj = h[j * m_ + j], hs = h[(j + 1) * m_ + j];
+                const C den = std::sqrt(std::real(std::conj(hjj) * hjj + std::conj(hs) * hs));
+                sn[q][j] = hs / den;
+                cn[q][j] = hjj / den;
+                h[j * m_ + j] = std::conj(cn[q][j]) * hjj + std::conj(sn[q][j]) * hs;
+                h[(j + 1) * m_ + j] = 0.0;
+                g[q][j + 1] = -sn[q][j] * g[q][j];
+                g[q][j] = std::conj(cn[q][j]) * g[q][j];
+                if (std::abs(g[q][j + 1]) < tol_ * nphi[q]) {
+                    maxIt[q] = j + 1;
+                    run &= ~(1ull << q);
+                }
+            }
+        }
+        // solve_upper_triangular (src/fgmres.cpp:118-127) and x += sum eta_j Z_j
+        for (int q = 0; q < nr; q++) {
+            nith[q] = 0;
+            if (!((live >> q) & 1)) continue;
+            const int nI = maxIt[q];
+            std::vector<C> eta(nI);
+            for (int i = nI - 1; i >= 0; i--) {
+                eta[i] = g[q][i];
+                for (int jj = i + 1; jj < nI; jj++) eta[i] -= H[q][(size_t)i * m_ + jj] * eta[jj];
+                eta[i] /= H[q][(size_t)i * m_ + i];
+            }
+            for (int jj = 0; jj < nI; jj++) etah[q * m_ + jj] = mk<double>(eta[jj].real(), eta[jj].imag());
+            nith[q] = nI;
+        }
+        DD_HIP(hipMemcpyAsync(eta_.as<void>(), etah, sizeof(cplx<double>) * m_ * nr, hipMemcpyHostToDevice, s_));
+        DD_HIP(hipMemcpyAsync(nit_.as<void>(), nith, sizeof(int) * nr, hipMemcpyHostToDevice, s_));
+        update_x<R>(s_, x, Z, vs, eta_.as<cplx<double>>(), nit_.as<int>(), m_, n, n, nr, live);
+        A(x, w, live);
+        axpy<R>(s_, r, phi, w, -1.0, n, n, nr, live);
+        dot_partial<R>(s_, r, r, n, n, nr, live, rb_);
+        dot_final<R>(s_, rb_, nrm + 1, 4, nr, live);
+        DD_HIP(hipMemcpyAsync(nrmh, nrm, sizeof(cplx<double>) * 4 * nr, hipMemcpyDeviceToHost, s_));
+        DD_HIP(hipStreamSynchronize(s_));
+        for (int q = 0; q < nr; q++) {
+            if (!((live >> q) & 1)) continue;
+            err[q] = std::sqrt(nrmh[4 * q + 1].x);
+            if (err[q] < tol_ * nphi[q]) {
+                iters[q] = k * m_ + maxIt[q];
+                done |= 1ull << q;
+            }
+        }
+    }
+    if (relres)
+        for (int q = 0; q < nr; q++)
+            if ((mask >> q) & 1) relres[q] = nphi[q] > 0 ? err[q] / nphi[q] : 0.0;
+    return mask & done;
+}
+
+// ---------------------------------------------------------------------------------------
+// Geometry and test-vector streams
+// ---------------------------------------------------------------------------------------
+std::vector<LevelGeom> make_geometry(int Nx, int Nt, const dd_params& p) {
+    if (p.levels < 2 || p.levels > DD_MAX_LEVELS) throw BadShape("levels must be in [2, 8]");
+    std::vector<LevelGeom> g(p.levels);
+    g[0].Nx = Nx;
+    g[0].Nt = Nt;
+    g[0].dof = 2;
+    for (int l = 0; l < p.levels; l++) {
+        if (l < p.levels - 1) {
+            g[l].ax = p.agg_x[l];
+            g[l].at = p.agg_t[l];
+            g[l].nv = p.nv[l];
+            g[l].bx = p.sap_x[l];
+            g[l].bt = p.sap_t[l];
+            if (g[l].ax < 1 || g[l].at < 1 || g[l].Nx % g[l].ax || g[l].Nt % g[l].at)
+                throw BadShape("aggregate size must divide the level lattice");
+            if (g[l].bx < 1 || g[l].bt < 1 || g[l].Nx % g[l].bx || g[l].Nt % g[l].bt)
+                throw BadShape("SAP block size must divide the level lattice");
+            const int nbx = g[l].Nx / g[l].bx, nbt = g[l].Nt / g[l].bt;
+            if ((nbx * nbt) % 2 || ((nbt & 1) && (nbx & 1))) throw BadShape("SAP needs an even number of blocks");
+            if (g[l].nv < 1 || 2 * g[l].nv > 32) throw BadShape("test vectors per level must be in [1, 16]");
+            g[l + 1].Nx = g[l].Nx / g[l].ax;
+            g[l + 1].Nt = g[l].Nt / g[l].at;
+            g[l + 1].dof = 2 * g[l].nv;
+        }
+    }
+    return g;
+}
+
+RankStreams::RankStreams(int Rx, int Rt) : Rx_(Rx), Rt_(Rt) {
+    if (Rx < 1 || Rt < 1) throw BadShape("virtual rank grid must be positive");
+    for (int r = 0; r < Rx * Rt; r++) g_.emplace_back((std::mt19937::result_type)r);
+}
+
+void RankStreams::fill(std::complex<double>* out, int first, int count, int Nx, int Nt, int dof) {
+    if (Nx % Rx_ || Nt % Rt_) throw BadShape("virtual rank grid must divide the level lattice");
+    const int wx = Nx / Rx_, wt = Nt / Rt_;
+    std::uniform_real_distribution<double> d(-1.0, 1.0);
+    const size_t V = (size_t)Nx * Nt;
+    for (int cx = 0; cx < Rx_; cx++)
+        for (int ct = 0; ct < Rt_; ct++) {
+            std::mt19937& g = g_[cx * Rt_ + ct];
+            for (int cc = first; cc < first + count; cc++)
+                for (int lx = 0; lx < wx; lx++)
+                    for (int lt = 0; lt < wt; lt++) {
+                        const size_t nsite = (size_t)(cx * wx + lx) * Nt + ct * wt + lt;
+                        for (int k = 0; k < dof; k++) {
+                            const double im = d(g);
+                            const double re = d(g);
+                            out[((size_t)cc * V + nsite) * dof + k] = std::complex<double>(re, im);
+                        }
+                    }
+        }
+}
+
+// ---------------------------------------------------------------------------------------
+// Multigrid hierarchy
+// ---------------------------------------------------------------------------------------
+template <typename R>
+Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const cplx<R>* U)
+    : s_(s), p_(p), g_(make_geometry(Nx, Nt, p)), U_(U) {
+    const int L = (int)g_.size();
+    const int cap = std::max(1, p.max_rhs);
+    if (cap > 64) throw BadShape("max_rhs must be <= 64");
+    cap_.resize(L);
+    links_.resize(L);
+    W_.resize(L);
+    res_.resize(L);
+    eta_.resize(L);
+    psi_.resize(L);
+    sapr_.resize(L);
+    gate_.resize(L);
+    nrm_.resize(L);
+    rb_.resize(L);
+    part_.resize(L);
+    kry_.resize(L);
+    for (int l = 0; l < L; l++) {
+        const LevelGeom& G = g_[l];
+        cap_[l] = (l < L - 1) ? std::max(cap, G.nv) : cap;
+        const size_t vb = sizeof(cplx<R>) * G.n();
+        res_[l].alloc(vb * cap_[l]);
+        sapr_[l].alloc(vb * cap_[l]);
+        gate_[l].alloc(sizeof(int) * 64);
+        nrm_[l].alloc(sizeof(cplx<double>) * 2 * 64);
+        part_[l].alloc(sizeof(cplx<double>) * 64 * 256);
+        rb_[l].partial = part_[l].as<void>();
+        if (l >= 1) {
+            eta_[l].alloc(vb * cap);
+            psi_[l].alloc(vb * cap);
+            links_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * 5 * G.dof * G.dof);
+            kry_[l].reset(new Krylov<R>(s, G.n(), p.kcycle_m, p.kcycle_restarts, p.kcycle_tol, cap, true, p.ortho_inner));
+        }
+        if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
+    }
+    coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap, false,
+                                p.ortho_coarse));
+    iters_scratch_.resize(64);
+}
+
+template <typename R>
+void Multigrid<R>::apply(int l, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mask) {
+    const LevelGeom& G = g_[l];
+    const int nr = mask_count(mask);
+    if (l == 0) wilson<R>(s_, U_, in, out, b, G.Nx, G.Nt, p_.m0, false, nr, mask, G.n());
+    else coarse_apply<R>(s_, links(l), in, out, b, G.Nx, G.Nt, G.dof, p_.m0, nr, mask, G.n());
+}
+
+template <typename R>
+void Multigrid<R>::restrict_(int l, const cplx<R>* in, cplx<R>* out, uint64_t mask) {
+    const LevelGeom& G = g_[l];
+    dd::restrict_<R>(s_, W(l), in, out, G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, mask_count(mask), mask, G.n(),
+                     g_[l + 1].n());
+}
+
+template <typename R>
+void Multigrid<R>::prolong(int l, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, uint64_t mask) {
+    const LevelGeom& G = g_[l];
+    dd::prolong<R>(s_, W(l), vc, y, out, G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, mask_count(mask), mask, G.n(),
+                   g_[l + 1].n());
+}
+
+template <typename R>
+void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, uint64_t mask) {
+    const LevelGeom& G = g_[l];
+    const int nr = mask_count(mask);
+    if (nr > cap_[l]) throw BadShape("SAP: batch larger than capacity");
+    cplx<R>* r = sapr_[l].as<cplx<R>>();
+    int* gate = gate_[l].as<int>();
+    cplx<double>* nrm = nrm_[l].as<cplx<double>>();
+    const size_t n = G.n();
+    k_fill_int<<<1, 64, 0, s_>>>(gate, 1, 64);
+    DD_HIP(hipGetLastError());
+    dot_partial<R>(s_, v, v, n, n, nr, mask, rb_[l]);
+    dot_final<R>(s_, rb_[l], nrm + 0, 2, nr, mask);
+    apply(l, x, r, v, mask);
+    for (int i = 0; i < nu; i++) {
+        for (int colour = 0; colour < 2; colour++) {
+            if (l == 0)
+                sap_blocks0<R>(s_, U_, r, x, G.Nx, G.Nt, G.bx, G.bt, colour, p_.m0, p_.sap_m, p_.sap_restarts,
+                               p_.sap_tol, nr, mask, n, gate);
+            else
+                sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, colour, p_.m0, p_.sap_m,
+                               p_.sap_restarts, p_.sap_tol, nr, mask, n, gate);
+            apply(l, x, r, v, mask);
+        }
+        dot_partial<R>(s_, r, r, n, n, nr, mask, rb_[l]);
+        dot_final<R>(s_, rb_[l], nrm + 1, 2, nr, mask);
+        gate_update(s_, gate, nrm + 1, nrm + 0, 2, tol, nr, mask);
+    }
+}
+
+template <typename R>
+void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    const int L = levels();
+    auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
+    typename Krylov<R>::Func fA = A;
+    coarse_->solve(eta, eta, psi, mask, fA, nullptr, iters_scratch_.data(), nullptr);
+    int mx = 0;
+    for (int q = 0; q < mask_count(mask); q++)
+        if ((mask >> q) & 1) mx = std::max(mx, iters_scratch_[q]);
+    coarse_iters.push_back(mx);
+}
+
+template <typename R>
+void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    const int L = levels();
+    if (l == L - 1) {
+        coarsest(eta, psi, mask);
+        return;
+    }
+    if (p_.nu1 > 0) sap(l, eta, psi, p_.nu1, 1e-10, mask);
+    cplx<R>* r = res_[l].as<cplx<R>>();
+    cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
+    cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
+    apply(l, psi, r, eta, mask);
+    restrict_(l, r, e1, mask);
+    typename Krylov<R>::Func fA = [this, l](const cplx<R>* in, cplx<R>* out, uint64_t m) {
+        apply(l + 1, in, out, nullptr, m);
+    };
+    typename Krylov<R>::Prec fM = [this, l](const cplx<R>* in, cplx<R>* io, uint64_t m) { kcycle(l + 1, in, io, m); };
+    std::vector<int> its(64);
+    kry_[l + 1]->solve(e1, e1, p1, mask, fA, &fM, its.data(), nullptr);
+    prolong(l, p1, psi, psi, mask);
+    if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
+}
+
+template <typename R>
+void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    const int L = levels();
+    if (l == L - 1) {
+        coarsest(eta, psi, mask);
+        return;
+    }
+    if (p_.nu1 > 0) sap(l, eta, psi, p_.nu1, 1e-10, mask);
+    cplx<R>* r = res_[l].as<cplx<R>>();
+    cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
+    cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
+    apply(l, psi, r, eta, mask);
+    restrict_(l, r, e1, mask);
+    const size_t n1 = g_[l + 1].n();
+    zero<R>(s_, p1, n1, n1, mask_count(mask), mask);
+    vcycle(l + 1, e1, p1, mask);
+    prolong(l, p1, psi, psi, mask);
+    if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
+}
+
+template <typename R>
+void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
+    const int L = levels();
+    std::vector<DevMem> raw(L);
+    for (int l = 0; l < L - 1; l++) raw[l].alloc(sizeof(cplx<R>) * g_[l].nv * g_[l].n());
+    DD_HIP(hipMemcpyAsync(raw[0].as<void>(), tv0, sizeof(cplx<R>) * g_[0].nv * g_[0].n(), hipMemcpyDeviceToDevice, s_));
+    // v_l = P^dagger_{l-1} v_{l-1} of the raw (unsmoothed) vectors, extra vectors random
+    for (int l = 1; l < L - 1; l++) {
+        const LevelGeom& Gp = g_[l - 1];
+        const LevelGeom& G = g_[l];
+        DevMem Wraw(sizeof(cplx<R>) * Gp.V() * Gp.nv * Gp.dof);
+        transpose_tv<R>(s_, raw[l - 1].as<cplx<R>>(), Wraw.as<cplx<R>>(), Gp.V(), Gp.nv, Gp.dof, true);
+        const int c = std::min(G.nv, Gp.nv);
+        dd::restrict_<R>(s_, Wraw.as<cplx<R>>(), raw[l - 1].as<cplx<R>>(), raw[l].as<cplx<R>>(), Gp.Nx, Gp.Nt,
+                         Gp.dof, Gp.ax, Gp.at, Gp.nv, c, full_mask(c), Gp.n(), G.n());
+        if (G.nv > Gp.nv) {
+            const int extra = G.nv - Gp.nv;
+            std::vector<std::complex<double>> h((size_t)G.nv * G.n());
+            rng.fill(h.data(), Gp.nv, extra, G.Nx, G.Nt, G.dof);
+            DevMem tmp(sizeof(cplx<double>) * extra * G.n());
+            DD_HIP(hipMemcpyAsync(tmp.as<void>(), h.data() + (size_t)Gp.nv * G.n(),
+                                  sizeof(cplx<double>) * extra * G.n(), hipMemcpyHostToDevice, s_));
+            convert<R, double>(s_, raw[l].as<cplx<R>>() + (size_t)Gp.nv * G.n(), tmp.as<cplx<double>>(),
+                               G.n() * extra, 0, 0, 1, 1);
+            DD_HIP(hipStreamSynchronize(s_));
+        }
+        DD_HIP(hipStreamSynchronize(s_));
+    }
+    DevMem L0(sizeof(cplx<R>) * (size_t)g_[0].V() * 5 * 4);
+    dirac_links<R>(s_, U_, L0.as<cplx<R>>(), g_[0].Nx, g_[0].Nt);
+    for (int l = 0; l < L - 1; l++) {
+        const LevelGeom& G = g_[l];
+        DevMem zero_rhs(sizeof(cplx<R>) * G.nv * G.n());
+        zero<R>(s_, zero_rhs.as<cplx<R>>(), G.n() * G.nv, 0, 1, 1);
+        // SAP(0, w, setup iterations) on all test vectors of the level at once
+        sap(l, zero_rhs.as<cplx<R>>(), raw[l].as<cplx<R>>(), p_.setup_sap_iters, 1e-10, full_mask(G.nv));
+        transpose_tv<R>(s_, raw[l].as<cplx<R>>(), W_[l].as<cplx<R>>(), G.V(), G.nv, G.dof, true);
+        orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
+        const cplx<R>* Lf = (l == 0) ? L0.as<cplx<R>>() : links(l);
+        galerkin<R>(s_, Lf, W(l), links_[l + 1].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, l > 0);
+        DD_HIP(hipStreamSynchronize(s_));
+    }
+}
+
+template class Krylov<double>;
+template class Krylov<float>;
+template class Multigrid<double>;
+template class Multigrid<float>;
+
+// ---------------------------------------------------------------------------------------
+// Top-level solver
+// ---------------------------------------------------------------------------------------
+Solver::Solver(int nx, int nt, int dev) : Nx(nx), Nt(nt), device(dev) {
+    if (Nx < 2 || Nt < 2) throw BadShape("lattice too small");
+    DD_HIP(hipSetDevice(device));
+    DD_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    U64.alloc(sizeof(cplx<double>) * (size_t)Nx * Nt * 2);
+    U32.alloc(sizeof(cplx<float>) * (size_t)Nx * Nt * 2);
+    theta.alloc(sizeof(double) * (size_t)Nx * Nt * 2);
+}
+
+Solver::~Solver() {
+    mg64.reset();
+    mg32.reset();
+    outer_.reset();
+    if (s) (void)hipStreamDestroy(s);
+}
+
+void Solver::set_gauge(const cplx<double>* U_dev) {
+    const size_t n = (size_t)Nx * Nt * 2;
+    DD_HIP(hipMemcpyAsync(U64.as<void>(), U_dev, sizeof(cplx<double>) * n, hipMemcpyDeviceToDevice, s));
+    convert<float, double>(s, U32.as<cplx<float>>(), U64.as<cplx<double>>(), n, 0, 0, 1, 1);
+    DD_HIP(hipStreamSynchronize(s));
+    have_gauge = true;
+    have_setup = false;
+}
+
+void Solver::apply_D(const cplx<double>* in, cplx<double>* out, int nrhs, double m0, bool dagger, int precision) {
+    const size_t n = (size_t)Nx * Nt * 2;
+    if (precision == 0) {
+        wilson<double>(s, U64.as<cplx<double>>(), in, out, nullptr, Nx, Nt, m0, dagger, nrhs, full_mask(nrhs), n);
+    } else {
+        DevMem a(sizeof(cplx<float>) * n * nrhs), b(sizeof(cplx<float>) * n * nrhs);
+        convert<float, double>(s, a.as<cplx<float>>(), in, n, n, n, nrhs, full_mask(nrhs));
+        wilson<float>(s, U32.as<cplx<float>>(), a.as<cplx<float>>(), b.as<cplx<float>>(), nullptr, Nx, Nt, m0, dagger,
+                      nrhs, full_mask(nrhs), n);
+        convert<double, float>(s, out, b.as<cplx<float>>(), n, n, n, nrhs, full_mask(nrhs));
+        DD_HIP(hipStreamSynchronize(s));
+    }
+}
+
+void Solver::setup(const dd_params& p) {
+    if (!have_gauge) throw BadShape("setup before the gauge field");
+    auto t0 = std::chrono::steady_clock::now();
+    prm = p;
+    mg64.reset();
+    mg32.reset();
+    outer_.reset();
+    std::vector<LevelGeom> g = make_geometry(Nx, Nt, p);
+    RankStreams rng(p.rng_ranks_x, p.rng_ranks_t);
+    std::vector<std::complex<double>> h((size_t)g[0].nv * g[0].n());
+    rng.fill(h.data(), 0, g[0].nv, Nx, Nt, 2);
+    DevMem tv(sizeof(cplx<double>) * h.size());
+    DD_HIP(hipMemcpyAsync(tv.as<void>(), h.data(), sizeof(cplx<double>) * h.size(), hipMemcpyHostToDevice, s));
+    if (p.precision == 0) {
+        mg64.reset(new Multigrid<double>(s, Nx, Nt, p, U64.as<cplx<double>>()));
+        mg64->setup(tv.as<cplx<double>>(), rng);
+    } else {
+        DevMem tv32(sizeof(cplx<float>) * h.size());
+        convert<float, double>(s, tv32.as<cplx<float>>(), tv.as<cplx<double>>(), h.size(), 0, 0, 1, 1);
+        mg32.reset(new Multigrid<float>(s, Nx, Nt, p, U32.as<cplx<float>>()));
+        mg32->setup(tv32.as<cplx<float>>(), rng);
+    }
+    DD_HIP(hipStreamSynchronize(s));
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    have_setup = true;
+}
+
+int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info) {
+    if (!have_setup) throw BadShape("solve before setup");
+    if (nrhs < 1 || nrhs > std::max(1, prm.max_rhs)) throw BadShape("nrhs exceeds max_rhs of the setup");
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t n = (size_t)Nx * Nt * 2;
+    const int cap = std::max(1, prm.max_rhs);
+    if (!outer_ || outer_cap_ != cap) {
+        outer_.reset(new Krylov<double>(s, n, prm.fgmres_m, prm.fgmres_restarts, prm.tol, cap, true, prm.ortho_outer));
+        outer_cap_ = cap;
+        if (prm.precision == 1) {
+            x32_.alloc(sizeof(cplx<float>) * n * cap);
+            y32_.alloc(sizeof(cplx<float>) * n * cap);
+        }
+    }
+    const uint64_t mask = full_mask(nrhs);
+    Krylov<double>::Func A = [this, n](const cplx<double>* in, cplx<double>* out, uint64_t m) {
+        wilson<double>(s, U64.as<cplx<double>>(), in, out, nullptr, Nx, Nt, prm.m0, false, mask_count(m), m, n);
+    };
+    Krylov<double>::Prec M = [this, n](const cplx<double>* in, cplx<double>* io, uint64_t m) {
+        const int nr = mask_count(m);
+        if (prm.precision == 0) {
+            zero<double>(s, io, n, n, nr, m);
+            if (prm.cycle == 1) mg64->kcycle(0, in, io, m);
+            else mg64->vcycle(0, in, io, m);
+        } else {
+            cplx<float>* a = x32_.as<cplx<float>>();
+            cplx<float>* y = y32_.as<cplx<float>>();
+            convert<float, double>(s, a, in, n, n, n, nr, m);
+            zero<float>(s, y, n, n, nr, m);
+            if (prm.cycle == 1) mg32->kcycle(0, a, y, m);
+            else mg32->vcycle(0, a, y, m);
+            convert<double, float>(s, io, y, n, n, n, nr, m);
+        }
+    };
+    zero<double>(s, x, n, n, nrhs, mask);
+    std::vector<int> its(64, 0);
+    std::vector<double> rr(64, 0.0);
+    const uint64_t conv = outer_->solve(b, x, x, mask, A, &M, its.data(), rr.data());
+    DD_HIP(hipStreamSynchronize(s));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int mx = 0;
+    double mr = 0;
+    for (int q = 0; q < nrhs; q++) {
+        if (iters) iters[q] = its[q];
+        mx = std::max(mx, its[q]);
+        mr = std::max(mr, rr[q]);
+    }
+    if (info) {
+        info->iterations = mx;
+        info->relres = mr;
+        info->seconds = secs;
+    }
+    return conv == mask ? DD_OK : DD_ERR_NOT_CONVERGED;
+}
+
+}  // namespace dd

@@ -1,0 +1,266 @@
+"""Python binding (ctypes) over the C ABI of libddamg.so (include/ddamg.h).
+
+The binding is a thin shell: every call goes to the HIP library; there is no CPU
+fallback.  If the library is missing the import of this module raises.
+
+Vectors use the reference's interior site order without halos: numpy complex128
+arrays of shape (Nx, Nt, dof) or (nrhs, Nx, Nt, dof) with dof = colour*2 + spin.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get('DDAMG_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libddamg.so'))
+HEADER = os.path.join(os.path.dirname(os.path.dirname(_HERE)), 'include', 'ddamg.h')
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f'libddamg.so not built ({LIB_PATH}); run python ddalphaamg-sm_amd/build.py')
+lib = C.CDLL(LIB_PATH)
+
+DD_OK, DD_ERR_NOT_CONVERGED, DD_ERR_BAD_SHAPE, DD_ERR_HIP, DD_ERR_STATE, DD_ERR_ARG = 0, 1, 2, 3, 4, 5
+MAX_LEVELS = 8
+
+
+class Params(C.Structure):
+    """Mirror of dd_params (include/ddamg.h)."""
+    _fields_ = [('levels', C.c_int),
+                ('agg_x', C.c_int * MAX_LEVELS), ('agg_t', C.c_int * MAX_LEVELS),
+                ('sap_x', C.c_int * MAX_LEVELS), ('sap_t', C.c_int * MAX_LEVELS),
+                ('nv', C.c_int * MAX_LEVELS),
+                ('m0', C.c_double), ('nu1', C.c_int), ('nu2', C.c_int), ('setup_sap_iters', C.c_int),
+                ('sap_m', C.c_int), ('sap_restarts', C.c_int), ('sap_tol', C.c_double),
+                ('coarse_m', C.c_int), ('coarse_restarts', C.c_int), ('coarse_tol', C.c_double),
+                ('kcycle_m', C.c_int), ('kcycle_restarts', C.c_int), ('kcycle_tol', C.c_double),
+                ('fgmres_m', C.c_int), ('fgmres_restarts', C.c_int), ('tol', C.c_double),
+                ('cycle', C.c_int), ('precision', C.c_int),
+                ('rng_ranks_x', C.c_int), ('rng_ranks_t', C.c_int),
+                ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
+                ('max_rhs', C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [('iterations', C.c_int), ('relres', C.c_double), ('seconds', C.c_double)]
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_ctx = C.c_void_p
+
+_SIGS = {
+    'dd_default_params': (C.c_int, [C.POINTER(Params), C.c_int]),
+    'dd_create': (C.c_int, [C.POINTER(_ctx), C.c_int, C.c_int, C.c_int]),
+    'dd_destroy': (C.c_int, [_ctx]),
+    'dd_set_gauge': (C.c_int, [_ctx, _dp]),
+    'dd_set_gauge_theta': (C.c_int, [_ctx, _dp]),
+    'dd_generate_gauge': (C.c_int, [_ctx, C.c_double, C.c_int, C.c_ulonglong, C.c_double, _dp]),
+    'dd_get_gauge': (C.c_int, [_ctx, _dp]),
+    'dd_apply_D': (C.c_int, [_ctx, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    'dd_apply_D_dev': (C.c_int, [_ctx, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int]),
+    'dd_setup': (C.c_int, [_ctx, C.POINTER(Params)]),
+    'dd_setup_seconds': (C.c_double, [_ctx]),
+    'dd_solve': (C.c_int, [_ctx, C.c_int, _dp, _dp, _ip, C.POINTER(SolveInfo)]),
+    'dd_solve_dev': (C.c_int, [_ctx, C.c_int, C.c_void_p, C.c_void_p, _ip, C.POINTER(SolveInfo)]),
+    'dd_solve_adjoint': (C.c_int, [_ctx, C.c_int, _dp, _dp, _ip, C.POINTER(SolveInfo)]),
+    'dd_solve_adjoint_dev': (C.c_int, [_ctx, C.c_int, C.c_void_p, C.c_void_p, _ip, C.POINTER(SolveInfo)]),
+    'dd_dD_dtheta': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double]),
+    'dd_level_apply': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
+    'dd_restrict': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
+    'dd_prolong': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
+    'dd_sap': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    'dd_get_test_vectors': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
+    'dd_get_links': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
+    'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
+    'dd_last_error': (C.c_char_p, []),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class DDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f'{msg} (status {code})')
+        self.code = code
+
+
+def _chk(code, allow=(DD_OK,)):
+    if code not in allow:
+        raise DDError(code, lib.dd_last_error().decode())
+    return code
+
+
+def _cptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def params(levels: int = 2, **kw) -> Params:
+    """dd_params with the reference defaults; per-level lists (agg, sap_block, nv) and
+    scalar knobs as keywords: agg=[(4,4),...], sap_block=[(4,4),...], nv=[8,...]."""
+    p = Params()
+    _chk(lib.dd_default_params(C.byref(p), levels))
+    for l, (ax, at) in enumerate(kw.pop('agg', [])):
+        p.agg_x[l], p.agg_t[l] = ax, at
+    for l, (bx, bt) in enumerate(kw.pop('sap_block', [])):
+        p.sap_x[l], p.sap_t[l] = bx, bt
+    for l, nv in enumerate(kw.pop('nv', [])):
+        p.nv[l] = nv
+    if 'rng_ranks' in kw:
+        p.rng_ranks_x, p.rng_ranks_t = kw.pop('rng_ranks')
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One lattice (Nx x Nt) on one GPU: gauge field, hierarchy and solves."""
+
+    def __init__(self, Nx: int, Nt: int, device: int = 0):
+        self.Nx, self.Nt = Nx, Nt
+        self._h = _ctx()
+        _chk(lib.dd_create(C.byref(self._h), Nx, Nt, device))
+        self.prm: Params | None = None
+
+    def close(self):
+        if self._h:
+            lib.dd_destroy(self._h)
+            self._h = _ctx()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- gauge ------------------------------------------------------------------------
+    def set_gauge(self, U: np.ndarray):
+        U = _c128(U)
+        assert U.shape == (self.Nx, self.Nt, 2)
+        _chk(lib.dd_set_gauge(self._h, _cptr(U)))
+
+    def set_gauge_theta(self, theta: np.ndarray):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        assert th.shape == (self.Nx, self.Nt, 2)
+        _chk(lib.dd_set_gauge_theta(self._h, _cptr(th)))
+
+    def generate_gauge(self, beta: float, sweeps: int, seed: int, step: float = 0.5) -> np.ndarray:
+        th = np.zeros((self.Nx, self.Nt, 2))
+        _chk(lib.dd_generate_gauge(self._h, beta, sweeps, seed, step, _cptr(th)))
+        return th
+
+    def gauge(self) -> np.ndarray:
+        U = np.zeros((self.Nx, self.Nt, 2), complex)
+        _chk(lib.dd_get_gauge(self._h, _cptr(U)))
+        return U
+
+    # -- operator -----------------------------------------------------------------------
+    def apply_D(self, v: np.ndarray, m0: float, dagger: bool = False, precision: int = 0) -> np.ndarray:
+        v = _c128(v)
+        batch = v.ndim == 4
+        vb = v if batch else v[None]
+        out = np.zeros_like(vb)
+        _chk(lib.dd_apply_D(self._h, _cptr(vb), _cptr(out), vb.shape[0], m0, int(dagger), precision))
+        return out if batch else out[0]
+
+    # -- hierarchy and solves -----------------------------------------------------------------
+    def setup(self, prm: Params) -> float:
+        _chk(lib.dd_setup(self._h, C.byref(prm)))
+        self.prm = prm
+        return lib.dd_setup_seconds(self._h)
+
+    def solve(self, b: np.ndarray, adjoint: bool = False, check: bool = True):
+        """Returns (x, iterations per system, info)."""
+        b = _c128(b)
+        batch = b.ndim == 4
+        bb = b if batch else b[None]
+        nr = bb.shape[0]
+        x = np.zeros_like(bb)
+        its = (C.c_int * nr)()
+        info = SolveInfo()
+        f = lib.dd_solve_adjoint if adjoint else lib.dd_solve
+        code = f(self._h, nr, _cptr(bb), _cptr(x), its, C.byref(info))
+        _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
+        its = np.array(list(its))
+        return (x if batch else x[0]), its, info
+
+    def dD_dtheta(self, y: np.ndarray, x: np.ndarray, scale: float = -1.0) -> np.ndarray:
+        y, x = _c128(y), _c128(x)
+        yb = y if y.ndim == 4 else y[None]
+        xb = x if x.ndim == 4 else x[None]
+        g = np.zeros((self.Nx, self.Nt, 2))
+        _chk(lib.dd_dD_dtheta(self._h, yb.shape[0], _cptr(yb), _cptr(xb), _cptr(g), scale))
+        return g
+
+    def level_shape(self, level: int):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _chk(lib.dd_level_shape(self._h, level, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def _level_op(self, fn, level, v, out_level, precision, *extra):
+        v = _c128(v)
+        batch = v.ndim == 4
+        vb = v if batch else v[None]
+        out = np.zeros((vb.shape[0],) + self.level_shape(out_level), complex)
+        _chk(fn(self._h, level, _cptr(vb), _cptr(out), *extra, vb.shape[0], precision))
+        return out if batch else out[0]
+
+    def level_apply(self, level, v, precision=0):
+        return self._level_op(lib.dd_level_apply, level, v, level, precision)
+
+    def restrict(self, level, v, precision=0):
+        return self._level_op(lib.dd_restrict, level, v, level + 1, precision)
+
+    def prolong(self, level, v, precision=0):
+        """P v for a level+1 vector v; the result lives on `level`."""
+        v = _c128(v)
+        batch = v.ndim == 4
+        vb = v if batch else v[None]
+        out = np.zeros((vb.shape[0],) + self.level_shape(level), complex)
+        _chk(lib.dd_prolong(self._h, level, _cptr(vb), _cptr(out), vb.shape[0], precision))
+        return out if batch else out[0]
+
+    def sap(self, level, v, x0, nu, tol=1e-10, precision=0):
+        v, x = _c128(v), _c128(x0).copy()
+        batch = v.ndim == 4
+        vb = v if batch else v[None]
+        xb = x if batch else x[None]
+        _chk(lib.dd_sap(self._h, level, _cptr(vb), _cptr(xb), nu, tol, vb.shape[0], precision))
+        return xb if batch else xb[0]
+
+    def test_vectors(self, level, precision=0):
+        Nx, Nt, d = self.level_shape(level)
+        nv = self.prm.nv[level]
+        out = np.zeros((nv, Nx, Nt, d), complex)
+        _chk(lib.dd_get_test_vectors(self._h, level, _cptr(out), precision))
+        return out
+
+    def links(self, level, precision=0):
+        """(A, B[2], C[2]) of level >= 1, each (Nx, Nt, dof, dof)."""
+        Nx, Nt, d = self.level_shape(level)
+        out = np.zeros((Nx, Nt, 5, d, d), complex)
+        _chk(lib.dd_get_links(self._h, level, _cptr(out), precision))
+        return out[:, :, 0], np.stack([out[:, :, 1], out[:, :, 2]]), np.stack([out[:, :, 3], out[:, :, 4]])
+
+
+def header_symbols(path: str = HEADER) -> list[str]:
+    """Function names declared in include/ddamg.h."""
+    import re
+    with open(path) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r'^\s*(?:int|double|const char\*)\s+(dd_\w+)\s*\(', txt, re.M)))

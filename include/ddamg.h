@@ -1,0 +1,146 @@
+/* C ABI of the MI355X DD-alphaAMG library (libddamg.so).
+ *
+ * The boundary sits under the reference's C++ host API: each entry point below
+ * replaces the reference function or class method cited next to it (paths relative to
+ * the reference root).  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * Data format at the boundary: complex128 (interleaved re, im) vectors in the
+ * reference's interior site order without halos, x outer, t inner, then the degree of
+ * freedom (dof = colour*2 + spin):  v[((x*Nt + t)*dof + d)*2 + {0:re, 1:im}].
+ * A batch of nrhs vectors is stored back to back.  The gauge field is U_mu(x) with
+ * mu = 0 the t direction: U[((x*Nt + t)*2 + mu)*2 + {re, im}].
+ * Pointers are host pointers unless the entry point's name ends in _dev (device
+ * pointers on the context's device, e.g. torch tensors).
+ *
+ * Every function returns a status code (DD_OK = 0).
+ */
+#ifndef DDAMG_H
+#define DDAMG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DD_OK 0
+#define DD_ERR_NOT_CONVERGED 1 /* solve ended without reaching the tolerance */
+#define DD_ERR_BAD_SHAPE 2     /* sizes that do not divide / unsupported shapes */
+#define DD_ERR_HIP 3           /* HIP runtime error (details on stderr) */
+#define DD_ERR_STATE 4         /* call out of order (e.g. solve before setup) */
+#define DD_ERR_ARG 5           /* null pointer or invalid argument */
+
+#define DD_MAX_LEVELS 8
+
+typedef struct dd_ctx dd_ctx;
+
+/* Solver parameters.  Defaults (dd_default_params) are the reference's hard-coded
+ * knobs, src/variables.cpp:90-134, and its per-level parameters file,
+ * include/params.h:7-57 (here in SITES per aggregate / SAP block, not block counts). */
+typedef struct dd_params {
+    int levels;                    /* number of levels L (>= 2)                     */
+    int agg_x[DD_MAX_LEVELS];      /* aggregate size in sites at level l < L-1      */
+    int agg_t[DD_MAX_LEVELS];
+    int sap_x[DD_MAX_LEVELS];      /* SAP block size in sites at level l < L-1      */
+    int sap_t[DD_MAX_LEVELS];
+    int nv[DD_MAX_LEVELS];         /* test vectors building level l+1               */
+    double m0;                     /* bare mass                                     */
+    int nu1, nu2;                  /* pre/post smoothing SAP iterations (0, 2)      */
+    int setup_sap_iters;           /* SAP iterations smoothing test vectors (4)     */
+    int sap_m, sap_restarts;       /* block GMRES restart length / restarts (5, 5)  */
+    double sap_tol;                /* block GMRES tolerance (1e-3)                  */
+    int coarse_m, coarse_restarts; /* coarsest GMRES (400, 10)                      */
+    double coarse_tol;             /* (0.1)                                         */
+    int kcycle_m, kcycle_restarts; /* K-cycle FGMRES (5, 2)                         */
+    double kcycle_tol;             /* (0.1)                                         */
+    int fgmres_m, fgmres_restarts; /* outer FGMRES (20, 50)                         */
+    double tol;                    /* outer tolerance (1e-10)                       */
+    int cycle;                     /* 0 = V-cycle, 1 = K-cycle                      */
+    int precision;                 /* 0 = FP64 throughout, 1 = FP32 hierarchy inside
+                                      the FP64 outer FGMRES (mixed)                 */
+    int rng_ranks_x, rng_ranks_t;  /* virtual MPI grid whose per-rank std::mt19937
+                                      streams draw the level-0 test vectors (2, 2)  */
+    int ortho_outer;               /* 0 = modified Gram-Schmidt (reference), 1 = CGS2 */
+    int ortho_inner;               /* same, for the K-cycle FGMRES                  */
+    int ortho_coarse;              /* same, for the coarsest GMRES (default 1)      */
+    int max_rhs;                   /* largest batch solve() will be asked for (1)   */
+} dd_params;
+
+typedef struct dd_solve_info {
+    int iterations;     /* reference count convention k*m + maxIt (max over the batch) */
+    double relres;      /* max over the batch of ||b - D x|| / ||b|| (FP64 D)          */
+    double seconds;     /* wall time of the solve                                      */
+} dd_solve_info;
+
+/* Fills p with the reference defaults for a `levels`-level hierarchy with 4x4
+ * aggregates, 4x4 SAP blocks and Nv = 8. */
+int dd_default_params(dd_params* p, int levels);
+
+/* Context for an Nx x Nt lattice on HIP device `device`.  Replaces the global set-up
+ * of src/main.cpp:57-72 (allocate_lattice_arrays, initializeMPI, boundaries). */
+int dd_create(dd_ctx** ctx, int Nx, int Nt, int device);
+int dd_destroy(dd_ctx* ctx);
+
+/* Gauge field (complex128, layout above).  Replaces read_binary of the configuration
+ * src/io.cpp:4-55 followed by the halo set-up of Level::makeDirac src/level.cpp:286-331. */
+int dd_set_gauge(dd_ctx* ctx, const double* U);
+/* Link phases theta_mu(x) (float64 [Nx][Nt][2]); U = exp(i theta). */
+int dd_set_gauge_theta(dd_ctx* ctx, const double* theta);
+/* Seeded quenched Metropolis field at coupling beta (cold start, `sweeps` sweeps,
+ * proposal step `step` rad), generated on the device and installed as the gauge
+ * field; theta_out (nullable) receives the phases.  No reference counterpart: the
+ * reference reads external configurations (src/tests.cpp:556-566). */
+int dd_generate_gauge(dd_ctx* ctx, double beta, int sweeps, unsigned long long seed, double step,
+                      double* theta_out);
+int dd_get_gauge(dd_ctx* ctx, double* U);
+
+/* out = D in (dagger = 0) or D^dagger in (dagger = 1), FP64 (precision 0) or FP32
+ * arithmetic (precision 1), for nrhs level-0 vectors.
+ * Replaces D_phi / D_dagger_phi, src/dirac_operator.cpp:24-85. */
+int dd_apply_D(dd_ctx* ctx, const double* in, double* out, int nrhs, double m0, int dagger, int precision);
+int dd_apply_D_dev(dd_ctx* ctx, const double* in, double* out, int nrhs, double m0, int dagger, int precision);
+
+/* Builds the hierarchy: AlgebraicMG(U, m0, nu1, nu2) + AlgebraicMG::setUpPhase,
+ * include/amg.h:96-114, src/amg.cpp:4-62 (and the FGMRES_AMG constructor that runs it,
+ * include/amg.h:150-165). */
+int dd_setup(dd_ctx* ctx, const dd_params* p);
+double dd_setup_seconds(dd_ctx* ctx);
+
+/* Solves D x = b for nrhs right-hand sides (x0 = 0) with FGMRES preconditioned by the
+ * K- or V-cycle.  Replaces FGMRES_AMG_{k,v}_cycle::fgmres, include/amg.h:240-270 and
+ * src/fgmres.cpp:5-97.  iters (nullable, nrhs ints) receives each system's count.
+ * Returns DD_ERR_NOT_CONVERGED if any system missed the tolerance. */
+int dd_solve(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
+int dd_solve_dev(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
+
+/* Adjoint solve D^dagger y = eta on the same hierarchy through sigma3-hermiticity:
+ * y = sigma3 D^-1 sigma3 eta.  The reference has D_dagger_phi only
+ * (src/dirac_operator.cpp:56-85); no adjoint solver. */
+int dd_solve_adjoint(dd_ctx* ctx, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info);
+int dd_solve_adjoint_dev(dd_ctx* ctx, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info);
+
+/* grad[x][t][mu] += scale * sum_r Re[ y_r^H (dD/dtheta_mu(x)) x_r ] for U = e^{i theta};
+ * with scale = -1 this is dL/dtheta of L = Re<eta, D^-1 b> for y = D^-dagger eta,
+ * x = D^-1 b.  No reference counterpart. */
+int dd_dD_dtheta(dd_ctx* ctx, int nrhs, const double* y, const double* x, double* grad, double scale);
+
+/* ---- per-component entry points (tests, benchmarks) ---------------------------------
+ * Level-l vectors have dof = 2 (l = 0) or 2*nv[l-1]. */
+/* Level::D_operator, src/level.cpp:394-426 (level 0: the Wilson stencil). */
+int dd_level_apply(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, int precision);
+/* Level::Pdagg_v / Level::P_vc, src/level.cpp:145-189 / :92-141 (level -> level+1). */
+int dd_restrict(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, int precision);
+int dd_prolong(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, int precision);
+/* SAP_C::SAP(v, x, nu, tol), src/sap.cpp:81-165, on level `level` (x in/out). */
+int dd_sap(dd_ctx* ctx, int level, const double* v, double* x, int nu, double tol, int nrhs, int precision);
+/* Test vectors of level l (after setup): out[nv][V][dof]. */
+int dd_get_test_vectors(dd_ctx* ctx, int level, double* out, int precision);
+/* Links of level l >= 1 as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1). */
+int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
+/* Shape of level l: Nx, Nt, dof. */
+int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);
+/* Last error message (thread-unsafe, per process). */
+const char* dd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+"""GPU parity of the HIP path (through the C ABI) against the reference's golden vectors
+and the numpy oracle.  Tolerances: FP64 operators 1e-14 relative (the north-star bar for
+D_W), hierarchy objects 1e-11 (they pass through 4 SAP iterations of inexact block
+GMRES), solves 1e-10 relative residual with the FP64 D_W and the reference's iteration
+counts."""
+import numpy as np
+import pytest
+
+from oracle import ddamg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+ddamg = pytest.importorskip('ddamg')
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _ctx_from_golden(meta, a):
+    N = meta['N']
+    ctx = ddamg.Context(N, N)
+    ctx.set_gauge(np.exp(1j * a['theta']))
+    return ctx
+
+
+def _prm(meta, **kw):
+    lv = meta['levels']
+    return ddamg.params(len(lv) + 1, agg=[tuple(l['agg']) for l in lv],
+                        sap_block=[tuple(l['sap_block']) for l in lv], nv=[l['nv'] for l in lv],
+                        m0=meta['m0'], rng_ranks=tuple(meta['ranks']), **kw)
+
+
+# ------------------------------------------------------------------------ operator
+def test_dirac_golden(golden):
+    meta, a = golden('g16_dirac')
+    with _ctx_from_golden(meta, a) as ctx:
+        assert rel(ctx.apply_D(a['rhs'], meta['m0']), a['Dphi']) < 1e-14
+        assert rel(ctx.apply_D(a['rhs'], meta['m0'], dagger=True), a['Ddagphi']) < 1e-14
+        assert rel(ctx.apply_D(a['rhs'], meta['m0'], precision=1), a['Dphi']) < 1e-6
+
+
+def test_dirac_oracle_nonsquare_batch():
+    rng = np.random.default_rng(11)
+    Nx, Nt, m0 = 24, 40, -0.37
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (Nx, Nt, 2)))
+    v = rng.standard_normal((3, Nx, Nt, 2)) + 1j * rng.standard_normal((3, Nx, Nt, 2))
+    with ddamg.Context(Nx, Nt) as ctx:
+        ctx.set_gauge(U)
+        out = ctx.apply_D(v, m0)
+        outd = ctx.apply_D(v, m0, dagger=True)
+    for r in range(3):
+        assert rel(out[r], orc.D_phi(U, v[r], m0)) < 1e-14
+        assert rel(outd[r], orc.D_dagger_phi(U, v[r], m0)) < 1e-14
+
+
+# ------------------------------------------------------------------------ hierarchy
+@pytest.mark.parametrize('name', ['g16_setup2', 'g16_setup3'])
+def test_setup_golden(golden, name):
+    meta, a = golden(name)
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta))
+        assert rel(ctx.test_vectors(0), a['tvec_l0']) < 1e-11
+        if 'tvec_l1' in a:
+            assert rel(ctx.test_vectors(1), a['tvec_l1']) < 1e-11
+        L = len(meta['levels'])
+        st = orc.ref_links_to_stencil(a[f'G1_l{L}'], a[f'G2_l{L}'], a[f'G3_l{L}'], meta['m0'])
+        A, B, Cm = ctx.links(L)
+        assert rel(A, st.A) < 1e-11 and rel(B, st.B) < 1e-11 and rel(Cm, st.C) < 1e-11
+        vc = ctx.restrict(0, a['rhs'])
+        assert rel(vc, a['Pdagg_rhs']) < 1e-11
+        assert rel(ctx.level_apply(1, vc), a['Dop1_Pdagg_rhs']) < 1e-11
+        assert rel(ctx.prolong(0, vc), a['P_Pdagg_rhs']) < 1e-11
+        assert rel(ctx.level_apply(0, a['rhs']), a['Dop0'] if 'Dop0' in a else orc.D_phi(
+            np.exp(1j * a['theta']), a['rhs'], meta['m0'])) < 1e-14
+        s0 = ctx.sap(0, a['rhs'], np.zeros_like(a['rhs']), meta['sap_nu'])
+        assert rel(s0, a['SAP0_rhs']) < 1e-11
+        if 'SAP1_Pdagg_rhs' in a:
+            s1 = ctx.sap(1, vc, np.zeros_like(vc), meta['sap_nu'])
+            assert rel(s1, a['SAP1_Pdagg_rhs']) < 1e-11
+
+
+def test_galerkin_and_orthonormality(golden):
+    """P^dagger P = 1 and P^dagger D_l P = D_{l+1} on the device hierarchy."""
+    meta, a = golden('g16_setup3')
+    rng = np.random.default_rng(2)
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta))
+        for l in range(len(meta['levels'])):
+            shape = ctx.level_shape(l + 1)
+            vc = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            Pv = ctx.prolong(l, vc)
+            assert rel(ctx.restrict(l, Pv), vc) < 1e-13
+            assert rel(ctx.restrict(l, ctx.level_apply(l, Pv)), ctx.level_apply(l + 1, vc)) < 1e-12
+
+
+# ------------------------------------------------------------------------ solves
+@pytest.mark.parametrize('cycle', ['K', 'V'])
+def test_solve_golden(golden, cycle):
+    meta, a = golden('g64_solve')
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'], cycle=1 if cycle == 'K' else 0))
+        x, its, info = ctx.solve(a['rhs'])
+        U = np.exp(1j * a['theta'])
+        rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, meta['m0'])) / np.linalg.norm(a['rhs'])
+        assert rr < 1e-10
+        assert int(its[0]) == meta[f'{cycle}_iterations']
+        assert info.relres < 1e-10
+        if cycle == 'K':
+            assert rel(x, a['x_K']) < 1e-9
+
+
+def test_solve_mixed_precision(golden):
+    meta, a = golden('g64_solve')
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'], cycle=1, precision=1))
+        x, its, info = ctx.solve(a['rhs'])
+        U = np.exp(1j * a['theta'])
+        rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, meta['m0'])) / np.linalg.norm(a['rhs'])
+        assert rr < 1e-10
+        # error against the FP64 solve of the same system (the reference's x)
+        assert rel(x, a['x_K']) < 1e-8
+        assert its[0] <= meta['K_iterations'] + 3
+
+
+def test_batched_equals_single(golden):
+    meta, a = golden('g64_solve')
+    rng = np.random.default_rng(3)
+    B = np.stack([a['rhs']] + [rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape)
+                               for _ in range(3)])
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'], cycle=1, max_rhs=4))
+        xb, itb, _ = ctx.solve(B)
+        for r in range(4):
+            x1, it1, _ = ctx.solve(B[r])
+            assert int(it1[0]) == int(itb[r])
+            assert rel(xb[r], x1) < 1e-12
+        assert int(itb[0]) == meta['K_iterations']
+
+
+def test_adjoint_solve(golden):
+    meta, a = golden('g64_solve')
+    U = np.exp(1j * a['theta'])
+    with _ctx_from_golden(meta, a) as ctx:
+        ctx.setup(_prm(meta, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'], cycle=1))
+        y, its, info = ctx.solve(a['rhs'], adjoint=True)
+        rr = np.linalg.norm(a['rhs'] - orc.D_dagger_phi(U, y, meta['m0'])) / np.linalg.norm(a['rhs'])
+        assert rr < 1e-10
+
+
+def test_gauge_generator_plaquette():
+    from scipy.special import iv
+    with ddamg.Context(64, 64) as ctx:
+        th = ctx.generate_gauge(2.0, 500, seed=5)
+        assert np.allclose(np.exp(1j * th), ctx.gauge())
+    pl = np.mean(np.cos(orc.plaquette_angle(th)))
+    assert abs(pl - iv(1, 2.0) / iv(0, 2.0)) < 0.01
