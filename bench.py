@@ -1,0 +1,207 @@
+"""Benchmark of the DD-alphaAMG solve on one MI355X.
+
+Metric: ms per right-hand side to reach 1e-10 relative residual (true FP64 D_W
+residual) at 512^2, beta=2, m0=-0.18, 3 levels, 4x4 aggregates, Nv=12, 4x4 SAP
+blocks, K-cycle, outer FGMRES(30).  The measured form is the mixed-precision batched
+solve (FP32 hierarchy inside the FP64 outer FGMRES, a batch of RHS per launch); the
+plain FP64 single-RHS solve of the same system is timed beside it.
+
+A step is one batched solve of --nrhs Gaussian right-hand sides already resident in
+HBM.  Setup (test vectors + Galerkin operators) is done once before the timed region
+and reported separately.  The CPU baseline is the reference's own CPU solve
+(oracle/_ref, built from the reference sources) under mpirun on the host cores, on the
+same gauge field and right-hand side 0.
+
+    python bench.py [--steps K] [--warmup W] [--N 512] [--nrhs 32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, 'ddalphaamg-sm_amd'))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
+    return ddamg.params(3, agg=[(4, 4), (4, 4)], sap_block=[(4, 4), (4, 4)], nv=[12, 12], m0=m0, nu2=2,
+                        fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--N', type=int, default=512)
+    ap.add_argument('--nrhs', type=int, default=32)
+    ap.add_argument('--sweeps', type=int, default=2000)
+    ap.add_argument('--seed', type=int, default=2024)
+    ap.add_argument('--cpu-baseline', choices=['reference', 'none'], default='reference')
+    ap.add_argument('--cpu-ranks', type=str, default='4x4')
+    ap.add_argument('--skip-plain', action='store_true')
+    ap.add_argument('--skip-mixed', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU only (the multi-GPU domain decomposition is '
+              'design-only); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import ddamg
+
+    N, nrhs = args.N, args.nrhs
+    m0 = -0.18
+    dev = torch.device('cuda:0')
+    # gauge field: seeded quenched Metropolis on the device (beta = 2, cold start)
+    ctx = ddamg.Context(N, N)
+    t0 = time.perf_counter()
+    theta = ctx.generate_gauge(2.0, args.sweeps, args.seed)
+    log(f'gauge {N}^2 {args.sweeps} sweeps: {time.perf_counter() - t0:.2f}s, '
+        f'plaquette {np.mean(np.cos(theta[..., 0] + np.roll(theta[..., 1], -1, 1) - np.roll(theta[..., 0], -1, 0) - theta[..., 1])):.4f}')
+    rng = np.random.default_rng(args.seed + 1)
+    B = rng.standard_normal((nrhs, N, N, 2)) + 1j * rng.standard_normal((nrhs, N, N, 2))
+    b_dev = torch.from_numpy(B.view(np.float64).copy()).to(dev)
+    x_dev = torch.zeros_like(b_dev)
+    torch.cuda.synchronize()
+
+    # ---- mixed precision, batched -----------------------------------------------------
+    if args.skip_mixed:
+        nrhs = 1
+    setup_mixed = ctx.setup(hierarchy_params(ddamg, N, 0 if args.skip_mixed else 1, nrhs, m0))
+    log(f'setup mixed: {setup_mixed:.3f}s')
+    # warm-up solves with every kernel family timed: per-family breakdown
+    fams = {ddamg.K_WILSON: 'wilson', ddamg.K_COARSE: 'coarse_stencil', ddamg.K_SAP0: 'sap_level0',
+            ddamg.K_SAPC: 'sap_coarse', ddamg.K_MULTIDOT: 'gs_multidot', ddamg.K_MULTIAXPY: 'gs_multiaxpy',
+            ddamg.K_RESTRICT: 'restrict', ddamg.K_PROLONG: 'prolong', ddamg.K_COARSEST: 'coarsest_gmres',
+            ddamg.K_DOT: 'dot'}
+    breakdown = {}
+    if args.warmup > 0:
+        for k in fams:
+            ddamg.timer_enable(k, True)
+        ddamg.timer_reset()
+        for _ in range(args.warmup):
+            its, info = ctx.solve_dev(b_dev.data_ptr(), x_dev.data_ptr(), nrhs)
+        for k, name in fams.items():
+            ms, n, by = ddamg.timer_read(k)
+            if n:
+                breakdown[name] = {'ms_per_solve': ms / args.warmup, 'launches': n // args.warmup,
+                                   'GB_per_s': by / (ms * 1e-3) / 1e9 if ms > 0 else None}
+            ddamg.timer_enable(k, False)
+        log('kernel breakdown (warm-up, per solve): ' + json.dumps(breakdown))
+    dom = max(breakdown, key=lambda k: breakdown[k]['ms_per_solve']) if breakdown else 'sap_coarse'
+    dom_id = {v: k for k, v in fams.items()}[dom]
+    torch.cuda.synchronize()
+    ddamg.timer_enable(dom_id, True)
+    ddamg.timer_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        its, info = ctx.solve_dev(b_dev.data_ptr(), x_dev.data_ptr(), nrhs)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    dom_ms, dom_n, dom_bytes = ddamg.timer_read(dom_id)
+    ddamg.timer_enable(dom_id, False)
+    peak = 8000.0
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': peak, 'unit': 'GB/s',
+                'frac': achieved / peak, 'traffic': None,
+                'avg_launch_us': dom_ms * 1e3 / max(dom_n, 1), 'launches': dom_n,
+                'algorithmic_bytes_per_launch': dom_bytes / max(dom_n, 1)}
+    log(f'roofline {dom}: {achieved:.1f} GB/s over {dom_n} launches')
+    x_mixed = x_dev.cpu().numpy().view(np.complex128).reshape(nrhs, N, N, 2)
+    # true FP64 residual of every system
+    r_dev = torch.zeros_like(b_dev)
+    ctx.apply_D_dev(x_dev.data_ptr(), r_dev.data_ptr(), nrhs, m0)
+    torch.cuda.synchronize()
+    Dx = r_dev.cpu().numpy().view(np.complex128).reshape(nrhs, N, N, 2)
+    relres = np.linalg.norm((B - Dx).reshape(nrhs, -1), axis=1) / np.linalg.norm(B.reshape(nrhs, -1), axis=1)
+    log(f'mixed batched: {t_mixed * 1e3:.1f} ms per {nrhs}-RHS solve, its {its.min()}..{its.max()}, '
+        f'max relres {relres.max():.3e}')
+    ctx.close()
+
+    # ---- plain FP64, one RHS -------------------------------------------------------------
+    t_plain, its_plain, err_vs_fp64, setup_plain = None, None, None, None
+    if not args.skip_plain:
+        ctx = ddamg.Context(N, N)
+        ctx.set_gauge_theta(theta)
+        setup_plain = ctx.setup(hierarchy_params(ddamg, N, 0, 1, m0))
+        xp = torch.zeros_like(b_dev[:1])
+        if not args.skip_mixed:
+            ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its_plain, info_p = ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        x64 = xp.cpu().numpy().view(np.complex128).reshape(N, N, 2)
+        err_vs_fp64 = float(np.linalg.norm(x_mixed[0] - x64) / np.linalg.norm(x64))
+        log(f'plain FP64: setup {setup_plain:.3f}s, solve {t_plain * 1e3:.1f} ms, its {its_plain[0]}, '
+            f'relres {info_p.relres:.3e}, mixed-vs-FP64 rel. error {err_vs_fp64:.3e}')
+        ctx.close()
+
+    # ---- CPU baseline: the reference's own solve on the host cores --------------------------
+    cpu = None
+    if args.cpu_baseline == 'reference':
+        cpu = cpu_reference(theta, B[0], N, m0, args.cpu_ranks)
+
+    ms_rhs = t_mixed * 1e3 / nrhs
+    line = {
+        'metric': 'ms/RHS to 1e-10 rel. residual at 512^2, mixed+batched vs plain FP64 on GPU',
+        'value': ms_rhs, 'unit': 'ms/RHS', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': t_mixed * 1e3, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 hierarchy / f64 outer (complex)', 'data': 'synthetic (seeded Metropolis gauge field, Gaussian RHS)',
+        'config': {'workload': f'DD-alphaAMG K-cycle solve, {N}^2 Schwinger Wilson-Dirac, beta=2, m0={m0}, '
+                               f'3 levels, 4x4 aggregates, Nv=12, 4x4 SAP blocks, FGMRES(30), tol 1e-10, '
+                               f'{nrhs} RHS per solve', 'lattice': [N, N], 'nrhs': nrhs},
+        'outer_iterations': [int(its.min()), int(its.max())],
+        'max_relres': float(relres.max()),
+        'setup_s_mixed': setup_mixed,
+        'plain_fp64_ms_per_rhs': None if t_plain is None else t_plain * 1e3,
+        'plain_fp64_iterations': None if its_plain is None else int(its_plain[0]),
+        'setup_s_fp64': setup_plain,
+        'speedup_vs_plain_fp64': None if t_plain is None else t_plain * 1e3 / ms_rhs,
+        'mixed_vs_fp64_rel_error': err_vs_fp64,
+        'cpu_baseline': cpu,
+        'roofline': roofline,
+        'kernel_breakdown_ms_per_solve': breakdown,
+    }
+    print(json.dumps(line))
+
+
+def cpu_reference(theta, b, N, m0, ranks_s):
+    """The reference's FGMRES + K-cycle solve (oracle/_ref, compiled from the reference
+    sources) under mpirun, same field, RHS 0 and hierarchy parameters."""
+    try:
+        from oracle import ddamg_oracle as orc
+        from oracle import run_ref
+    except Exception as e:  # noqa: BLE001
+        return {'error': f'oracle import failed: {e}'}
+    if not run_ref.available(N):
+        return {'error': f'oracle/_ref/ddamg_ref_{N} not built'}
+    rx, rt = (int(v) for v in ranks_s.split('x'))
+    levels = [orc.LevelSpec((4, 4), (4, 4), 12), orc.LevelSpec((4, 4), (4, 4), 12)]
+    t0 = time.perf_counter()
+    try:
+        wd, _ = run_ref.run('solve', np.exp(1j * theta), b, levels, m0, ranks=(rx, rt), nu2=2, fgmres_m=30,
+                            cycle='K', dump_x=0, timeout=900)
+        res = run_ref.result(wd)
+        run_ref.cleanup(wd)
+    except Exception as e:  # noqa: BLE001
+        return {'error': str(e)[-400:]}
+    log(f'cpu reference: {res} ({time.perf_counter() - t0:.1f}s wall)')
+    return {'value': res['solve_s'] * 1e3, 'unit': 'ms/RHS', 'cores': rx * rt, 'kind': 'reference',
+            'sample': f'one full reference solve (FGMRES(30)+K-cycle, {N}^2, same field and RHS 0) under '
+                      f'mpirun -n {rx * rt} ({rx}x{rt} ranks); setup {res["setup_s"]:.2f}s excluded',
+            'iterations': res['iterations'], 'relres': res['relres'], 'setup_s': res['setup_s']}
+
+
+if __name__ == '__main__':
+    main()

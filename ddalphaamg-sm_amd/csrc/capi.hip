@@ -433,4 +433,28 @@ int dd_get_links(dd_ctx* c, int level, double* out, int precision) {
     });
 }
 
+int dd_timer_enable(int kernel, int on) {
+    if (kernel < 0 || kernel >= dd::K_NUM) return DD_ERR_ARG;
+    dd::timer_enable(kernel, on != 0);
+    return DD_OK;
+}
+
+int dd_timer_reset(void) {
+    dd::timer_reset();
+    return DD_OK;
+}
+
+int dd_timer_read(int kernel, double* total_ms, long long* launches, double* bytes) {
+    if (kernel < 0 || kernel >= dd::K_NUM) return DD_ERR_ARG;
+    return guard([&] {
+        long long c = 0;
+        double by = 0;
+        const double t = dd::timer_read(kernel, &c, &by);
+        if (total_ms) *total_ms = t;
+        if (launches) *launches = c;
+        if (bytes) *bytes = by;
+        return DD_OK;
+    });
+}
+
 }  // extern "C"

@@ -5,6 +5,20 @@
 
 namespace dd {
 
+// ---- live kernel timing (HIP events on the launching stream) -----------------------
+enum KernelId {
+    K_WILSON = 0, K_COARSE, K_SAP0, K_SAPC, K_MULTIDOT, K_MULTIAXPY, K_RESTRICT, K_PROLONG, K_COARSEST,
+    K_DOT, K_NUM
+};
+void timer_enable(int id, bool on);
+void timer_reset();
+void timer_begin(hipStream_t s, int id);
+// `bytes`: algorithmic HBM bytes of the launch (each operand counted once)
+void timer_end(hipStream_t s, int id, double bytes);
+// total elapsed ms, number of timed launches and algorithmic bytes of kernel `id`
+// since the last reset (synchronises the events)
+double timer_read(int id, long long* count, double* bytes);
+
 // Maximum block-GMRES restart length compiled into the SAP block kernels.
 constexpr int kSapM = 5;
 

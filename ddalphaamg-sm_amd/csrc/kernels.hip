@@ -16,12 +16,67 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace dd {
 
 #define DD_LAUNCH_CHECK() DD_HIP(hipGetLastError())
 
+// ---- live kernel timing ------------------------------------------------------------
+namespace {
+struct TimerSlot {
+    bool on = false;
+    std::vector<hipEvent_t> pool;   // start/end pairs
+    size_t used = 0;
+    double bytes = 0;
+};
+TimerSlot g_timers[K_NUM];
+}  // namespace
+
+void timer_enable(int id, bool on) {
+    if (id >= 0 && id < K_NUM) g_timers[id].on = on;
+}
+void timer_reset() {
+    for (auto& t : g_timers) {
+        t.used = 0;
+        t.bytes = 0;
+    }
+}
+void timer_begin(hipStream_t s, int id) {
+    TimerSlot& t = g_timers[id];
+    if (!t.on) return;
+    if (t.used + 2 > t.pool.size()) {
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            DD_HIP(hipEventCreate(&e));
+            t.pool.push_back(e);
+        }
+    }
+    DD_HIP(hipEventRecord(t.pool[t.used], s));
+}
+void timer_end(hipStream_t s, int id, double bytes) {
+    TimerSlot& t = g_timers[id];
+    if (!t.on) return;
+    DD_HIP(hipEventRecord(t.pool[t.used + 1], s));
+    t.used += 2;
+    t.bytes += bytes;
+}
+double timer_read(int id, long long* count, double* bytes) {
+    TimerSlot& t = g_timers[id];
+    double tot = 0;
+    for (size_t k = 0; k + 1 < t.used; k += 2) {
+        DD_HIP(hipEventSynchronize(t.pool[k + 1]));
+        float ms = 0;
+        DD_HIP(hipEventElapsedTime(&ms, t.pool[k], t.pool[k + 1]));
+        tot += ms;
+    }
+    if (count) *count = (long long)(t.used / 2);
+    if (bytes) *bytes = t.bytes;
+    return tot;
+}
+
 static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+static inline double nact(uint64_t mask, int nrhs) { return (double)__builtin_popcountll(mask & ((nrhs >= 64) ? ~0ull : ((1ull << nrhs) - 1))); }
 
 // ======================================================================================
 // Level-0 Wilson-Dirac stencil.  One thread per site and system.
@@ -77,6 +132,7 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U, c
 template <typename R>
 void wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx, int Nt,
             double m0, bool dagger, int nrhs, uint64_t mask, size_t ld) {
+    timer_begin(s, K_WILSON);
     dim3 grid(cdivu((size_t)Nx * Nt, 256), nrhs);
     if (dagger) {
         if (b) k_wilson<R, true, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
@@ -85,6 +141,7 @@ void wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, co
         if (b) k_wilson<R, false, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
         else k_wilson<R, false, false><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
     }
+    timer_end(s, K_WILSON, (double)Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + nact(mask, nrhs) * (b ? 3 : 2)));
     DD_LAUNCH_CHECK();
 }
 
@@ -178,6 +235,7 @@ static int lcm64(int dof) {
 template <typename R>
 void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx,
                   int Nt, int dof, double m0, int nrhs, uint64_t mask, size_t ld) {
+    timer_begin(s, K_COARSE);
     const int threads = lcm64(dof);
     const int spb = threads / dof;
     dim3 grid(cdivu((size_t)Nx * Nt, spb), nrhs);
@@ -193,6 +251,7 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
         default:
             throw BadShape("coarse_apply: unsupported dof");
     }
+    timer_end(s, K_COARSE, (double)Nx * Nt * dof * sizeof(cplx<R>) * (5.0 * dof + nact(mask, nrhs) * (b ? 3 : 2)));
     DD_LAUNCH_CHECK();
 }
 
@@ -374,6 +433,7 @@ template <typename R>
 void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
                  int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
                  const int* gate) {
+    timer_begin(s, K_SAP0);
     const int S = bx * bt;
     if (S > 64 || (S & (S - 1)) != 0) throw BadShape("level-0 SAP block must have a power-of-two site count <= 64");
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
@@ -381,6 +441,7 @@ void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, 
     dim3 grid(cdivu(nblk, 64 / S), nrhs);
     k_sap0<R, kSapM><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld, gate,
                                          nblk);
+    timer_end(s, K_SAP0, 0.5 * Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + 3.0 * nact(mask, nrhs)));
     DD_LAUNCH_CHECK();
 }
 
@@ -467,6 +528,7 @@ template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
                  int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
                  const int* gate) {
+    timer_begin(s, K_SAPC);
     const int S = bx * bt;
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
     const int threads = ((S * dof + 63) / 64) * 64;
@@ -485,6 +547,7 @@ void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, 
         default:
             throw BadShape("sap_blocksc: unsupported dof");
     }
+    timer_end(s, K_SAPC, 0.5 * Nx * Nt * dof * sizeof(cplx<R>) * (5.0 * dof + 3.0 * nact(mask, nrhs)));
     DD_LAUNCH_CHECK();
 }
 
@@ -518,9 +581,11 @@ __global__ void k_restrict(const cplx<R>* __restrict__ W, const cplx<R>* __restr
 template <typename R>
 void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, int Nx, int Nt, int dof, int ax,
                int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
+    timer_begin(s, K_RESTRICT);
     const size_t tot = (size_t)(Nx / ax) * (Nt / at) * 2 * nv;
     dim3 grid(cdivu(tot, 256), nrhs);
     k_restrict<R><<<grid, 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    timer_end(s, K_RESTRICT, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
@@ -546,8 +611,10 @@ __global__ void k_prolong(const cplx<R>* __restrict__ W, const cplx<R>* __restri
 template <typename R>
 void prolong(hipStream_t s, const cplx<R>* W, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, int Nx, int Nt,
              int dof, int ax, int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
+    timer_begin(s, K_PROLONG);
     dim3 grid(cdivu((size_t)Nx * Nt * dof, 256), nrhs);
     k_prolong<R><<<grid, 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    timer_end(s, K_PROLONG, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + (y ? 2 : 1) * nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
@@ -735,9 +802,11 @@ static inline int red_blocks(size_t n) { return (int)std::min<size_t>(256, std::
 template <typename R>
 void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, size_t ld, int nrhs, uint64_t mask,
                  RedBuf& rb) {
+    timer_begin(s, K_DOT);
     rb.nblk = red_blocks(n);
     dim3 grid(rb.nblk, nrhs);
     k_dot_partial<R><<<grid, 256, 0, s>>>(a, b, n, ld, mask, (cplx<double>*)rb.partial);
+    timer_end(s, K_DOT, nact(mask, nrhs) * (double)n * 2 * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
@@ -901,27 +970,29 @@ void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const
 template <typename R>
 __global__ __launch_bounds__(256) void k_multidot(const cplx<R>* __restrict__ w, const cplx<R>* __restrict__ V,
                                                   size_t vstride, int nvec, size_t n, size_t ld, uint64_t mask,
-                                                  cplx<double>* __restrict__ partial) {
+                                                  cplx<double>* __restrict__ partial, int nblk) {
+    // grid: (nblk * nvec, nrhs); block b = (vector i, chunk c)
     __shared__ R scratch[32];
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
+    const int i = blockIdx.x / nblk, c = blockIdx.x - i * nblk;
     const cplx<R>* wr = w + r * ld;
-    for (int i = 0; i < nvec; i++) {
-        const cplx<R>* vr = V + i * vstride + r * ld;
-        cplx<R> s = czero<R>();
-        for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-            s = s + mulc(wr[e], vr[e]);
-        s = block_sum(s, scratch);
-        if (threadIdx.x == 0) partial[((size_t)r * nvec + i) * gridDim.x + blockIdx.x] = mk<double>(s.x, s.y);
-    }
+    const cplx<R>* vr = V + i * vstride + r * ld;
+    cplx<R> s = czero<R>();
+    for (size_t e = (size_t)c * blockDim.x + threadIdx.x; e < n; e += (size_t)nblk * blockDim.x)
+        s = s + mulc(wr[e], vr[e]);
+    s = block_sum(s, scratch);
+    if (threadIdx.x == 0) partial[((size_t)r * nvec + i) * nblk + c] = mk<double>(s.x, s.y);
 }
 
 template <typename R>
 void multidot(hipStream_t s, const cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld,
               int nrhs, uint64_t mask, RedBuf& rb) {
+    timer_begin(s, K_MULTIDOT);
     rb.nblk = red_blocks(n);
-    dim3 grid(rb.nblk, nrhs);
-    k_multidot<R><<<grid, 256, 0, s>>>(w, V, vstride, nvec, n, ld, mask, (cplx<double>*)rb.partial);
+    dim3 grid(rb.nblk * nvec, nrhs);
+    k_multidot<R><<<grid, 256, 0, s>>>(w, V, vstride, nvec, n, ld, mask, (cplx<double>*)rb.partial, rb.nblk);
+    timer_end(s, K_MULTIDOT, nact(mask, nrhs) * (double)n * (nvec + 1) * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
@@ -956,10 +1027,12 @@ __global__ __launch_bounds__(256) void k_multi_axpy(cplx<R>* __restrict__ w, con
 template <typename R>
 void multi_axpy(hipStream_t s, cplx<R>* w, const cplx<R>* V, size_t vstride, int nvec, size_t n, size_t ld, int nrhs,
                 uint64_t mask, const RedBuf& rb, cplx<double>* H, int hstride, bool accumulate) {
+    timer_begin(s, K_MULTIAXPY);
     dim3 grid(ew_blocks(n), nrhs);
     const size_t sh = sizeof(cplx<double>) * nvec;
     k_multi_axpy<R><<<grid, 256, sh, s>>>(w, V, vstride, nvec, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H,
                                           hstride, accumulate ? 1 : 0);
+    timer_end(s, K_MULTIAXPY, nact(mask, nrhs) * (double)n * (nvec + 2) * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 

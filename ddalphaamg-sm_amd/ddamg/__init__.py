@@ -75,7 +75,27 @@ _SIGS = {
     'dd_get_links': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
     'dd_last_error': (C.c_char_p, []),
+    'dd_timer_enable': (C.c_int, [C.c_int, C.c_int]),
+    'dd_timer_reset': (C.c_int, []),
+    'dd_timer_read': (C.c_int, [C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
 }
+K_WILSON, K_COARSE, K_SAP0, K_SAPC, K_MULTIDOT, K_MULTIAXPY, K_RESTRICT, K_PROLONG, K_COARSEST, K_DOT = range(10)
+
+
+def timer_enable(kernel: int, on: bool = True):
+    _chk(lib.dd_timer_enable(kernel, int(on)))
+
+
+def timer_reset():
+    _chk(lib.dd_timer_reset())
+
+
+def timer_read(kernel: int):
+    """(total ms, launches, algorithmic bytes) of the timed launches of `kernel` since
+    the last reset."""
+    t, n, b = C.c_double(), C.c_longlong(), C.c_double()
+    _chk(lib.dd_timer_read(kernel, C.byref(t), C.byref(n), C.byref(b)))
+    return t.value, n.value, b.value
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.restype = _res
@@ -198,6 +218,21 @@ class Context:
         _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
         its = np.array(list(its))
         return (x if batch else x[0]), its, info
+
+    def solve_dev(self, b_ptr: int, x_ptr: int, nrhs: int, adjoint: bool = False, check: bool = True):
+        """Solve with device pointers (complex128 interleaved, nrhs systems back to back),
+        e.g. torch tensors' data_ptr().  Returns (iterations per system, info)."""
+        its = (C.c_int * nrhs)()
+        info = SolveInfo()
+        f = lib.dd_solve_adjoint_dev if adjoint else lib.dd_solve_dev
+        code = f(self._h, nrhs, C.c_void_p(b_ptr), C.c_void_p(x_ptr), its, C.byref(info))
+        _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
+        return np.array(list(its)), info
+
+    def apply_D_dev(self, in_ptr: int, out_ptr: int, nrhs: int, m0: float, dagger: bool = False,
+                    precision: int = 0):
+        _chk(lib.dd_apply_D_dev(self._h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), nrhs, m0, int(dagger),
+                                precision))
 
     def dD_dtheta(self, y: np.ndarray, x: np.ndarray, scale: float = -1.0) -> np.ndarray:
         y, x = _c128(y), _c128(x)

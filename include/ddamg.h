@@ -137,6 +137,27 @@ int dd_get_test_vectors(dd_ctx* ctx, int level, double* out, int precision);
 int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
 /* Shape of level l: Nx, Nt, dof. */
 int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);
+/* ---- live kernel timing -----------------------------------------------------------------
+ * HIP events recorded on the library's stream around every launch of one kernel family,
+ * for benchmarks (no reference counterpart; the reference times whole methods with
+ * MPI_Wtime, src/methods.cpp:10-163).  Kernel ids: */
+#define DD_K_WILSON 0    /* level-0 Wilson-Dirac stencil                 */
+#define DD_K_COARSE 1    /* coarse-level stencil                         */
+#define DD_K_SAP0 2      /* level-0 SAP block solves (one colour)         */
+#define DD_K_SAPC 3      /* coarse-level SAP block solves (one colour)    */
+#define DD_K_MULTIDOT 4  /* Gram-Schmidt multi-dot                        */
+#define DD_K_MULTIAXPY 5 /* Gram-Schmidt multi-axpy                       */
+#define DD_K_RESTRICT 6
+#define DD_K_PROLONG 7
+#define DD_K_COARSEST 8  /* coarsest-level GMRES (persistent kernel)      */
+#define DD_K_DOT 9
+int dd_timer_enable(int kernel, int on);
+int dd_timer_reset(void);
+/* total_ms: summed launch durations; bytes: summed algorithmic HBM bytes of the timed
+ * launches (every operand counted once: links, test vectors and the vectors read or
+ * written; re-reads from cache are not counted). */
+int dd_timer_read(int kernel, double* total_ms, long long* launches, double* bytes);
+
 /* Last error message (thread-unsafe, per process). */
 const char* dd_last_error(void);
 
