@@ -77,7 +77,8 @@ def main():
     if args.skip_mixed:
         nrhs = 1
     setup_mixed = ctx.setup(hierarchy_params(ddamg, N, 0 if args.skip_mixed else 1, nrhs, m0))
-    log(f'setup mixed: {setup_mixed:.3f}s')
+    tune = ctx.tuning(1)
+    log(f'setup mixed: {setup_mixed:.3f}s, autotuned coarse SAP variant {tune[0]} ({tune[1]:.3f} ms/half-sweep)')
     # warm-up solves with every kernel family timed: per-family breakdown
     fams = {ddamg.K_WILSON: 'wilson', ddamg.K_COARSE: 'coarse_stencil', ddamg.K_SAP0: 'sap_level0',
             ddamg.K_SAPC: 'sap_coarse', ddamg.K_MULTIDOT: 'gs_multidot', ddamg.K_MULTIAXPY: 'gs_multiaxpy',
@@ -164,6 +165,7 @@ def main():
         'outer_iterations': [int(its.min()), int(its.max())],
         'max_relres': float(relres.max()),
         'setup_s_mixed': setup_mixed,
+        'autotuned_coarse_sap_variant': tune[0],
         'plain_fp64_ms_per_rhs': None if t_plain is None else t_plain * 1e3,
         'plain_fp64_iterations': None if its_plain is None else int(its_plain[0]),
         'setup_s_fp64': setup_plain,

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'lib', 'libddamg.so')
-SOURCES = ['kernels.hip', 'solver.hip', 'capi.hip']
+SOURCES = (['kernels.hip', 'sap_f32.hip', 'sap_f64.hip', 'solver.hip', 'capi.hip'] +
+           [f'sap_{p}_d{d}.hip' for p in ('f32', 'f64') for d in (8, 16, 24, 32)])
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-Xclang', '-target-feature', '-Xclang',
          '-packed-fp32-ops', '-Wno-unused-command-line-argument']
@@ -35,7 +36,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + '.tmp'
     cmd = [HIPCC] + FLAGS + ['-shared', '-Wl,--no-undefined', '-o', tmp] + objs

@@ -130,6 +130,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->ortho_inner = 0;
     p->ortho_coarse = 1;
     p->max_rhs = 1;
+    p->autotune = 1;
     return DD_OK;
 }
 
@@ -254,6 +255,18 @@ int dd_solve(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_sol
     });
 }
 
+int dd_solve_x0(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
+    if (check(c) || !b || !x || nrhs < 1) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c) * nrhs;
+        Staged bb(b, n, s), xx(x, n, s);
+        const int st = c->s->solve(nrhs, bb.p(), xx.p(), iters, info, true);
+        download(x, xx.p(), n, s);
+        return st;
+    });
+}
+
 int dd_solve_adjoint_dev(dd_ctx* c, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info) {
     if (check(c) || !eta || !y || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
     return guard([&] {
@@ -298,6 +311,20 @@ int dd_dD_dtheta(dd_ctx* c, int nrhs, const double* y, const double* x, double* 
     });
 }
 
+int dd_get_tuning(dd_ctx* c, int level, int* variant, double* ms) {
+    if (check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
+        int v = -1;
+        double t = 0;
+        if (c->s->mg64) c->s->mg64->tuning(level, &v, &t);
+        else c->s->mg32->tuning(level, &v, &t);
+        if (variant) *variant = v;
+        if (ms) *ms = t;
+        return DD_OK;
+    });
+}
+
 int dd_level_shape(dd_ctx* c, int level, int* Nx, int* Nt, int* dof) {
     if (check(c)) return DD_ERR_ARG;
     return guard([&] {
@@ -323,6 +350,29 @@ int dd_level_apply(dd_ctx* c, int level, const double* in, double* out, int nrhs
             with_level_io<R>(c, in, out, n, n, nrhs, [&](const cplx<R>* a, cplx<R>* b) {
                 m->apply(level, a, b, nullptr, dd::full_mask(nrhs));
             });
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_cycle(dd_ctx* c, int level, int cycle, const double* eta, double* psi, int nrhs, int precision) {
+    if (check(c) || !eta || !psi || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels()) throw dd::BadShape("bad level");
+            if (nrhs > std::max(1, c->s->prm.max_rhs)) throw dd::BadShape("nrhs above max_rhs");
+            const size_t n = m->geom(level).n();
+            with_level_io<R>(
+                c, eta, psi, n, n, nrhs,
+                [&](const cplx<R>* a, cplx<R>* b) {
+                    if (cycle == 1) m->kcycle(level, a, b, dd::full_mask(nrhs));
+                    else m->vcycle(level, a, b, dd::full_mask(nrhs));
+                },
+                psi);
         };
         if (precision == 0) run(double());
         else run(float());

@@ -1,6 +1,8 @@
 // Launchers of the gfx950 kernels (kernels.hip).  All launches go to the given stream
 // and check the launch status; shapes are validated by the callers (solver.hip).
 #pragma once
+#include <vector>
+
 #include "common.h"
 
 namespace dd {
@@ -47,10 +49,14 @@ template <typename R>
 void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
                  int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
                  const int* gate);
+// variant: kernel variant (systems per work-group, load batching), -1 = default;
+// sapc_variants<R>() lists the compiled ones (autotuned in solver.hip)
 template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
                  int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask,
-                 size_t ld, const int* gate);
+                 size_t ld, const int* gate, int variant = -1);
+template <typename R>
+std::vector<int> sapc_variants();
 
 // ---- transfers ----------------------------------------------------------------------
 // out(coarse) = P^dagger v (fine);  W: [V][nv][dof]

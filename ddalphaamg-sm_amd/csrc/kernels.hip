@@ -230,7 +230,7 @@ static int lcm64(int dof) {
     return dof / a * 64;
 }
 
-#define DD_DOF_LIST(X) X(4) X(8) X(12) X(16) X(20) X(24) X(32)
+#define DD_DOF_LIST(X) X(8) X(16) X(24) X(32)
 
 template <typename R>
 void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx,
@@ -252,302 +252,6 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
             throw BadShape("coarse_apply: unsupported dof");
     }
     timer_end(s, K_COARSE, (double)Nx * Nt * dof * sizeof(cplx<R>) * (5.0 * dof + nact(mask, nrhs) * (b ? 3 : 2)));
-    DD_LAUNCH_CHECK();
-}
-
-// ======================================================================================
-// Block GMRES of the SAP smoother: FGMRES::fgmres with the identity preconditioner,
-// initial guess = block right-hand side, restart length M, the break on
-// |g_{j+1}| < tol ||phi||, maxIt kept across restarts and the true-residual test after
-// each cycle (src/fgmres.cpp:5-97, src/sap.cpp:65).  Every thread of a block holds one
-// element of each Krylov vector and the (uniform) Hessenberg scalars.
-// ======================================================================================
-template <int M>
-struct Tri {
-    static constexpr int at(int i, int j) { return j * (j + 1) / 2 + i; }   // i <= j
-    static constexpr int size = M * (M + 1) / 2;
-};
-
-template <typename R, int M, typename E, typename Op, typename Red>
-__device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int restarts, R tol) {
-    E x = phi;
-    E r = phi - A(x);
-    const R nphi = sqrt(red(phi.norm2()).x);
-    R err = sqrt(red(r.norm2()).x);
-    int maxIt = M;
-    E V[M + 1];
-#pragma unroll
-    for (int i = 0; i <= M; i++) V[i] = E::zero();
-    cplx<R> H[Tri<M>::size];
-    cplx<R> g[M + 1], cn[M], sn[M];
-#pragma unroll
-    for (int i = 0; i < M; i++) {
-        cn[i] = czero<R>();
-        sn[i] = czero<R>();
-    }
-    for (int k = 0; k < restarts; k++) {
-        V[0] = (R(1) / err) * r;
-        g[0] = mk<R>(err, 0);
-#pragma unroll
-        for (int j = 0; j < M; j++) {
-            E w = A(V[j]);
-#pragma unroll
-            for (int i = 0; i <= j; i++) {
-                const cplx<R> h = red(w.dotc(V[i]));
-                H[Tri<M>::at(i, j)] = h;
-                w = w - h * V[i];
-            }
-            const R hn = sqrt(red(w.norm2()).x);
-            if (j + 1 <= M && hn > R(0)) V[j + 1] = (R(1) / hn) * w;
-            // Givens rotations of column j (src/fgmres.cpp:99-115)
-#pragma unroll
-            for (int i = 0; i < j; i++) {
-                const cplx<R> a = H[Tri<M>::at(i, j)], bb = H[Tri<M>::at(i + 1, j)];
-                H[Tri<M>::at(i, j)] = cmul(cn[i], a) + cmul(sn[i], bb);
-                H[Tri<M>::at(i + 1, j)] = cn[i] * bb - sn[i] * a;
-            }
-            const cplx<R> hjj = H[Tri<M>::at(j, j)];
-            const cplx<R> hs = mk<R>(hn, 0);
-            const R den = sqrt(norm2(hjj) + norm2(hs));
-            sn[j] = mk<R>(hs.x / den, hs.y / den);
-            cn[j] = mk<R>(hjj.x / den, hjj.y / den);
-            H[Tri<M>::at(j, j)] = cmul(cn[j], hjj) + cmul(sn[j], hs);
-            g[j + 1] = -(sn[j] * g[j]);
-            g[j] = cmul(cn[j], g[j]);
-            if (sqrt(norm2(g[j + 1])) < tol * nphi) {
-                maxIt = j + 1;
-                break;
-            }
-        }
-        // back substitution (src/fgmres.cpp:118-127) and x += sum eta_j Z_j
-        cplx<R> eta[M];
-#pragma unroll
-        for (int i = M - 1; i >= 0; i--) {
-            if (i < maxIt) {
-                cplx<R> e = g[i];
-#pragma unroll
-                for (int jj = i + 1; jj < M; jj++)
-                    if (jj < maxIt) e = e - H[Tri<M>::at(i, jj)] * eta[jj];
-                eta[i] = cdiv(e, H[Tri<M>::at(i, i)]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < M; j++)
-            if (j < maxIt) x = x + eta[j] * V[j];
-        r = phi - A(x);
-        err = sqrt(red(r.norm2()).x);
-        if (err < tol * nphi) break;
-    }
-    return x;
-}
-
-// Colour-c block number k -> block coordinates (red iff BX + BT even).
-__device__ __forceinline__ void colour_block(int k, int colour, int nbx, int nbt, int& BX, int& BT) {
-    if ((nbt & 1) == 0) {
-        const int h = nbt >> 1;
-        BX = k / h;
-        BT = 2 * (k - BX * h) + ((BX + colour) & 1);
-    } else {
-        const int bidx = 2 * k + colour;
-        BX = bidx / nbt;
-        BT = bidx - BX * nbt;
-    }
-}
-
-// ---- level 0: one site (two spin components) per lane; S = bx*bt lanes per block,
-// 64/S blocks per wave; neighbours through cross-lane permutes, sums through
-// butterflies inside the S-lane segment.  No LDS and no barriers.
-template <typename R>
-struct Spin2 {
-    cplx<R> a, b;
-    __device__ static Spin2 zero() { return {czero<R>(), czero<R>()}; }
-    __device__ Spin2 operator+(const Spin2& o) const { return {a + o.a, b + o.b}; }
-    __device__ Spin2 operator-(const Spin2& o) const { return {a - o.a, b - o.b}; }
-    __device__ cplx<R> dotc(const Spin2& o) const { return mulc(a, o.a) + mulc(b, o.b); }
-    __device__ cplx<R> norm2() const { return mk<R>(dd::norm2(a) + dd::norm2(b), R(0)); }
-};
-template <typename R>
-__device__ __forceinline__ Spin2<R> operator*(cplx<R> s, const Spin2<R>& v) { return {s * v.a, s * v.b}; }
-template <typename R>
-__device__ __forceinline__ Spin2<R> operator*(R s, const Spin2<R>& v) { return {s * v.a, s * v.b}; }
-
-template <typename R>
-__device__ __forceinline__ cplx<R> shfl_c(cplx<R> v, int src) {
-    return mk<R>(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
-template <typename R, int M>
-__global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ rr,
-                                             cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt, int colour,
-                                             R m0, int restarts, R tol, uint64_t mask, size_t ld,
-                                             const int* __restrict__ gate, int nblk_colour) {
-    const int rh = blockIdx.y;
-    if (!rhs_on(mask, rh)) return;
-    if (gate && gate[rh] == 0) return;
-    const int S = bx * bt;
-    const int lane = threadIdx.x;
-    const int seg = lane / S, k = lane - seg * S;
-    const int blk = blockIdx.x * (64 / S) + seg;
-    if (blk >= nblk_colour) return;
-    const int nbx = Nx / bx, nbt = Nt / bt;
-    int BX, BT;
-    colour_block(blk, colour, nbx, nbt, BX, BT);
-    const int lx = k / bt, lt = k - lx * bt;
-    const int X = BX * bx + lx, T = BT * bt + lt;
-    const int n = X * Nt + T;
-    const int base = seg * S;
-    const bool hasTp = lt < bt - 1, hasTm = lt > 0, hasXp = lx < bx - 1, hasXm = lx > 0;
-    const int sTp = hasTp ? lane + 1 : lane, sTm = hasTm ? lane - 1 : lane;
-    const int sXp = hasXp ? lane + bt : lane, sXm = hasXm ? lane - bt : lane;
-    const cplx<R> U0 = U[2 * n], U1 = U[2 * n + 1];
-    const cplx<R> U0m = hasTm ? conj(U[2 * (n - 1)]) : czero<R>();
-    const cplx<R> U1m = hasXm ? conj(U[2 * (n - Nt) + 1]) : czero<R>();
-    const R mass = m0 + R(2);
-    (void)base;
-    // D_B restricted to the block, zero outside, no boundary sign
-    auto A = [&](const Spin2<R>& v) -> Spin2<R> {
-        cplx<R> tp0 = shfl_c(v.a, sTp), tp1 = shfl_c(v.b, sTp);
-        cplx<R> xp0 = shfl_c(v.a, sXp), xp1 = shfl_c(v.b, sXp);
-        cplx<R> tm0 = shfl_c(v.a, sTm), tm1 = shfl_c(v.b, sTm);
-        cplx<R> xm0 = shfl_c(v.a, sXm), xm1 = shfl_c(v.b, sXm);
-        if (!hasTp) tp0 = tp1 = czero<R>();
-        if (!hasXp) xp0 = xp1 = czero<R>();
-        if (!hasTm) tm0 = tm1 = czero<R>();
-        if (!hasXm) xm0 = xm1 = czero<R>();
-        const cplx<R> a0 =
-            U0 * (tp0 - tp1) + U1 * (xp0 + times_i(xp1)) + U0m * (tm0 + tm1) + U1m * (xm0 - times_i(xm1));
-        const cplx<R> a1 =
-            U0 * (tp1 - tp0) + U1 * (xp1 - times_i(xp0)) + U0m * (tm0 + tm1) + U1m * (times_i(xm0) + xm1);
-        return {mass * v.a - R(0.5) * a0, mass * v.b - R(0.5) * a1};
-    };
-    auto red = [&](cplx<R> v) -> cplx<R> { return wave_sum(v, S); };
-    const cplx<R>* rv = rr + rh * ld;
-    const Spin2<R> phi = {rv[2 * n], rv[2 * n + 1]};
-    const Spin2<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
-    cplx<R>* xv = xx + rh * ld;
-    xv[2 * n] = xv[2 * n] + e.a;
-    xv[2 * n + 1] = xv[2 * n + 1] + e.b;
-}
-
-template <typename R>
-void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
-                 int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
-                 const int* gate) {
-    timer_begin(s, K_SAP0);
-    const int S = bx * bt;
-    if (S > 64 || (S & (S - 1)) != 0) throw BadShape("level-0 SAP block must have a power-of-two site count <= 64");
-    if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
-    const int nblk = (Nx / bx) * (Nt / bt) / 2;
-    dim3 grid(cdivu(nblk, 64 / S), nrhs);
-    k_sap0<R, kSapM><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld, gate,
-                                         nblk);
-    timer_end(s, K_SAP0, 0.5 * Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + 3.0 * nact(mask, nrhs)));
-    DD_LAUNCH_CHECK();
-}
-
-// ---- coarse levels: one work-group per block, one thread per (site, row)
-template <typename R>
-struct Elem1 {
-    cplx<R> a;
-    __device__ static Elem1 zero() { return {czero<R>()}; }
-    __device__ Elem1 operator+(const Elem1& o) const { return {a + o.a}; }
-    __device__ Elem1 operator-(const Elem1& o) const { return {a - o.a}; }
-    __device__ cplx<R> dotc(const Elem1& o) const { return mulc(a, o.a); }
-    __device__ cplx<R> norm2() const { return mk<R>(dd::norm2(a), R(0)); }
-};
-template <typename R>
-__device__ __forceinline__ Elem1<R> operator*(cplx<R> s, const Elem1<R>& v) { return {s * v.a}; }
-template <typename R>
-__device__ __forceinline__ Elem1<R> operator*(R s, const Elem1<R>& v) { return {s * v.a}; }
-
-template <typename R, int DOF, int M>
-__global__ __launch_bounds__(512) void k_sapc(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ rr,
-                                               cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
-                                               int colour, R m0, int restarts, R tol, uint64_t mask, size_t ld,
-                                               const int* __restrict__ gate) {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* sv = reinterpret_cast<cplx<R>*>(smem_raw);
-    R* scratch = reinterpret_cast<R*>(sv + bx * bt * DOF);
-    const int rh = blockIdx.y;
-    if (!rhs_on(mask, rh)) return;
-    if (gate && gate[rh] == 0) return;
-    const int S = bx * bt;
-    const int tid = threadIdx.x;
-    const bool act = tid < S * DOF;
-    const int k = act ? tid / DOF : 0, row = act ? tid - (tid / DOF) * DOF : 0;
-    const int nbx = Nx / bx, nbt = Nt / bt;
-    int BX, BT;
-    colour_block(blockIdx.x, colour, nbx, nbt, BX, BT);
-    const int lx = k / bt, lt = k - lx * bt;
-    const int n = (BX * bx + lx) * Nt + BT * bt + lt;
-    const bool hasTp = lt < bt - 1, hasTm = lt > 0, hasXp = lx < bx - 1, hasXm = lx > 0;
-    const cplx<R>* Ln = L + (size_t)n * 5 * DOF * DOF + row;
-    const R mass = m0 + R(2);
-    auto A = [&](const Elem1<R>& v) -> Elem1<R> {
-        __syncthreads();
-        if (act) sv[tid] = v.a;
-        __syncthreads();
-        if (!act) return Elem1<R>::zero();
-        cplx<R> acc = czero<R>();
-        const cplx<R>* s0 = sv + k * DOF;
-#pragma unroll 8
-        for (int col = 0; col < DOF; col++) acc = acc + Ln[col * DOF] * s0[col];
-        if (hasTp) {
-            const cplx<R>* s1 = sv + (k + 1) * DOF;
-#pragma unroll 8
-            for (int col = 0; col < DOF; col++) acc = acc + Ln[(1 * DOF + col) * DOF] * s1[col];
-        }
-        if (hasXp) {
-            const cplx<R>* s1 = sv + (k + bt) * DOF;
-#pragma unroll 8
-            for (int col = 0; col < DOF; col++) acc = acc + Ln[(2 * DOF + col) * DOF] * s1[col];
-        }
-        if (hasTm) {
-            const cplx<R>* s1 = sv + (k - 1) * DOF;
-#pragma unroll 8
-            for (int col = 0; col < DOF; col++) acc = acc + Ln[(3 * DOF + col) * DOF] * s1[col];
-        }
-        if (hasXm) {
-            const cplx<R>* s1 = sv + (k - bt) * DOF;
-#pragma unroll 8
-            for (int col = 0; col < DOF; col++) acc = acc + Ln[(4 * DOF + col) * DOF] * s1[col];
-        }
-        return {mass * v.a - acc};
-    };
-    auto red = [&](cplx<R> v) -> cplx<R> { return block_sum(v, scratch); };
-    const cplx<R>* rv = rr + rh * ld;
-    Elem1<R> phi = {act ? rv[(size_t)n * DOF + row] : czero<R>()};
-    const Elem1<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
-    if (act) {
-        cplx<R>* xv = xx + rh * ld + (size_t)n * DOF + row;
-        *xv = *xv + e.a;
-    }
-}
-
-template <typename R>
-void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
-                 int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
-                 const int* gate) {
-    timer_begin(s, K_SAPC);
-    const int S = bx * bt;
-    if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
-    const int threads = ((S * dof + 63) / 64) * 64;
-    if (threads > 512) throw BadShape("coarse SAP block too large (sites*dof > 512)");
-    const int nblk = (Nx / bx) * (Nt / bt) / 2;
-    dim3 grid(nblk, nrhs);
-    const size_t sh = sizeof(cplx<R>) * S * dof + sizeof(R) * 32;
-    switch (dof) {
-#define DD_CASE(D)                                                                                               \
-    case D:                                                                                                      \
-        k_sapc<R, D, kSapM><<<grid, threads, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, \
-                                                     mask, ld, gate);                                           \
-        break;
-        DD_DOF_LIST(DD_CASE)
-#undef DD_CASE
-        default:
-            throw BadShape("sap_blocksc: unsupported dof");
-    }
-    timer_end(s, K_SAPC, 0.5 * Nx * Nt * dof * sizeof(cplx<R>) * (5.0 * dof + 3.0 * nact(mask, nrhs)));
     DD_LAUNCH_CHECK();
 }
 
@@ -1257,10 +961,6 @@ void dtheta_contract(hipStream_t s, const cplx<double>* U, const cplx<double>* y
     template void dirac_links<R>(hipStream_t, const cplx<R>*, cplx<R>*, int, int);                                  \
     template void coarse_apply<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, const cplx<R>*, int, int,  \
                                   int, double, int, uint64_t, size_t);                                              \
-    template void sap_blocks0<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int,    \
-                                 double, int, int, double, int, uint64_t, size_t, const int*);                      \
-    template void sap_blocksc<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int,    \
-                                 int, double, int, int, double, int, uint64_t, size_t, const int*);                 \
     template void restrict_<R>(hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int, int, int, int, \
                                int, uint64_t, size_t, size_t);                                                      \
     template void prolong<R>(hipStream_t, const cplx<R>*, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, int,  \

@@ -1,0 +1,10 @@
+// SAP block-solve kernels instantiated for float (see sap_impl.h).
+#include "sap_impl.h"
+
+namespace dd {
+template void sap_blocks0<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
+                               double, int, int, double, int, uint64_t, size_t, const int*);
+template void sap_blocksc<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
+                               int, double, int, int, double, int, uint64_t, size_t, const int*, int);
+template std::vector<int> sapc_variants<float>();
+}  // namespace dd

@@ -1,0 +1,8 @@
+// Coarse SAP kernels for float, dof = 16 (see sap_impl.h).
+#define DD_SAPC_UNIT
+#include "sap_impl.h"
+
+namespace dd {
+template void dispatch_sapc<float, 16>(int, hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int,
+                                      int, int, int, double, int, double, int, uint64_t, size_t, const int*);
+}  // namespace dd

@@ -1,0 +1,10 @@
+// SAP block-solve kernels instantiated for double (see sap_impl.h).
+#include "sap_impl.h"
+
+namespace dd {
+template void sap_blocks0<double>(hipStream_t, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int, int, int, int,
+                               double, int, int, double, int, uint64_t, size_t, const int*);
+template void sap_blocksc<double>(hipStream_t, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int, int, int, int,
+                               int, double, int, int, double, int, uint64_t, size_t, const int*, int);
+template std::vector<int> sapc_variants<double>();
+}  // namespace dd

@@ -138,6 +138,13 @@ class Multigrid {
     const cplx<R>* links(int l) const { return links_[l].template as<cplx<R>>(); }
     const cplx<R>* W(int l) const { return W_[l].template as<cplx<R>>(); }
     int cap(int l) const { return cap_[l]; }
+    // times every compiled coarse-SAP variant on the built hierarchy (a batch of
+    // max_rhs pseudo-random systems, one colour) and keeps the fastest per level
+    void autotune();
+    void tuning(int l, int* variant, double* ms) const {
+        *variant = (l >= 0 && l < (int)sapc_var_.size()) ? sapc_var_[l] : -1;
+        *ms = (l >= 0 && l < (int)sapc_ms_.size()) ? sapc_ms_[l] : 0.0;
+    }
     std::vector<long long> coarse_iters;   // coarsest GMRES iterations per call (stats)
 
    private:
@@ -155,6 +162,8 @@ class Multigrid {
     std::vector<std::unique_ptr<Krylov<R>>> kry_;   // K-cycle FGMRES of level l (index l)
     std::unique_ptr<Krylov<R>> coarse_;
     std::vector<int> iters_scratch_;
+    std::vector<int> sapc_var_;
+    std::vector<double> sapc_ms_;
 };
 
 // ---- top-level solver --------------------------------------------------------------------
@@ -165,7 +174,9 @@ class Solver {
     void set_gauge(const cplx<double>* U_dev);
     void setup(const dd_params& p);
     // b, x: device pointers, nrhs systems of V*2 entries back to back
-    int solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info);
+    // use_x0: x holds the initial guess on entry (otherwise x0 = 0)
+    int solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info,
+              bool use_x0 = false);
     void apply_D(const cplx<double>* in, cplx<double>* out, int nrhs, double m0, bool dagger, int precision);
     int Nx, Nt, device;
     hipStream_t s = nullptr;

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace dd {
@@ -42,6 +43,18 @@ void PinMem::release() {
 __global__ void k_fill_int(int* p, int v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// deterministic pseudo-random complex entries in [-1, 1) (autotuner inputs)
+template <typename R>
+__global__ void k_fill_hash(cplx<R>* p, size_t n, uint64_t seed) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    p[i] = mk<R>((R)((double)(z >> 40) / 8388608.0 - 1.0), (R)((double)(z & 0xffffff) / 8388608.0 - 1.0));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -360,7 +373,8 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
                                p_.sap_tol, nr, mask, n, gate);
             else
                 sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, colour, p_.m0, p_.sap_m,
-                               p_.sap_restarts, p_.sap_tol, nr, mask, n, gate);
+                               p_.sap_restarts, p_.sap_tol, nr, mask, n, gate,
+                               l < (int)sapc_var_.size() ? sapc_var_[l] : -1);
             apply(l, x, r, v, mask);
         }
         dot_partial<R>(s_, r, r, n, n, nr, mask, rb_[l]);
@@ -468,6 +482,54 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
     }
 }
 
+template <typename R>
+void Multigrid<R>::autotune() {
+    const int L = levels();
+    sapc_var_.assign(L, -1);
+    sapc_ms_.assign(L, 0.0);
+    const int nr = std::max(1, p_.max_rhs);
+    hipEvent_t e0, e1;
+    DD_HIP(hipEventCreate(&e0));
+    DD_HIP(hipEventCreate(&e1));
+    for (int l = 1; l < L - 1; l++) {
+        const LevelGeom& G = g_[l];
+        const size_t n = G.n();
+        cplx<R>* r = res_[l].as<cplx<R>>();
+        cplx<R>* x = sapr_[l].as<cplx<R>>();
+        k_fill_hash<R><<<(unsigned)((n * nr + 255) / 256), 256, 0, s_>>>(r, n * nr, 12345 + l);
+        DD_HIP(hipGetLastError());
+        double best = 1e300;
+        int bestv = -1;
+        for (int v : sapc_variants<R>()) {
+            if (v / 2 > 1 && nr == 1) continue;   // multi-system variants only for batches
+            double t = 0;
+            for (int rep = 0; rep < 2; rep++) {
+                zero<R>(s_, x, n * nr, 0, 1, 1);
+                DD_HIP(hipEventRecord(e0, s_));
+                sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, 0, p_.m0, p_.sap_m,
+                               p_.sap_restarts, p_.sap_tol, nr, full_mask(nr), n, nullptr, v);
+                DD_HIP(hipEventRecord(e1, s_));
+                DD_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+                t = ms;   // keep the second (warm) run
+            }
+            if (t < best) {
+                best = t;
+                bestv = v;
+            }
+            if (std::getenv("DDAMG_VERBOSE"))
+                fprintf(stderr, "autotune level %d: coarse SAP variant %d (%d systems/work-group%s): %.3f ms\n", l, v,
+                        v / 2, (v & 1) ? ", batched link loads" : "", t);
+        }
+        if (const char* f = std::getenv("DDAMG_SAPC_VARIANT")) bestv = std::atoi(f);   // explicit override
+        sapc_var_[l] = bestv;
+        sapc_ms_[l] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 template class Krylov<double>;
 template class Krylov<float>;
 template class Multigrid<double>;
@@ -531,18 +593,20 @@ void Solver::setup(const dd_params& p) {
     if (p.precision == 0) {
         mg64.reset(new Multigrid<double>(s, Nx, Nt, p, U64.as<cplx<double>>()));
         mg64->setup(tv.as<cplx<double>>(), rng);
+        if (p.autotune) mg64->autotune();
     } else {
         DevMem tv32(sizeof(cplx<float>) * h.size());
         convert<float, double>(s, tv32.as<cplx<float>>(), tv.as<cplx<double>>(), h.size(), 0, 0, 1, 1);
         mg32.reset(new Multigrid<float>(s, Nx, Nt, p, U32.as<cplx<float>>()));
         mg32->setup(tv32.as<cplx<float>>(), rng);
+        if (p.autotune) mg32->autotune();
     }
     DD_HIP(hipStreamSynchronize(s));
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     have_setup = true;
 }
 
-int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info) {
+int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info, bool use_x0) {
     if (!have_setup) throw BadShape("solve before setup");
     if (nrhs < 1 || nrhs > std::max(1, prm.max_rhs)) throw BadShape("nrhs exceeds max_rhs of the setup");
     auto t0 = std::chrono::steady_clock::now();
@@ -576,7 +640,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             convert<double, float>(s, io, y, n, n, n, nr, m);
         }
     };
-    zero<double>(s, x, n, n, nrhs, mask);
+    if (!use_x0) zero<double>(s, x, n, n, nrhs, mask);
     std::vector<int> its(64, 0);
     std::vector<double> rr(64, 0.0);
     const uint64_t conv = outer_->solve(b, x, x, mask, A, &M, its.data(), rr.data());

@@ -39,7 +39,7 @@ class Params(C.Structure):
                 ('cycle', C.c_int), ('precision', C.c_int),
                 ('rng_ranks_x', C.c_int), ('rng_ranks_t', C.c_int),
                 ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
-                ('max_rhs', C.c_int)]
+                ('max_rhs', C.c_int), ('autotune', C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -65,15 +65,18 @@ _SIGS = {
     'dd_solve': (C.c_int, [_ctx, C.c_int, _dp, _dp, _ip, C.POINTER(SolveInfo)]),
     'dd_solve_dev': (C.c_int, [_ctx, C.c_int, C.c_void_p, C.c_void_p, _ip, C.POINTER(SolveInfo)]),
     'dd_solve_adjoint': (C.c_int, [_ctx, C.c_int, _dp, _dp, _ip, C.POINTER(SolveInfo)]),
+    'dd_solve_x0': (C.c_int, [_ctx, C.c_int, _dp, _dp, _ip, C.POINTER(SolveInfo)]),
     'dd_solve_adjoint_dev': (C.c_int, [_ctx, C.c_int, C.c_void_p, C.c_void_p, _ip, C.POINTER(SolveInfo)]),
     'dd_dD_dtheta': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double]),
     'dd_level_apply': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
+    'dd_cycle': (C.c_int, [_ctx, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int]),
     'dd_restrict': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
     'dd_prolong': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_int]),
     'dd_sap': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int]),
     'dd_get_test_vectors': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_get_links': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
+    'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
     'dd_last_error': (C.c_char_p, []),
     'dd_timer_enable': (C.c_int, [C.c_int, C.c_int]),
     'dd_timer_reset': (C.c_int, []),
@@ -242,6 +245,12 @@ class Context:
         _chk(lib.dd_dD_dtheta(self._h, yb.shape[0], _cptr(yb), _cptr(xb), _cptr(g), scale))
         return g
 
+    def tuning(self, level: int):
+        """(variant, ms) the autotuner chose for the coarse SAP of `level`."""
+        v, t = C.c_int(), C.c_double()
+        _chk(lib.dd_get_tuning(self._h, level, C.byref(v), C.byref(t)))
+        return v.value, t.value
+
     def level_shape(self, level: int):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _chk(lib.dd_level_shape(self._h, level, C.byref(a), C.byref(b), C.byref(c)))
@@ -277,6 +286,15 @@ class Context:
         xb = x if batch else x[None]
         _chk(lib.dd_sap(self._h, level, _cptr(vb), _cptr(xb), nu, tol, vb.shape[0], precision))
         return xb if batch else xb[0]
+
+    def cycle(self, level, eta, psi0, kcycle=True, precision=0):
+        """One K- or V-cycle from `level` with initial guess psi0 (returns psi)."""
+        eta, psi = _c128(eta), _c128(psi0).copy()
+        batch = eta.ndim == 4
+        eb = eta if batch else eta[None]
+        pb = psi if batch else psi[None]
+        _chk(lib.dd_cycle(self._h, level, int(kcycle), _cptr(eb), _cptr(pb), eb.shape[0], precision))
+        return pb if batch else pb[0]
 
     def test_vectors(self, level, precision=0):
         Nx, Nt, d = self.level_shape(level)

@@ -62,6 +62,8 @@ typedef struct dd_params {
     int ortho_inner;               /* same, for the K-cycle FGMRES                  */
     int ortho_coarse;              /* same, for the coarsest GMRES (default 1)      */
     int max_rhs;                   /* largest batch solve() will be asked for (1)   */
+    int autotune;                  /* 1: time the coarse-SAP kernel variants on the
+                                      built hierarchy at setup and keep the fastest  */
 } dd_params;
 
 typedef struct dd_solve_info {
@@ -111,6 +113,10 @@ double dd_setup_seconds(dd_ctx* ctx);
 int dd_solve(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 int dd_solve_dev(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 
+/* Same with the initial guess taken from x (x0 = x on entry; the reference's
+ * fgmres(phi, x0, x) signature, src/fgmres.cpp:5-10). */
+int dd_solve_x0(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
+
 /* Adjoint solve D^dagger y = eta on the same hierarchy through sigma3-hermiticity:
  * y = sigma3 D^-1 sigma3 eta.  The reference has D_dagger_phi only
  * (src/dirac_operator.cpp:56-85); no adjoint solver. */
@@ -124,6 +130,9 @@ int dd_dD_dtheta(dd_ctx* ctx, int nrhs, const double* y, const double* x, double
 
 /* ---- per-component entry points (tests, benchmarks) ---------------------------------
  * Level-l vectors have dof = 2 (l = 0) or 2*nv[l-1]. */
+/* One K-cycle (cycle = 1) or V-cycle (cycle = 0) from `level` on psi (in/out):
+ * AlgebraicMG::k_cycle / v_cycle, src/amg.cpp:64-174. */
+int dd_cycle(dd_ctx* ctx, int level, int cycle, const double* eta, double* psi, int nrhs, int precision);
 /* Level::D_operator, src/level.cpp:394-426 (level 0: the Wilson stencil). */
 int dd_level_apply(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, int precision);
 /* Level::Pdagg_v / Level::P_vc, src/level.cpp:145-189 / :92-141 (level -> level+1). */
@@ -135,6 +144,9 @@ int dd_sap(dd_ctx* ctx, int level, const double* v, double* x, int nu, double to
 int dd_get_test_vectors(dd_ctx* ctx, int level, double* out, int precision);
 /* Links of level l >= 1 as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1). */
 int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
+/* Autotuner result for level l >= 1: chosen coarse-SAP kernel variant
+ * (2*systems_per_workgroup + batched_link_loads) and its measured half-sweep time. */
+int dd_get_tuning(dd_ctx* ctx, int level, int* variant, double* ms);
 /* Shape of level l: Nx, Nt, dof. */
 int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);
 /* ---- live kernel timing -----------------------------------------------------------------

@@ -154,3 +154,33 @@ def test_gauge_generator_plaquette():
         assert np.allclose(np.exp(1j * th), ctx.gauge())
     pl = np.mean(np.cos(orc.plaquette_angle(th)))
     assert abs(pl - iv(1, 2.0) / iv(0, 2.0)) < 0.01
+
+
+@pytest.mark.parametrize('precision', [0, 1])
+def test_coarse_sap_batched(precision):
+    """Coarse-level SAP on batches that fill 4- and 8-system work-groups partially equals
+    per-system SAP, and (FP64) the oracle's SAP on the device's own level-1 operator."""
+    rng = np.random.default_rng(21)
+    N = 32
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.4)
+    prm = ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[8, 8], m0=-0.2,
+                       precision=precision, max_rhs=9, rng_ranks=(2, 2))
+    with ddamg.Context(N, N) as ctx:
+        ctx.set_gauge(U)
+        ctx.setup(prm)
+        shape = ctx.level_shape(1)
+        v = rng.standard_normal((9,) + shape) + 1j * rng.standard_normal((9,) + shape)
+        x0 = np.zeros_like(v)
+        tol_b = 1e-12 if precision == 0 else 1e-5
+        for nr in (1, 3, 6, 9):
+            xb = ctx.sap(1, v[:nr], x0[:nr], 2, precision=precision)
+            for r in range(nr):
+                x1 = ctx.sap(1, v[r], x0[r], 2, precision=precision)
+                assert rel(xb[r], x1) < tol_b
+        if precision == 0:
+            A, B, Cm = ctx.links(1)
+            op = orc.Stencil(A, B, Cm, -0.2)
+            blocks = orc.SapBlocks(shape[0], shape[1], 2, 2)
+            for r in range(3):
+                ref, _ = orc.sap(op, blocks, v[r], x0[r], 2, 1e-10)
+                assert rel(ctx.sap(1, v[r], x0[r], 2), ref) < 1e-11
