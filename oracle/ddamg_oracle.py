@@ -831,3 +831,40 @@ def ref_links_to_stencil(G1: np.ndarray, G2: np.ndarray, G3: np.ndarray, m0: flo
         B[mu] = np.transpose(b2, (0, 1, 4, 2, 5, 3)).reshape(Nx, Nt, d, d)
         C[mu] = np.transpose(b3, (0, 1, 4, 2, 5, 3)).reshape(Nx, Nt, d, d)
     return Stencil(A, B, C, m0)
+
+
+# --------------------------------------------------------------------------------------
+# Differentiation (no reference counterpart; pinned by dense finite differences)
+
+
+def dL_dtheta(U: np.ndarray, y: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Gradient of L = Re<eta, D^-1 b> with respect to the link phases theta_mu(n)
+    (U = e^{i theta}), given x = D^-1 b and y = D^-dagger eta:
+        dL/dtheta_mu(n) = -Re[ y^H (dD/dtheta_mu(n)) x ],
+    where U_mu(n) enters row n as -1/2 (1 - sigma_mu) s+ U x(n+mu) and row n+mu as
+    -1/2 (1 + sigma_mu) s- U^* x(n) (src/dirac_operator.cpp:24-54).  Returns (Nx, Nt, 2)."""
+    Nt = x.shape[1]
+    g = np.zeros(U.shape[:2] + (2,))
+    for mu in (0, 1):
+        s = sign_fwd(Nt)[..., 0] if mu == 0 else np.ones(U.shape[:2])
+        u = U[..., mu]
+        xf = fwd(x, mu)
+        yf = fwd(y, mu)
+        M, P = _M[mu], _P[mu]
+        f = (1j * u * s)[..., None] * np.einsum('ab,xtb->xta', M, xf)
+        bk = (-1j * np.conj(u) * s)[..., None] * np.einsum('ab,xtb->xta', P, x)
+        d = -0.5 * (np.sum(np.conj(y) * f, axis=-1) + np.sum(np.conj(yf) * bk, axis=-1))
+        g[..., mu] = -np.real(d)
+    return g
+
+
+def dense_D(U: np.ndarray, m0: float) -> np.ndarray:
+    """Dense matrix of D_phi on a small lattice (columns = D applied to unit vectors)."""
+    Nx, Nt = U.shape[:2]
+    n = Nx * Nt * 2
+    M = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1
+        M[:, j] = D_phi(U, e.reshape(Nx, Nt, 2), m0).ravel()
+    return M

@@ -203,3 +203,32 @@ def test_records_roundtrip(tmp_path):
     import os
     assert os.path.getsize(p) == 4 * 6 * 2 * 28
     assert np.array_equal(orc.read_records(p, 4, 6), a)
+
+
+def test_gradient_formula_dense_fd():
+    """dL/dtheta and dL/dm0 of L = Re<eta, D^-1 b> by central differences of dense
+    solves on a 4x6 lattice (h = 1e-6, exact linear algebra)."""
+    rng = np.random.default_rng(8)
+    Nx, Nt, m0 = 4, 6, -0.2
+    th = rng.uniform(-np.pi, np.pi, (Nx, Nt, 2))
+    b = rng.standard_normal((Nx, Nt, 2)) + 1j * rng.standard_normal((Nx, Nt, 2))
+    eta = rng.standard_normal((Nx, Nt, 2)) + 1j * rng.standard_normal((Nx, Nt, 2))
+
+    def L(theta, m):
+        D = orc.dense_D(np.exp(1j * theta), m)
+        return np.real(np.vdot(eta.ravel(), np.linalg.solve(D, b.ravel())))
+
+    U = np.exp(1j * th)
+    D = orc.dense_D(U, m0)
+    x = np.linalg.solve(D, b.ravel()).reshape(b.shape)
+    y = np.linalg.solve(D.conj().T, eta.ravel()).reshape(b.shape)
+    g = orc.dL_dtheta(U, y, x)
+    h = 1e-6
+    for (ix, it, mu) in [(0, 0, 0), (1, 5, 0), (3, 2, 1), (2, 5, 1), (0, 3, 1)]:
+        tp, tm = th.copy(), th.copy()
+        tp[ix, it, mu] += h
+        tm[ix, it, mu] -= h
+        fd = (L(tp, m0) - L(tm, m0)) / (2 * h)
+        assert abs(fd - g[ix, it, mu]) < 1e-7 * max(1, abs(fd))
+    fd_m = (L(th, m0 + h) - L(th, m0 - h)) / (2 * h)
+    assert abs(fd_m - (-np.real(np.vdot(y, x)))) < 1e-7 * max(1, abs(fd_m))
