@@ -131,6 +131,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->ortho_coarse = 1;
     p->max_rhs = 1;
     p->autotune = 1;
+    p->sap_variant = -1;
     return DD_OK;
 }
 

@@ -57,6 +57,8 @@ void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, 
                  size_t ld, const int* gate, int variant = -1);
 template <typename R>
 std::vector<int> sapc_variants();
+template <typename R>
+bool sapc_supported(int variant, int dof, int S);
 
 // ---- transfers ----------------------------------------------------------------------
 // out(coarse) = P^dagger v (fine);  W: [V][nv][dof]

@@ -13,6 +13,7 @@
 //   k_ortho         Level::orthonormalize         src/level.cpp:193-284
 //   k_galerkin      Level::makeCoarseLinks        src/level.cpp:431-542
 #include "kernels.h"
+#include "mfma.h"
 
 #include <algorithm>
 #include <cmath>
@@ -177,57 +178,58 @@ void dirac_links(hipStream_t s, const cplx<R>* U, cplx<R>* L, int Nx, int Nt) {
 }
 
 // ======================================================================================
-// Generic level stencil  out = (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu)).
-// Work-group of SPB sites x DOF rows; one thread per (site, row).  The five input
-// spinors of each site are staged in LDS, the links stream from HBM (coalesced on
-// rows thanks to the column-major matrix layout).
+// Generic level stencil  out = (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))
+// on the matrix cores: one wave per site and tile of 16 systems; each link element is
+// read once per launch for all 16 systems of the tile (real-block 16x16x4 products,
+// layout in mfma.h).  The neighbour spinors are read straight from the vectors (four
+// lanes per system share each 16-byte segment).
 // ======================================================================================
 template <typename R, int DOF, bool RES>
-__global__ void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
-                         const cplx<R>* __restrict__ b, int Nx, int Nt, R m0, uint64_t mask, size_t ld, int spb) {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<R>* sv = reinterpret_cast<cplx<R>*>(smem_raw);
-    const int r = blockIdx.y;
-    if (!rhs_on(mask, r)) return;
-    const int tid = threadIdx.x;
-    const int sl = tid / DOF, row = tid - sl * DOF;
-    const int V = Nx * Nt;
-    const int n = blockIdx.x * spb + sl;
-    const bool ok = n < V;
-    const cplx<R>* v = in + r * ld;
-    cplx<R> self = czero<R>();
-    if (ok) {
-        const int x = n / Nt, t = n - x * Nt;
-        const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
-                           (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
-        const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+__global__ __launch_bounds__(256) void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
+                                                cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
+                                                int Nt, R m0, uint64_t mask, size_t ld, int nrhs) {
+    using acc_t = typename Mfma<R>::acc_t;
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (n >= Nx * Nt) return;   // uniform per wave
+    const int j = blockIdx.y * 16 + i;
+    const bool on = j < nrhs && rhs_on(mask, j);
+    const int x = n / Nt, t = n - x * Nt;
+    const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
+                       (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
+    const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+    const LinkLane ll = LinkLane::make<R, DOF>(lane);
+    const R* Ln = reinterpret_cast<const R*>(L + (size_t)n * 5 * DOF * DOF);
+    const R* vin = reinterpret_cast<const R*>(in + (size_t)(on ? j : 0) * ld);
+    acc_t acc[MT];
 #pragma unroll
-        for (int k = 0; k < 5; k++) sv[(sl * 5 + k) * DOF + row] = sg[k] * v[(size_t)nb[k] * DOF + row];
-        self = v[(size_t)n * DOF + row];
-    }
-    __syncthreads();
-    if (!ok) return;
-    cplx<R> acc = czero<R>();
-    const cplx<R>* Ln = L + (size_t)n * 5 * DOF * DOF + row;
-    const cplx<R>* s0 = sv + sl * 5 * DOF;
+    for (int tt = 0; tt < MT; tt++) acc[tt] = acc_t{0, 0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-#pragma unroll 8
-        for (int col = 0; col < DOF; col++) acc = acc + Ln[(k * DOF + col) * DOF] * s0[k * DOF + col];
+    for (int h = 0; h < 5; h++) {
+        // B operand: part (g & 1) of column 2s + (g >> 1) -> real index 4s + g of the site
+        const R* vb = vin + (size_t)nb[h] * DOF * 2 + g;
+        R bv[KS];
+#pragma unroll
+        for (int s2 = 0; s2 < KS; s2++) bv[s2] = on ? sg[h] * vb[4 * s2] : R(0);
+        mfma_hop<R, DOF>(acc, Ln + (size_t)h * DOF * DOF * 2, ll, bv);
     }
-    cplx<R> o = (m0 + R(2)) * self - acc;
-    if (RES) o = b[r * ld + (size_t)n * DOF + row] - o;
-    out[r * ld + (size_t)n * DOF + row] = o;
-}
-
-static int lcm64(int dof) {
-    int a = dof, b = 64;
-    while (b) {
-        int t = a % b;
-        a = b;
-        b = t;
+    if (!on) return;
+    const R mass = m0 + R(2);
+    const size_t base = (size_t)j * ld + (size_t)n * DOF;
+#pragma unroll
+    for (int tt = 0; tt < MT; tt++) {
+        const int r0 = 8 * tt + 2 * g;
+        const cplx<R> v0 = in[base + r0], v1 = in[base + r0 + 1];
+        cplx<R> o0 = mk<R>(mass * v0.x - acc[tt][0], mass * v0.y - acc[tt][1]);
+        cplx<R> o1 = mk<R>(mass * v1.x - acc[tt][2], mass * v1.y - acc[tt][3]);
+        if (RES) {
+            o0 = b[base + r0] - o0;
+            o1 = b[base + r0 + 1] - o1;
+        }
+        out[base + r0] = o0;
+        out[base + r0 + 1] = o1;
     }
-    return dof / a * 64;
 }
 
 #define DD_DOF_LIST(X) X(8) X(16) X(24) X(32)
@@ -236,15 +238,14 @@ template <typename R>
 void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx,
                   int Nt, int dof, double m0, int nrhs, uint64_t mask, size_t ld) {
     timer_begin(s, K_COARSE);
-    const int threads = lcm64(dof);
-    const int spb = threads / dof;
-    dim3 grid(cdivu((size_t)Nx * Nt, spb), nrhs);
-    const size_t sh = sizeof(cplx<R>) * spb * 5 * dof;
+    constexpr int kWaves = 4;   // sites per work-group
+    const int top = mask ? 64 - __builtin_clzll(mask) : 1;   // systems up to the highest one set
+    dim3 grid(cdivu((size_t)Nx * Nt, kWaves), cdivu((size_t)std::min(top, nrhs), 16));
     switch (dof) {
 #define DD_CASE(D)                                                                                        \
     case D:                                                                                               \
-        if (b) k_coarse<R, D, true><<<grid, threads, sh, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, spb); \
-        else k_coarse<R, D, false><<<grid, threads, sh, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, spb); \
+        if (b) k_coarse<R, D, true><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+        else k_coarse<R, D, false><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
         break;
         DD_DOF_LIST(DD_CASE)
 #undef DD_CASE

@@ -1,0 +1,89 @@
+// Complex block products on the gfx950 matrix cores (16x16x4 MFMA, FP32 or FP64).
+//
+// A coarse-level hop  out_j += L v_j  (L a DOF x DOF complex link, v_j the neighbour's
+// DOF-vector of system j, 16 systems per tile) is computed as one real product
+//   Out (2DOF x 16) += A (2DOF x 2DOF) B (2DOF x 16)
+// with real rows and columns interleaving re/im:  m = 2r + p, k = 2c + q,
+//   A[m][k] = [[Lr, -Li], [Li, Lr]][p][q],   B[k][j] = (q ? Im : Re) v_j[c].
+// Row tile t covers m in [16t, 16t+16) (complex rows 8t..8t+7), K-step s covers
+// k in [4s, 4s+4) (complex columns 2s, 2s+1); MT = DOF/8 tiles and KS = DOF/2 steps.
+//
+// Lane layout of v_mfma_*_16x16x4 (one wave): lane l supplies A[l&15][l>>4] and
+// B[l>>4][l&15].  The accumulator element e (0..3) of lane l is D[4(l>>4) + e][l&15]
+// for FP32 and D[(l>>4) + 4e][l&15] for FP64 (both measured on gfx950).  The MFMA row
+// index i is mapped to the real row m of the tile so that, in both precisions, lane l
+// ends up holding for system j = l&15 the complex rows r = 8t + 2(l>>4) (elements
+// 0, 1 = re, im) and r + 1 (elements 2, 3):
+//   FP32: m - 16t = i                      (r = 8t + i/2,             p = i&1)
+//   FP64: r = 8t + 2(i&3) + (i>>3),  p = (i>>2)&1.
+#pragma once
+#include "common.h"
+
+namespace dd {
+
+template <typename R>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+// Per-lane addressing of the A operand inside one column-major complex link matrix
+// (viewed as 2*DOF*DOF reals): element (s, t) sits at aoff + (2s*DOF + 8t)*2 and is
+// negated when aneg (the -Li quadrant).
+struct LinkLane {
+    int aoff;
+    bool aneg;
+    template <typename R, int DOF>
+    __device__ __forceinline__ static LinkLane make(int lane) {
+        const int i = lane & 15, g = lane >> 4, q = g & 1;
+        const int rr = sizeof(R) == 4 ? (i >> 1) : 2 * (i & 3) + (i >> 3);   // complex row within the tile
+        const int p = sizeof(R) == 4 ? (i & 1) : ((i >> 2) & 1);
+        LinkLane ll;
+        ll.aoff = ((g >> 1) * DOF + rr) * 2 + (p ^ q);
+        ll.aneg = (p == 0) && (q == 1);
+        return ll;
+    }
+};
+
+template <int DOF>
+__device__ __forceinline__ constexpr int link_step(int s, int t) {
+    return (2 * s * DOF + 8 * t) * 2;
+}
+
+// acc[t] += hop contribution of one complex link matrix Lh (reals) on a B operand
+// given per K-step in bv[s] (already multiplied by the hop's boundary sign).  The link
+// operands are loaded CH K-steps at a time (all MT tiles) ahead of their products.
+template <typename R, int DOF, int CH = DOF / 2>
+__device__ __forceinline__ void mfma_hop(typename Mfma<R>::acc_t (&acc)[DOF / 8], const R* __restrict__ Lh,
+                                         const LinkLane& ll, const R (&bv)[DOF / 2]) {
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+    static_assert(KS % CH == 0, "chunk must divide the K-steps");
+#pragma unroll
+    for (int s0 = 0; s0 < KS; s0 += CH) {
+        R a[CH][MT];
+#pragma unroll
+        for (int s = 0; s < CH; s++)
+#pragma unroll
+            for (int t = 0; t < MT; t++) a[s][t] = Lh[ll.aoff + link_step<DOF>(s0 + s, t)];
+#pragma unroll
+        for (int s = 0; s < CH; s++)
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+                acc[t] = Mfma<R>::mac(ll.aneg ? -a[s][t] : a[s][t], bv[s0 + s], acc[t]);
+    }
+}
+
+}  // namespace dd

@@ -7,4 +7,5 @@ template void sap_blocks0<float>(hipStream_t, const cplx<float>*, const cplx<flo
 template void sap_blocksc<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
                                int, double, int, int, double, int, uint64_t, size_t, const int*, int);
 template std::vector<int> sapc_variants<float>();
+template bool sapc_supported<float>(int, int, int);
 }  // namespace dd

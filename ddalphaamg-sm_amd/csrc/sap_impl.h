@@ -117,6 +117,10 @@ __device__ __forceinline__ void colour_block(int k, int colour, int nbx, int nbt
     }
 }
 
+}  // namespace dd
+#include "sapm.h"
+namespace dd {
+
 // ---- level 0: one site (two spin components) per lane; S = bx*bt lanes per block,
 // 64/S blocks per wave; neighbours through cross-lane permutes, sums through
 // butterflies inside the S-lane segment.  No LDS and no barriers.
@@ -501,14 +505,23 @@ static void launch_sapc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<
 // Kernel variants of the coarse SAP: systems per work-group (NB) and whether the link
 // loads of a hop are issued together.  variant = 2*NB + FULL; sapc_variants lists the
 // ones compiled for precision R (the autotuner in solver.hip times them).
+// Variant 32 (FP32 only) is the matrix-core kernel k_sapm (16 systems per work-group,
+// sapm.h), available for dof <= 24 and blocks of at most 16 sites.
 template <typename R>
 std::vector<int> sapc_variants() {
-    if (sizeof(R) == 4) return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1, 2 * 8 + 0, 2 * 8 + 1};
+    if (sizeof(R) == 4) return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1, 2 * 8 + 0, 2 * 8 + 1, 32};
     return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1};
 }
 
 template <typename R>
-int sapc_default_variant(int nrhs) {
+bool sapc_supported(int variant, int dof, int S) {
+    if (variant == 32) return sizeof(R) == 4 && dof <= 24 && S <= 16;
+    return S * dof <= 512;
+}
+
+template <typename R>
+int sapc_default_variant(int nrhs, int dof, int S) {
+    if (sapc_supported<R>(32, dof, S)) return 32;
     if (nrhs <= 1) return 2 * 1 + 0;
     if (sizeof(R) == 8 || nrhs <= 4) return 2 * 4 + 0;
     return 2 * 8 + 0;
@@ -532,6 +545,11 @@ void dispatch_sapc(int variant, hipStream_t s, const cplx<R>* L, const cplx<R>* 
         case 17:
             if constexpr (sizeof(R) == 4) DD_V(8, true);
             else throw BadShape("sap_blocksc: variant not compiled for FP64");
+            break;
+        case 32:
+            if constexpr (sizeof(R) == 4 && D <= 24)
+                launch_sapm<D>(s, L, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs, mask, ld, gate);
+            else throw BadShape("sap_blocksc: matrix-core variant needs FP32 and dof <= 24");
             break;
         default:
             throw BadShape("sap_blocksc: unknown variant");
@@ -562,9 +580,8 @@ void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, 
     timer_begin(s, K_SAPC);
     const int S = bx * bt;
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
-    const int threads = ((S * dof + 63) / 64) * 64;
-    if (threads > 512) throw BadShape("coarse SAP block too large (sites*dof > 512)");
-    if (variant < 0) variant = sapc_default_variant<R>(nrhs);
+    if (variant < 0) variant = sapc_default_variant<R>(nrhs, dof, S);
+    if (!sapc_supported<R>(variant, dof, S)) throw BadShape("coarse SAP: variant unsupported for this block/dof");
     switch (dof) {
 #define DD_CASE(D)                                                                                             \
     case D:                                                                                                    \

@@ -1,0 +1,360 @@
+// Coarse-level SAP block solves on the matrix cores (FP32), 16 systems per work-group.
+//
+// Same algorithm as k_sapc (block GMRES(5) of the SAP smoother with the reference's
+// control flow, src/fgmres.cpp:5-127 with the GMRES_D_B hooks include/sap.h:14-83,
+// x0 = block right-hand side src/sap.cpp:65; block operator
+// Level::SAP_level_l::D_local src/level.cpp:607-653), organised around the MFMA
+// complex block product of mfma.h:
+//   * wave w owns the block sites w + W*u (u < SPW); a lane owns, for system j = lane&15
+//     and each owned site, the complex rows 8t + 2(lane>>4) + {0, 1} of every row tile t
+//     (the accumulator layout of the 16x16x4 product), so the Krylov basis, w and the
+//     iterate updates live in registers in exactly the layout the products produce;
+//   * D_B v: v is staged in LDS as [site][row][system] (the B operand layout, read
+//     without bank conflicts), every link element of the block is read once per
+//     product for all 16 systems;
+//   * per-system reductions: xor-shuffles over the four lane groups, then one LDS
+//     slot per wave (double-buffered), summed in a fixed order so every lane of a
+//     system sees the same bits;
+//   * the Givens/back-substitution scalars of system j live in LDS and are updated by
+//     lane j of wave 0; per-system flags are uniform 16-bit masks (ballots).
+#pragma once
+#include "mfma.h"
+
+namespace dd {
+
+template <int DOF, int SPW>
+struct SapmVec {
+    static constexpr int MT = DOF / 8, N = SPW * MT * 2;
+    cplx<float> e[N];   // e[(u*MT + t)*2 + h]: site u, complex row 8t + 2g + h
+};
+
+template <int DOF>
+constexpr size_t sapm_lds_bytes(int S) {
+    return sizeof(cplx<float>) * ((size_t)3 * S * DOF * 16 + 2 * 16 * 16 + 16 * (Tri<kSapM>::size + (kSapM + 1) + 3 * kSapM)) +
+           sizeof(float) * 16 + sizeof(int) * 20;
+}
+
+template <int DOF, int SPW>
+__global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L, const cplx<float>* __restrict__ rr,
+                                              cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
+                                              int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
+                                              const int* __restrict__ gate, int nrhs) {
+    using R = float;
+    using acc_t = Mfma<R>::acc_t;
+    using Vec = SapmVec<DOF, SPW>;
+    constexpr int M = kSapM, MT = DOF / 8, KS = DOF / 2, TS = Tri<M>::size, NE = Vec::N;
+    extern __shared__ unsigned char smem_raw[];
+    const int S = bx * bt, W = blockDim.x >> 6;
+    cplx<R>* stg = reinterpret_cast<cplx<R>*>(smem_raw);   // [S][DOF][16] product operand
+    cplx<R>* sphi = stg + S * DOF * 16;                   // [S][DOF][16] block right-hand sides
+    cplx<R>* sx = sphi + S * DOF * 16;                    // [S][DOF][16] block iterates
+    cplx<R>* red = sx + S * DOF * 16;                     // [2][16 waves max][16]
+    cplx<R>* Hs = red + 2 * 16 * 16;                      // [16][TS]
+    cplx<R>* gs = Hs + 16 * TS;                           // [16][M+1]
+    cplx<R>* cns = gs + 16 * (M + 1);                     // [16][M]
+    cplx<R>* sns = cns + 16 * M;                          // [16][M]
+    cplx<R>* etas = sns + 16 * M;                         // [16][M]
+    int* maxIts = reinterpret_cast<int*>(reinterpret_cast<R*>(etas + 16 * M) + 16);   // [16]
+    int* flag = maxIts + 16;                                                          // [1]
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
+    const bool lead = (wv == 0) && (g == 0);   // lane j of wave 0: scalars of system j
+    const int nbx = Nx / bx, nbt = Nt / bt;
+    int BX, BT;
+    colour_block(blockIdx.x, colour, nbx, nbt, BX, BT);
+    const int r0sys = blockIdx.y * 16;
+    unsigned onm = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int rh = r0sys + q;
+        if (rh < nrhs && rhs_on(mask, rh) && (!gate || gate[rh] != 0)) onm |= 1u << q;
+    }
+    if (!onm) return;
+    const int rh = r0sys + j;
+    const bool on = (onm >> j) & 1;
+
+    // owned sites: local index, global site, neighbour local indices (-1 outside the block)
+    int kl[SPW], gn[SPW], nbl[SPW][5];
+    bool own[SPW];
+#pragma unroll
+    for (int u = 0; u < SPW; u++) {
+        const int k = wv + W * u;
+        own[u] = k < S;
+        const int kk = own[u] ? k : 0;
+        const int lx = kk / bt, lt = kk - lx * bt;
+        kl[u] = kk;
+        gn[u] = (BX * bx + lx) * Nt + BT * bt + lt;
+        nbl[u][0] = kk;
+        nbl[u][1] = lt < bt - 1 ? kk + 1 : -1;
+        nbl[u][2] = lx < bx - 1 ? kk + bt : -1;
+        nbl[u][3] = lt > 0 ? kk - 1 : -1;
+        nbl[u][4] = lx > 0 ? kk - bt : -1;
+    }
+    auto lidx = [&](int u, int t, int h) { return (kl[u] * DOF + 8 * t + 2 * g + h) * 16 + j; };
+    const LinkLane ll = LinkLane::make<R, DOF>(lane);
+    const R mass = m0 + R(2);
+    const R* stgf = reinterpret_cast<const R*>(stg);
+
+    // v <- D_B v (zero outside the block, no boundary sign).  The caller reaches a
+    // work-group barrier (a reduction) before the next call re-stages.
+    auto apply = [&](Vec& v) {
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+            if (own[u]) {
+#pragma unroll
+                for (int t = 0; t < MT; t++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) stg[lidx(u, t, h)] = v.e[(u * MT + t) * 2 + h];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < SPW; u++) {
+            if (!own[u]) continue;
+            acc_t acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
+            const R* Ln = reinterpret_cast<const R*>(L + (size_t)gn[u] * 5 * DOF * DOF);
+#pragma unroll 1
+            for (int hop = 0; hop < 5; hop++) {
+                const int nk = nbl[u][hop];
+                if (nk < 0) continue;
+                R bv[KS];
+#pragma unroll
+                for (int s2 = 0; s2 < KS; s2++) bv[s2] = stgf[((nk * DOF + 2 * s2 + (g >> 1)) * 16 + j) * 2 + (g & 1)];
+                mfma_hop<R, DOF, (DOF / 2) % 4 == 0 ? 4 : DOF / 2>(acc, Ln + (size_t)hop * DOF * DOF * 2, ll, bv);
+            }
+#pragma unroll
+            for (int t = 0; t < MT; t++) {
+                cplx<R>& a0 = v.e[(u * MT + t) * 2];
+                cplx<R>& a1 = v.e[(u * MT + t) * 2 + 1];
+                a0 = mk<R>(mass * a0.x - acc[t][0], mass * a0.y - acc[t][1]);
+                a1 = mk<R>(mass * a1.x - acc[t][2], mass * a1.y - acc[t][3]);
+            }
+        }
+    };
+    // total over the block of this lane's partial, per system j (identical bits in every
+    // lane of the system)
+    int buf = 0;
+    auto rsum = [&](cplx<R> p) -> cplx<R> {
+        p.x += __shfl_xor(p.x, 16, 64);
+        p.y += __shfl_xor(p.y, 16, 64);
+        p.x += __shfl_xor(p.x, 32, 64);
+        p.y += __shfl_xor(p.y, 32, 64);
+        cplx<R>* rb = red + buf * 256;
+        if (g == 0) rb[wv * 16 + j] = p;
+        __syncthreads();
+        cplx<R> t = czero<R>();
+        for (int w = 0; w < W; w++) t = t + rb[w * 16 + j];
+        buf ^= 1;
+        return t;
+    };
+    auto dotc = [&](const Vec& a, const Vec& b) -> cplx<R> {   // sum a * conj(b) over owned rows
+        cplx<R> p = czero<R>();
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+            if (own[u])
+#pragma unroll
+                for (int e = 0; e < 2 * MT; e++) p = p + mulc(a.e[u * 2 * MT + e], b.e[u * 2 * MT + e]);
+        return p;
+    };
+    auto nrm2 = [&](const Vec& a) -> R {
+        R p = 0;
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+            if (own[u])
+#pragma unroll
+                for (int e = 0; e < 2 * MT; e++) p += norm2(a.e[u * 2 * MT + e]);
+        return rsum(mk<R>(p, R(0))).x;
+    };
+    auto load_lds = [&](Vec& v, const cplx<R>* src) {
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) v.e[(u * MT + t) * 2 + h] = own[u] ? src[lidx(u, t, h)] : czero<R>();
+    };
+
+    auto resid = [&](Vec& r, const Vec& dw) {   // r = phi - dw
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int e = (u * MT + t) * 2 + h;
+                    r.e[e] = (own[u] ? sphi[lidx(u, t, h)] : czero<R>()) - dw.e[e];
+                }
+    };
+
+    // right-hand sides (initial guess = right-hand side)
+    Vec w, V[M];
+    {
+        const cplx<R>* rv = rr + (size_t)(on ? rh : 0) * ld;
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const cplx<R> ph =
+                        (own[u] && on) ? rv[(size_t)gn[u] * DOF + 8 * t + 2 * g + h] : czero<R>();
+                    if (own[u]) {
+                        sphi[lidx(u, t, h)] = ph;
+                        sx[lidx(u, t, h)] = ph;
+                    }
+                    w.e[(u * MT + t) * 2 + h] = ph;
+                }
+    }
+    const R nphi = sqrt(nrm2(w));
+    apply(w);
+    resid(V[0], w);
+    R err = sqrt(nrm2(V[0]));
+    if (lead) maxIts[j] = M;
+
+    unsigned livem = onm;
+    for (int kk = 0; kk < restarts && livem; kk++) {
+        unsigned runm = livem;
+        const bool live = (livem >> j) & 1;
+        if (live) {
+            const R ie = R(1) / err;
+#pragma unroll
+            for (int e = 0; e < NE; e++) V[0].e[e] = ie * V[0].e[e];
+        }
+        if (lead && live) gs[j * (M + 1)] = mk<R>(err, R(0));
+#pragma unroll
+        for (int jj = 0; jj < M; jj++) {
+            if (!runm) break;
+            const bool run = (runm >> j) & 1;
+            w = V[jj];
+            apply(w);
+#pragma unroll
+            for (int ii = 0; ii <= jj; ii++) {
+                const cplx<R> h = rsum(dotc(w, V[ii]));
+                if (lead && run) Hs[j * TS + Tri<M>::at(ii, jj)] = h;
+                if (run)
+#pragma unroll
+                    for (int e = 0; e < NE; e++) w.e[e] = w.e[e] - h * V[ii].e[e];
+            }
+            const R hv = sqrt(nrm2(w));
+            if (jj + 1 < M && run && hv > R(0)) {
+                const R ih = R(1) / hv;
+#pragma unroll
+                for (int e = 0; e < NE; e++) V[jj + 1].e[e] = ih * w.e[e];
+            }
+            bool broke = false;
+            if (lead && run) {
+                // Givens rotations of column jj (src/fgmres.cpp:99-115) for system j
+                cplx<R>* H = Hs + j * TS;
+                cplx<R>* cn = cns + j * M;
+                cplx<R>* sn = sns + j * M;
+                cplx<R>* gg = gs + j * (M + 1);
+                for (int i2 = 0; i2 < jj; i2++) {
+                    const cplx<R> a = H[Tri<M>::at(i2, jj)], bb = H[Tri<M>::at(i2 + 1, jj)];
+                    H[Tri<M>::at(i2, jj)] = cmul(cn[i2], a) + cmul(sn[i2], bb);
+                    H[Tri<M>::at(i2 + 1, jj)] = cn[i2] * bb - sn[i2] * a;
+                }
+                const cplx<R> hjj = H[Tri<M>::at(jj, jj)];
+                const cplx<R> hs = mk<R>(hv, R(0));
+                const R den = sqrt(norm2(hjj) + norm2(hs));
+                sn[jj] = mk<R>(hs.x / den, hs.y / den);
+                cn[jj] = mk<R>(hjj.x / den, hjj.y / den);
+                H[Tri<M>::at(jj, jj)] = cmul(cn[jj], hjj) + cmul(sn[jj], hs);
+                gg[jj + 1] = -(sn[jj] * gg[jj]);
+                gg[jj] = cmul(cn[jj], gg[jj]);
+                if (sqrt(norm2(gg[jj + 1])) < tol * nphi) {
+                    maxIts[j] = jj + 1;
+                    broke = true;
+                }
+            }
+            const unsigned brk = (unsigned)__ballot(broke) & 0xffffu;   // wave 0 only
+            if (wv == 0 && lane == 0) *flag = (int)brk;
+            __syncthreads();
+            runm &= ~(unsigned)(*flag);
+        }
+        // back substitution (src/fgmres.cpp:118-127) by lane j of wave 0
+        if (lead && live) {
+            const int nI = maxIts[j];
+            const cplx<R>* H = Hs + j * TS;
+            const cplx<R>* gg = gs + j * (M + 1);
+            cplx<R>* eta = etas + j * M;
+            for (int i2 = nI - 1; i2 >= 0; i2--) {
+                cplx<R> e = gg[i2];
+                for (int j2 = i2 + 1; j2 < nI; j2++) e = e - H[Tri<M>::at(i2, j2)] * eta[j2];
+                eta[i2] = cdiv(e, H[Tri<M>::at(i2, i2)]);
+            }
+        }
+        __syncthreads();
+        load_lds(w, sx);
+        if (live) {
+            const int nI = maxIts[j];
+#pragma unroll
+            for (int i2 = 0; i2 < M; i2++)
+                if (i2 < nI) {
+                    const cplx<R> et = etas[j * M + i2];
+#pragma unroll
+                    for (int e = 0; e < NE; e++) w.e[e] = w.e[e] + et * V[i2].e[e];
+                }
+#pragma unroll
+            for (int u = 0; u < SPW; u++)
+                if (own[u])
+#pragma unroll
+                    for (int t = 0; t < MT; t++)
+#pragma unroll
+                        for (int h = 0; h < 2; h++) sx[lidx(u, t, h)] = w.e[(u * MT + t) * 2 + h];
+        }
+        apply(w);
+        resid(V[0], w);
+        const R e2 = nrm2(V[0]);
+        const bool conv = live && sqrt(e2) < tol * nphi;
+        if (live) err = sqrt(e2);
+        const unsigned cm = (unsigned)__ballot(conv && g == 0) & 0xffffu;
+        livem &= ~cm;
+    }
+    if (on) {
+        cplx<R>* xv = xx + (size_t)rh * ld;
+#pragma unroll
+        for (int u = 0; u < SPW; u++)
+            if (own[u])
+#pragma unroll
+                for (int t = 0; t < MT; t++)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const size_t o = (size_t)gn[u] * DOF + 8 * t + 2 * g + h;
+                        xv[o] = xv[o] + sx[lidx(u, t, h)];
+                    }
+    }
+}
+
+template <int DOF>
+static void launch_sapm(hipStream_t s, const cplx<float>* L, const cplx<float>* r, cplx<float>* x, int Nx, int Nt,
+                        int bx, int bt, int colour, double m0, int restarts, double tol, int nrhs, uint64_t mask,
+                        size_t ld, const int* gate) {
+    const int S = bx * bt;
+    if (S > 16) throw BadShape("matrix-core coarse SAP: at most 16 sites per block");
+    const size_t sh = sapm_lds_bytes<DOF>(S);
+    if (sh > 160 * 1024) throw BadShape("matrix-core coarse SAP: block does not fit the LDS");
+    const int nblk = (Nx / bx) * (Nt / bt) / 2;
+    dim3 grid(nblk, (nrhs + 15) / 16);
+    if (S > 8) {   // two sites per wave
+        static bool attr = false;
+        if (!attr) {
+            DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sapm<DOF, 2>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = true;
+        }
+        k_sapm<DOF, 2><<<grid, 64 * ((S + 1) / 2), sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts,
+                                                           (float)tol, mask, ld, gate, nrhs);
+    } else {
+        static bool attr = false;
+        if (!attr) {
+            DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sapm<DOF, 1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr = true;
+        }
+        k_sapm<DOF, 1><<<grid, 64 * S, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
+                                                mask, ld, gate, nrhs);
+    }
+}
+
+}  // namespace dd

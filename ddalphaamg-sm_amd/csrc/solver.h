@@ -141,6 +141,8 @@ class Multigrid {
     // times every compiled coarse-SAP variant on the built hierarchy (a batch of
     // max_rhs pseudo-random systems, one colour) and keeps the fastest per level
     void autotune();
+    // uses coarse-SAP kernel variant v on every coarse level (BadShape if unavailable)
+    void set_sap_variant(int v);
     void tuning(int l, int* variant, double* ms) const {
         *variant = (l >= 0 && l < (int)sapc_var_.size()) ? sapc_var_[l] : -1;
         *ms = (l >= 0 && l < (int)sapc_ms_.size()) ? sapc_ms_[l] : 0.0;

@@ -501,7 +501,8 @@ void Multigrid<R>::autotune() {
         double best = 1e300;
         int bestv = -1;
         for (int v : sapc_variants<R>()) {
-            if (v / 2 > 1 && nr == 1) continue;   // multi-system variants only for batches
+            if (!sapc_supported<R>(v, G.dof, G.bx * G.bt)) continue;
+            if (v != 32 && v / 2 > 1 && nr == 1) continue;   // multi-system vector variants only for batches
             double t = 0;
             for (int rep = 0; rep < 2; rep++) {
                 zero<R>(s_, x, n * nr, 0, 1, 1);
@@ -520,14 +521,25 @@ void Multigrid<R>::autotune() {
             }
             if (std::getenv("DDAMG_VERBOSE"))
                 fprintf(stderr, "autotune level %d: coarse SAP variant %d (%d systems/work-group%s): %.3f ms\n", l, v,
-                        v / 2, (v & 1) ? ", batched link loads" : "", t);
+                        v == 32 ? 16 : v / 2, v == 32 ? ", matrix cores" : ((v & 1) ? ", batched link loads" : ""), t);
         }
-        if (const char* f = std::getenv("DDAMG_SAPC_VARIANT")) bestv = std::atoi(f);   // explicit override
         sapc_var_[l] = bestv;
         sapc_ms_[l] = best;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+}
+
+template <typename R>
+void Multigrid<R>::set_sap_variant(int v) {
+    const int L = levels();
+    sapc_var_.assign(L, -1);
+    sapc_ms_.assign(L, 0.0);
+    for (int l = 1; l < L - 1; l++) {
+        if (!sapc_supported<R>(v, g_[l].dof, g_[l].bx * g_[l].bt))
+            throw BadShape("sap_variant not available for this level's precision, dof or block");
+        sapc_var_[l] = v;
+    }
 }
 
 template class Krylov<double>;
@@ -593,13 +605,15 @@ void Solver::setup(const dd_params& p) {
     if (p.precision == 0) {
         mg64.reset(new Multigrid<double>(s, Nx, Nt, p, U64.as<cplx<double>>()));
         mg64->setup(tv.as<cplx<double>>(), rng);
-        if (p.autotune) mg64->autotune();
+        if (p.sap_variant >= 0) mg64->set_sap_variant(p.sap_variant);
+        else if (p.autotune) mg64->autotune();
     } else {
         DevMem tv32(sizeof(cplx<float>) * h.size());
         convert<float, double>(s, tv32.as<cplx<float>>(), tv.as<cplx<double>>(), h.size(), 0, 0, 1, 1);
         mg32.reset(new Multigrid<float>(s, Nx, Nt, p, U32.as<cplx<float>>()));
         mg32->setup(tv32.as<cplx<float>>(), rng);
-        if (p.autotune) mg32->autotune();
+        if (p.sap_variant >= 0) mg32->set_sap_variant(p.sap_variant);
+        else if (p.autotune) mg32->autotune();
     }
     DD_HIP(hipStreamSynchronize(s));
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -64,6 +64,9 @@ typedef struct dd_params {
     int max_rhs;                   /* largest batch solve() will be asked for (1)   */
     int autotune;                  /* 1: time the coarse-SAP kernel variants on the
                                       built hierarchy at setup and keep the fastest  */
+    int sap_variant;               /* coarse-SAP kernel variant for every coarse level
+                                      (2*systems_per_workgroup + batched_link_loads;
+                                      32 = matrix cores, FP32); -1: autotuned/default */
 } dd_params;
 
 typedef struct dd_solve_info {

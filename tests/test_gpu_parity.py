@@ -184,3 +184,44 @@ def test_coarse_sap_batched(precision):
             for r in range(3):
                 ref, _ = orc.sap(op, blocks, v[r], x0[r], 2, 1e-10)
                 assert rel(ctx.sap(1, v[r], x0[r], 2), ref) < 1e-11
+
+
+@pytest.mark.parametrize('blk', [(4, 4), (2, 2)])
+def test_coarse_sap_matrix_cores(blk):
+    """FP32 coarse SAP on the matrix cores (variant 32, 16 systems per work-group).
+    (a) With a tolerance FP32 cannot reach, block GMRES runs all 5x5 iterations, so the
+    control flow is fixed; against the oracle's FP64 SAP on the same (FP32) level-1
+    operator the matrix-core result must be as accurate as the FP32 vector kernel
+    (variant 2): error at most twice the vector kernel's (or 1e-5) and below 1e-3.
+    (b) Batches of 1, 3 and 17 systems (two 16-system tiles) give each system the same
+    result as alone: the product columns and reductions of a system do not depend on
+    the others."""
+    rng = np.random.default_rng(33)
+    N = 64
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.4)
+    out, ref = {}, None
+    v = x0 = None
+    for variant in (32, 2):
+        prm = ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), blk], nv=[12, 8], m0=-0.2,
+                           precision=1, max_rhs=17, rng_ranks=(2, 2), sap_variant=variant)
+        with ddamg.Context(N, N) as ctx:
+            ctx.set_gauge(U)
+            ctx.setup(prm)
+            assert ctx.tuning(1)[0] == variant
+            shape = ctx.level_shape(1)
+            assert shape[2] == 24
+            if v is None:
+                v = rng.standard_normal((17,) + shape) + 1j * rng.standard_normal((17,) + shape)
+                x0 = np.zeros_like(v)
+                A, B, Cm = ctx.links(1, precision=1)
+                op = orc.Stencil(A, B, Cm, -0.2)
+                blocks = orc.SapBlocks(shape[0], shape[1], *blk)
+                ref = np.stack([orc.sap(op, blocks, v[r], x0[r], 2, 1e-12)[0] for r in range(3)])
+            out[variant] = ctx.sap(1, v[:3], x0[:3], 2, tol=1e-12, precision=1)
+            if variant == 32:
+                xb = ctx.sap(1, v, x0, 2, precision=1)
+                for r in (0, 5, 16):
+                    assert rel(xb[r], ctx.sap(1, v[r], x0[r], 2, precision=1)) < 1e-6
+    e32, e2 = rel(out[32], ref), rel(out[2], ref)
+    print(f'blocks {blk}: FP32 error vs FP64 oracle: matrix cores {e32:.2e}, vector kernel {e2:.2e}')
+    assert e32 < max(2 * e2, 1e-5) and e32 < 1e-3
