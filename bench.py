@@ -126,10 +126,14 @@ def main():
     relres = np.linalg.norm((B - Dx).reshape(nrhs, -1), axis=1) / np.linalg.norm(B.reshape(nrhs, -1), axis=1)
     log(f'mixed batched: {t_mixed * 1e3:.1f} ms per {nrhs}-RHS solve, its {its.min()}..{its.max()}, '
         f'max relres {relres.max():.3e}')
+    cst = ctx.coarsest_stats()
+    nsolve = args.warmup + args.steps
+    coarsest_mixed = {k: (v / nsolve if k not in ('persistent',) else v) for k, v in cst.items()}
+    log('coarsest solver per outer solve: ' + json.dumps(coarsest_mixed))
     ctx.close()
 
     # ---- plain FP64, one RHS -------------------------------------------------------------
-    t_plain, its_plain, err_vs_fp64, setup_plain = None, None, None, None
+    t_plain, its_plain, err_vs_fp64, setup_plain, coarsest_plain = None, None, None, None, None
     if not args.skip_plain:
         ctx = ddamg.Context(N, N)
         ctx.set_gauge_theta(theta)
@@ -146,6 +150,10 @@ def main():
         err_vs_fp64 = float(np.linalg.norm(x_mixed[0] - x64) / np.linalg.norm(x64))
         log(f'plain FP64: setup {setup_plain:.3f}s, solve {t_plain * 1e3:.1f} ms, its {its_plain[0]}, '
             f'relres {info_p.relres:.3e}, mixed-vs-FP64 rel. error {err_vs_fp64:.3e}')
+        cst = ctx.coarsest_stats()
+        nsolve = 1 if args.skip_mixed else 2
+        coarsest_plain = {k: (v / nsolve if k not in ('persistent',) else v) for k, v in cst.items()}
+        log('coarsest solver per plain solve: ' + json.dumps(coarsest_plain))
         ctx.close()
 
     # ---- CPU baseline: the reference's own solve on the host cores --------------------------
@@ -174,6 +182,8 @@ def main():
         'cpu_baseline': cpu,
         'roofline': roofline,
         'kernel_breakdown_ms_per_solve': breakdown,
+        'coarsest_solver_mixed': coarsest_mixed,
+        'coarsest_solver_plain': coarsest_plain,
     }
     print(json.dumps(line))
 

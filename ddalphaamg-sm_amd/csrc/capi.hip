@@ -128,7 +128,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->rng_ranks_t = 2;
     p->ortho_outer = 0;
     p->ortho_inner = 0;
-    p->ortho_coarse = 1;
+    p->ortho_coarse = 2;
     p->max_rhs = 1;
     p->autotune = 1;
     p->sap_variant = -1;
@@ -322,6 +322,30 @@ int dd_get_tuning(dd_ctx* c, int level, int* variant, double* ms) {
         else c->s->mg32->tuning(level, &v, &t);
         if (variant) *variant = v;
         if (ms) *ms = t;
+        return DD_OK;
+    });
+}
+
+int dd_coarsest_stats(dd_ctx* c, double* out) {
+    if (check(c) || !out) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
+        for (int k = 0; k < 10; k++) out[k] = 0;
+        int dev = 0, khz = 0;
+        DD_HIP(hipGetDevice(&dev));
+        DD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        auto fill = [&](auto* m) {
+            const long long* st = nullptr;
+            const double* clk = nullptr;
+            long long calls = 0;
+            m->coarsest_stats(&st, &clk, &calls);
+            out[0] = (double)calls;
+            out[1] = st ? (double)*st : 0.0;
+            for (int k = 0; k < 7 && clk; k++) out[2 + k] = khz > 0 ? clk[k] / khz : 0.0;
+            out[9] = clk ? 1.0 : 0.0;
+        };
+        if (c->s->mg64) fill(c->s->mg64.get());
+        else fill(c->s->mg32.get());
         return DD_OK;
     });
 }

@@ -1,0 +1,40 @@
+// Coarsest-level GMRES as one persistent cooperative kernel (coarsest.hip).
+//
+// Solves D_c psi = eta on the coarsest level for a batch of systems with the GMRES of
+// GMRES_level_l (include/level.h:70-140; FGMRES::fgmres src/fgmres.cpp:5-127 with the
+// identity preconditioner): x0 = eta (src/amg.cpp:69,124), restart length m, `restarts`
+// cycles, break on |g_{j+1}| < tol ||eta||, maxIt kept across restarts, true residual
+// after each cycle, count k*m + maxIt.  The Arnoldi basis is orthogonalised by classical
+// Gram-Schmidt with delayed re-orthogonalisation (DCGS2: one global reduction and two
+// passes over the basis per step); the Hessenberg matrix, Givens rotations and the
+// triangular solve run on the device, so one launch performs the whole solve.
+#pragma once
+#include "devmem.h"
+#include "kernels.h"
+
+namespace dd {
+
+template <typename R>
+class CoarsestGmres {
+   public:
+    CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap);
+    // false when the lattice is too large for one co-resident grid (the caller then
+    // uses the launch-per-step Krylov engine)
+    bool supported() const { return ok_; }
+    // psi = D_c^-1 eta for the systems in mask (x0 = eta); iters[r] per system
+    void solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int* iters);
+    long long steps = 0;   // Arnoldi steps taken (cumulative)
+    // wall-clock ticks of work-group 0 per phase (cumulative): stencil + reduction,
+    // barrier, scalar phase, barrier, update, barrier, cycle end
+    double phase_clk[7] = {0, 0, 0, 0, 0, 0, 0};
+
+   private:
+    hipStream_t s_;
+    int Nx_, Nt_, dof_, m_, restarts_, cap_, G_ = 0, spw_ = 0;
+    double tol_;
+    bool ok_ = false;
+    DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_;
+    PinMem ctrlh_;
+};
+
+}  // namespace dd

@@ -1,0 +1,707 @@
+// Persistent cooperative coarsest-level GMRES (gfx950), see coarsest.h.
+//
+// One launch solves the coarsest system for a batch: a grid of G co-resident
+// work-groups (8 waves each) owns the lattice in slices of `spw` sites; work-group s
+// also owns the scalar state of system s (Givens rotations, right-hand side of the
+// least-squares problem, last Hessenberg column) in its LDS.  Grid barriers separate the
+// phases of every Arnoldi step.
+//
+// Arnoldi with delayed re-orthogonalisation (DCGS2).  Before step j the basis
+// q_0..q_{j-1} is final and u_j = A q_{j-1} - sum_i h1_{i,j-1} q_i has had one
+// projection.  Step j:
+//   A  w = A u_j (matrix-core stencil); per work-group partial sums of
+//      a_i = <q_i,u_j>, b_i = <q_i,w> (i < j), c = <u_j,u_j>, d = <u_j,w>: one pass over
+//      the basis, 8 (FP32) or 4 (FP64) vectors in flight, their 32 or 16 partial sums
+//      reduced across the wave with about as many shuffles;
+//   B  (work-group s) beta = sqrt(c - |a|^2); column j-1 of the Hessenberg matrix is
+//      h1_{.,j-1} + a with beta below (the second projection, one step late); Givens
+//      rotation of that column and the break test |g_j| < tol ||eta||; first-pass
+//      coefficients of the next column h1_{i,j} = b_i/beta, h1_{j,j} = (d - a^H b)/beta^2;
+//   C  q_j = (u_j - Q a)/beta,  u_{j+1} = w/beta - Q_j h1_{.,j}  (one pass).
+// Using w/beta for A q_j leaves an error A Q a/beta inside span(Q_j); the next step's
+// projection removes it and adds it to the Hessenberg column, so the Arnoldi relation
+// holds to working precision.  The break of column j-1 is seen at step j, one stencil
+// application later than with modified Gram-Schmidt; the solution only uses the
+// columns before the break.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "coarsest.h"
+#include "mfma.h"
+
+namespace cg = cooperative_groups;
+
+namespace dd {
+
+namespace {
+
+constexpr int kCT = 512;     // threads per work-group
+constexpr int kNW = kCT / 64;
+constexpr int kSpwMax = 8;   // sites per work-group at most
+constexpr int kMaxM = 448;   // restart length at most
+constexpr int kBB = 32;      // block of the triangular solve
+
+enum : int { F_LIVE = 1, F_RUN = 2 };
+
+struct Ctrl {
+    int flags, maxit, iters, pad;
+};
+
+template <typename R>
+struct CArgs {
+    const cplx<R>* L;       // links of the coarsest level
+    const cplx<R>* eta;     // right-hand sides [nsys][n]
+    cplx<R>* x;             // solutions [nsys][n]
+    cplx<R>* Q;             // basis [m+1][nsys][n]
+    cplx<R>* u;             // [nsys][n]
+    cplx<R>* w;             // [nsys][n]
+    cplx<double>* part;     // [nsys][2m+2][G] per-work-group partial sums
+    cplx<double>* Hrot;     // [nsys][m+1][m] rotated (triangular) Hessenberg columns
+    cplx<double>* etals;    // [nsys][m] least-squares solution
+    cplx<double>* coef;     // [nsys][2m+2]: a_i (i<j) | h1_{i,j} (i<=j) | 1/beta
+    double* nrm;            // [nsys][2]: ||eta||, ||r||
+    Ctrl* ctrl;             // [nsys]
+    double* stats;          // [2 + 8]: algorithmic bytes, Arnoldi steps, phase clocks of
+                            // work-group 0 (A, barrier, B, barrier, C, barrier, cycle end)
+    uint64_t mask;
+    int Nx, Nt, nsys, m, restarts, spw;
+    R m0;
+    double tol;
+};
+
+__device__ __forceinline__ cplx<double> cdivr(cplx<double> a, double d) { return mk<double>(a.x / d, a.y / d); }
+
+// Sums NV values (a power of two <= 32) over the 64 lanes of a wave with about NV
+// shuffles (each step hands half of the values to the partner lane): afterwards the
+// lanes with f = lane >> (6 - log2 NV) hold the total of value f.
+template <typename R, int NV>
+__device__ __forceinline__ R butterfly(R (&F)[NV], int lane) {
+    constexpr int LG = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : NV == 4 ? 2 : NV == 2 ? 1 : 0;
+    static_assert((1 << LG) == NV, "NV must be a power of two <= 32");
+#pragma unroll
+    for (int st = 0; st < LG; st++) {
+        const int o = 32 >> st, half = (NV / 2) >> st;
+        const bool upper = (lane & o) != 0;
+#pragma unroll
+        for (int k = 0; k < half; k++) {
+            const R send = upper ? F[k] : F[k + half];
+            const R keep = upper ? F[k + half] : F[k];
+            F[k] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    R t = F[0];
+#pragma unroll
+    for (int o = (32 >> LG); o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    return t;
+}
+
+template <typename R>
+struct Chunk {   // basis vectors in flight per pass
+    static constexpr int CH = sizeof(R) == 4 ? 8 : 4;
+};
+
+template <typename R, int DOF>
+__global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
+    constexpr int RPL = (kSpwMax * DOF + 63) / 64;   // own rows per lane (at most)
+    constexpr int kCH = Chunk<R>::CH, NVAL = 4 * kCH, LGV = kCH == 8 ? 5 : 4;
+    cg::grid_group grid = cg::this_grid();
+    extern __shared__ unsigned char smem_raw[];
+    const int m = a.m, VALS = 2 * m + 2, CST = 2 * m + 2;
+    // LDS: per-system state of the work-group's system, then scratch
+    cplx<double>* tot = reinterpret_cast<cplx<double>*>(smem_raw);   // [VALS] step totals
+    cplx<double>* h1p = tot + VALS;                                    // [m+1] first-pass column
+    cplx<double>* csl = h1p + (m + 1);                                 // [m]
+    cplx<double>* snl = csl + m;                                       // [m]
+    cplx<double>* gls = snl + m;                                       // [m+1]
+    cplx<double>* hcol = gls + (m + 1);                                // [m+1]
+    double* sc = reinterpret_cast<double*>(hcol + (m + 1));            // [4]
+    int* si = reinterpret_cast<int*>(sc + 4);                          // [4]
+    uint64_t* smask = reinterpret_cast<uint64_t*>(si + 4);             // [1]
+    unsigned char* scr = reinterpret_cast<unsigned char*>(smask + 2);  // scratch (phase C, x update, solve)
+    cplx<R>* scoef = reinterpret_cast<cplx<R>*>(scr);                  // [kNW][2(m+1)]
+    cplx<double>* blk = reinterpret_cast<cplx<double>*>(scr);          // [kBB][kBB]
+
+    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int V = a.Nx * a.Nt;
+    const size_t n = (size_t)V * DOF, vs = n * (size_t)a.nsys;
+    const int s0 = wg * a.spw, s1 = min(V, s0 + a.spw);
+    const int nrow = max(0, s1 - s0) * DOF;
+    const size_t row0 = (size_t)s0 * DOF;
+    const int ntile = (a.nsys + 15) / 16;
+    const double sR = sizeof(cplx<R>);
+    const int sys = wg;   // the system whose scalars this work-group owns (G >= nsys)
+    const bool owner = sys < a.nsys && ((a.mask >> sys) & 1ull);
+
+    auto sysmask = [&](int flag) -> uint64_t {   // uniform: read after a grid barrier
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t mk2 = 0;
+            for (int q = 0; q < a.nsys; q++)
+                if (((a.mask >> q) & 1ull) &&
+                    (__hip_atomic_load(&a.ctrl[q].flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & flag))
+                    mk2 |= 1ull << q;
+            *smask = mk2;
+        }
+        __syncthreads();
+        return *smask;
+    };
+    auto set_flags = [&](int q, int fl) {
+        __hip_atomic_store(&a.ctrl[q].flags, fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto stencil = [&](const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mk2, bool res) {
+        for (int site = s0 + wv; site < s1; site += kNW)
+            for (int tl = 0; tl < ntile; tl++) {
+                if (!((mk2 >> (16 * tl)) & 0xffffull)) continue;
+                if (res)
+                    coarse_site<R, DOF, true>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
+                else
+                    coarse_site<R, DOF, false>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                               lane);
+            }
+    };
+    auto load_rows = [&](cplx<R> (&r)[RPL], const cplx<R>* vec) {   // vec: system base + row0
+#pragma unroll
+        for (int k = 0; k < RPL; k++) {
+            const int row = lane + 64 * k;
+            r[k] = row < nrow ? vec[row] : czero<R>();
+        }
+    };
+    auto put1 = [&](int q, int v, cplx<R> p) {   // single work-group partial
+        p = wave_sum(p, 64);
+        if (lane == 0) a.part[((size_t)q * VALS + v) * G + wg] = mk<double>((double)p.x, (double)p.y);
+    };
+    // ---- phase A reduction: a_i, c and (hw) b_i, d of system q over the own rows
+    auto reduce_step = [&](int q, int j, bool hw) {
+        cplx<R> ur[RPL], wr[RPL];
+        load_rows(ur, a.u + (size_t)q * n + row0);
+        if (hw) load_rows(wr, a.w + (size_t)q * n + row0);
+        else
+#pragma unroll
+            for (int k = 0; k < RPL; k++) wr[k] = czero<R>();
+        cplx<R> c = czero<R>(), d = czero<R>();
+#pragma unroll
+        for (int k = 0; k < RPL; k++) {
+            c.x += norm2(ur[k]);
+            d = d + cmul(ur[k], wr[k]);
+        }
+        put1(q, 2 * m, c);
+        if (hw) put1(q, 2 * m + 1, d);
+        double* pd = reinterpret_cast<double*>(a.part);
+        for (int i0 = 0; i0 < j; i0 += kCH) {
+            cplx<R> qv[kCH][RPL];
+#pragma unroll
+            for (int ii = 0; ii < kCH; ii++) {
+                const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+#pragma unroll
+                for (int k = 0; k < RPL; k++) {
+                    const int row = lane + 64 * k;
+                    qv[ii][k] = (i0 + ii < j && row < nrow) ? qi[row] : czero<R>();
+                }
+            }
+            R F[NVAL];
+#pragma unroll
+            for (int ii = 0; ii < kCH; ii++) {
+                cplx<R> pa = czero<R>(), pb = czero<R>();
+#pragma unroll
+                for (int k = 0; k < RPL; k++) {
+                    pa = pa + cmul(qv[ii][k], ur[k]);
+                    pb = pb + cmul(qv[ii][k], wr[k]);
+                }
+                F[4 * ii] = pa.x;
+                F[4 * ii + 1] = pa.y;
+                F[4 * ii + 2] = pb.x;
+                F[4 * ii + 3] = pb.y;
+            }
+            const R tsum = butterfly<R, NVAL>(F, lane);
+            const int f = lane >> (6 - LGV), ii = f >> 2, comp = f & 3;
+            if ((lane & ((1 << (6 - LGV)) - 1)) == 0 && i0 + ii < j && (hw || comp < 2)) {
+                const int v = (comp < 2 ? 0 : m) + i0 + ii;
+                pd[(((size_t)q * VALS + v) * G + wg) * 2 + (comp & 1)] = (double)tsum;
+            }
+        }
+    };
+    auto reduce_norm = [&](uint64_t mk2, int v, const cplx<R>* vec) {
+        for (int q = wv; q < a.nsys; q += kNW) {
+            if (!((mk2 >> q) & 1ull)) continue;
+            cplx<R> r[RPL];
+            load_rows(r, vec + (size_t)q * n + row0);
+            cplx<R> c = czero<R>();
+#pragma unroll
+            for (int k = 0; k < RPL; k++) c.x += norm2(r[k]);
+            put1(q, v, c);
+        }
+    };
+    // total of value v of system q over the work-groups (whole work-group; every thread)
+    auto total1 = [&](int q, int v) -> cplx<double> {
+        if (wv == 0) {
+            const cplx<double>* pp = a.part + ((size_t)q * VALS + v) * G;
+            cplx<double> t = czero<double>();
+            for (int k = lane; k < G; k += 64) t = t + pp[k];
+            t = wave_sum(t, 64);
+            if (lane == 0) tot[v] = t;
+        }
+        __syncthreads();
+        const cplx<double> r = tot[v];
+        __syncthreads();
+        return r;
+    };
+    // ---- phase B for the owned system at step j (whole work-group)
+    auto phaseB = [&](int j) {
+        const int q = sys;
+        const bool hw = j < m;
+        const int nv = hw ? 2 * j + 2 : j + 1;
+        {   // totals: 16 lanes per value
+            const int grp = tid >> 4, sub = tid & 15;
+            for (int base = 0; base < nv; base += kCT / 16) {
+                const int idx = base + grp;
+                cplx<double> t = czero<double>();
+                int v = 0;
+                if (idx < nv) {
+                    if (idx < j) v = idx;
+                    else if (hw && idx < 2 * j) v = m + (idx - j);
+                    else v = 2 * m + (idx - (hw ? 2 * j : j));
+                    const cplx<double>* pp = a.part + ((size_t)q * VALS + v) * G;
+                    for (int k = sub; k < G; k += 16) t = t + pp[k];
+                }
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    t.x += __shfl_xor(t.x, o, 64);
+                    t.y += __shfl_xor(t.y, o, 64);
+                }
+                if (idx < nv && sub == 0) tot[v] = t;
+            }
+        }
+        __syncthreads();
+        if (wv == 0) {
+            double sa = 0;
+            cplx<double> ab = czero<double>();
+            for (int i = lane; i < j; i += 64) {
+                sa += norm2(tot[i]);
+                if (hw) ab = ab + cmul(tot[i], tot[m + i]);
+            }
+            sa = wave_sum(sa, 64);
+            ab = wave_sum(ab, 64);
+            if (lane == 0) {
+                const double b2 = tot[2 * m].x - sa;
+                sc[0] = b2 > 0 ? sqrt(b2) : 0.0;
+                sc[1] = ab.x;
+                sc[2] = ab.y;
+            }
+        }
+        __syncthreads();
+        const double beta = sc[0];
+        if (j >= 1)   // column j-1 of the Arnoldi Hessenberg matrix
+            for (int i = tid; i <= j; i += kCT) hcol[i] = i < j ? h1p[i] + tot[i] : mk<double>(beta, 0.0);
+        __syncthreads();
+        if (tid == 0) {
+            int stop = 0, maxit = a.ctrl[q].maxit;
+            if (j == 0) {
+                gls[0] = mk<double>(beta, 0.0);
+            } else {
+                // Givens rotations of column c = j-1 (src/fgmres.cpp:99-115)
+                const int c = j - 1;
+                cplx<double> carry = hcol[0];
+                for (int i = 0; i < c; i++) {
+                    const cplx<double> h1 = hcol[i + 1], cn = csl[i], sn = snl[i];
+                    hcol[i] = cmul(cn, carry) + cmul(sn, h1);
+                    carry = cn * h1 - sn * carry;
+                }
+                const cplx<double> hs = hcol[j];
+                const double den = sqrt(norm2(carry) + norm2(hs));
+                const cplx<double> sn = den > 0 ? cdivr(hs, den) : czero<double>();
+                const cplx<double> cn = den > 0 ? cdivr(carry, den) : mk<double>(1.0, 0.0);
+                snl[c] = sn;
+                csl[c] = cn;
+                hcol[c] = cmul(cn, carry) + cmul(sn, hs);
+                gls[c + 1] = -(sn * gls[c]);
+                gls[c] = cmul(cn, gls[c]);
+                if (sqrt(norm2(gls[c + 1])) < a.tol * a.nrm[2 * q]) {
+                    maxit = j;
+                    stop = 1;
+                }
+            }
+            if (j == m || beta == 0.0) stop = 1;
+            if (stop) {
+                a.ctrl[q].maxit = maxit;
+                set_flags(q, a.ctrl[q].flags & ~F_RUN);
+            }
+            si[0] = stop;
+        }
+        __syncthreads();
+        cplx<double>* Ho = a.Hrot + (size_t)q * (m + 1) * m;
+        if (j >= 1)
+            for (int i = tid; i < j; i += kCT) Ho[(size_t)i * m + (j - 1)] = hcol[i];
+        if (!si[0]) {   // first-pass coefficients of column j and the update coefficients
+            const double ib = 1.0 / beta;
+            cplx<double>* cf = a.coef + (size_t)q * CST;
+            for (int i = tid; i <= j; i += kCT) {
+                const cplx<double> h1 =
+                    i < j ? ib * tot[m + i] : (ib * ib) * (tot[2 * m + 1] - mk<double>(sc[1], sc[2]));
+                h1p[i] = h1;
+                cf[m + i] = h1;
+                if (i < j) cf[i] = tot[i];
+            }
+            if (tid == 0) cf[2 * m + 1] = mk<double>(ib, 0.0);
+        }
+        __syncthreads();
+    };
+    // ---- phase C: q_j and u_{j+1} on the own rows; systems in rounds of kNW (one per wave)
+    auto update = [&](int j, uint64_t mk2) {
+        cplx<R>* sco = scoef + (size_t)wv * 2 * (m + 1);
+        for (int r0 = 0; r0 < a.nsys; r0 += kNW) {
+            const int q = r0 + wv;
+            const bool act = q < a.nsys && ((mk2 >> q) & 1ull);
+            __syncthreads();
+            if (act) {
+                const cplx<double>* cf = a.coef + (size_t)q * CST;
+                for (int i = lane; i <= j; i += 64) {
+                    sco[i] = i < j ? cast<R>(cf[i]) : czero<R>();
+                    sco[m + 1 + i] = cast<R>(cf[m + i]);
+                }
+            }
+            __syncthreads();
+            if (!act) continue;
+            const R ib = (R)a.coef[(size_t)q * CST + 2 * m + 1].x;
+            cplx<R> qj[RPL], un[RPL];
+            load_rows(qj, a.u + (size_t)q * n + row0);
+            load_rows(un, a.w + (size_t)q * n + row0);
+#pragma unroll
+            for (int k = 0; k < RPL; k++) un[k] = ib * un[k];
+            for (int i0 = 0; i0 < j; i0 += kCH) {
+                cplx<R> qv[kCH][RPL];
+#pragma unroll
+                for (int ii = 0; ii < kCH; ii++) {
+                    const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+#pragma unroll
+                    for (int k = 0; k < RPL; k++) {
+                        const int row = lane + 64 * k;
+                        qv[ii][k] = (i0 + ii < j && row < nrow) ? qi[row] : czero<R>();
+                    }
+                }
+#pragma unroll
+                for (int ii = 0; ii < kCH; ii++) {
+                    const cplx<R> ai = sco[min(i0 + ii, m)], ci = sco[m + 1 + min(i0 + ii, m)];
+                    if (i0 + ii < j)
+#pragma unroll
+                        for (int k = 0; k < RPL; k++) {
+                            qj[k] = qj[k] - ai * qv[ii][k];
+                            un[k] = un[k] - ci * qv[ii][k];
+                        }
+                }
+            }
+            const cplx<R> cj = sco[m + 1 + j];
+            cplx<R>* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0;
+            cplx<R>* uo = a.u + (size_t)q * n + row0;
+#pragma unroll
+            for (int k = 0; k < RPL; k++) {
+                const int row = lane + 64 * k;
+                if (row < nrow) {
+                    qj[k] = ib * qj[k];
+                    un[k] = un[k] - cj * qj[k];
+                    qo[row] = qj[k];
+                    uo[row] = un[k];
+                }
+            }
+        }
+    };
+    // ---- triangular solve of the owned system (src/fgmres.cpp:118-127), blocked:
+    // kBB x kBB diagonal blocks in LDS on one wave, the rest as work-group updates
+    auto backsub = [&]() {
+        const int q = sys;
+        const int nI = a.ctrl[q].maxit;
+        const cplx<double>* Ho = a.Hrot + (size_t)q * (m + 1) * m;
+        for (int b1 = nI; b1 > 0; b1 -= kBB) {
+            const int b0 = max(0, b1 - kBB), nb = b1 - b0;
+            for (int e = tid; e < nb * nb; e += kCT) {
+                const int r = e / nb, c = e - r * nb;
+                blk[r * kBB + c] = r <= c ? Ho[(size_t)(b0 + r) * m + (b0 + c)] : czero<double>();
+            }
+            __syncthreads();
+            if (wv == 0) {
+                cplx<double> rh = lane < nb ? gls[b0 + lane] : czero<double>();
+                for (int i = nb - 1; i >= 0; i--) {
+                    cplx<double> ri;
+                    ri.x = __shfl(rh.x, i, 64);
+                    ri.y = __shfl(rh.y, i, 64);
+                    const cplx<double> e = cdiv(ri, blk[i * kBB + i]);
+                    if (lane == i) rh = e;
+                    if (lane < i) rh = rh - blk[lane * kBB + i] * e;
+                }
+                if (lane < nb) gls[b0 + lane] = rh;
+            }
+            __syncthreads();
+            for (int k = tid; k < b0; k += kCT) {
+                cplx<double> acc = czero<double>();
+                const cplx<double>* row = Ho + (size_t)k * m + b0;
+                for (int c = 0; c < nb; c++) acc = acc + row[c] * gls[b0 + c];
+                gls[k] = gls[k] - acc;
+            }
+            __syncthreads();
+        }
+        cplx<double>* et = a.etals + (size_t)q * m;
+        for (int i = tid; i < nI; i += kCT) et[i] = gls[i];
+    };
+    // ---- x += sum_{i < maxit} eta_i q_i on the own rows (rounds of kNW systems)
+    auto xupdate = [&](uint64_t mk2) {
+        cplx<R>* sco = scoef + (size_t)wv * 2 * (m + 1);
+        for (int r0 = 0; r0 < a.nsys; r0 += kNW) {
+            const int q = r0 + wv;
+            const bool act = q < a.nsys && ((mk2 >> q) & 1ull);
+            const int nI = act ? a.ctrl[q].maxit : 0;
+            __syncthreads();
+            if (act)
+                for (int i = lane; i < nI; i += 64) sco[i] = cast<R>(a.etals[(size_t)q * m + i]);
+            __syncthreads();
+            if (!act) continue;
+            cplx<R> xr[RPL];
+            load_rows(xr, a.x + (size_t)q * n + row0);
+            for (int i0 = 0; i0 < nI; i0 += kCH) {
+                cplx<R> qv[kCH][RPL];
+#pragma unroll
+                for (int ii = 0; ii < kCH; ii++) {
+                    const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+#pragma unroll
+                    for (int k = 0; k < RPL; k++) {
+                        const int row = lane + 64 * k;
+                        qv[ii][k] = (i0 + ii < nI && row < nrow) ? qi[row] : czero<R>();
+                    }
+                }
+#pragma unroll
+                for (int ii = 0; ii < kCH; ii++) {
+                    const cplx<R> e = sco[min(i0 + ii, m)];
+                    if (i0 + ii < nI)
+#pragma unroll
+                        for (int k = 0; k < RPL; k++) xr[k] = xr[k] + e * qv[ii][k];
+                }
+            }
+            cplx<R>* xo = a.x + (size_t)q * n + row0;
+#pragma unroll
+            for (int k = 0; k < RPL; k++) {
+                const int row = lane + 64 * k;
+                if (row < nrow) xo[row] = xr[k];
+            }
+        }
+    };
+    const double links_bytes = (double)V * 5 * DOF * DOF * sR;
+    auto count = [&](double bytes, double steps) {
+        if (wg == 0 && tid == 0) {
+            a.stats[0] += bytes;
+            a.stats[1] += steps;
+        }
+    };
+    uint64_t tclk = wall_clock64();
+    auto clk = [&](int slot) {   // wall-clock time of work-group 0 per phase
+        if (wg == 0 && tid == 0) {
+            const uint64_t t = wall_clock64();
+            a.stats[2 + slot] += (double)(t - tclk);
+            tclk = t;
+        }
+    };
+
+    // ---- x0 = eta, r = eta - A x0, ||eta||, ||r||
+    if (wg == 0 && tid == 0)
+        for (int k = 0; k < 10; k++) a.stats[k] = 0;
+    for (int q = wv; q < a.nsys; q += kNW) {
+        if (!((a.mask >> q) & 1ull)) continue;
+        for (int row = lane; row < nrow; row += 64) a.x[(size_t)q * n + row0 + row] = a.eta[(size_t)q * n + row0 + row];
+    }
+    grid.sync();
+    stencil(a.x, a.u, a.eta, a.mask, true);
+    __syncthreads();
+    reduce_norm(a.mask, 2 * m, a.u);
+    reduce_norm(a.mask, 2 * m + 1, a.eta);
+    const int nmask = __builtin_popcountll(a.mask);
+    count(links_bytes + nmask * n * sR * 6, 0);
+    grid.sync();
+    if (owner) {
+        const double err = sqrt(total1(sys, 2 * m).x), nphi = sqrt(total1(sys, 2 * m + 1).x);
+        if (tid == 0) {
+            a.nrm[2 * sys] = nphi;
+            a.nrm[2 * sys + 1] = err;
+            a.ctrl[sys].maxit = m;
+            a.ctrl[sys].iters = err == 0.0 ? 0 : a.restarts * m;
+            set_flags(sys, err == 0.0 ? 0 : (F_LIVE | F_RUN));
+        }
+    }
+    grid.sync();
+
+    for (int k = 0; k < a.restarts; k++) {
+        const uint64_t live = sysmask(F_LIVE);
+        if (!live) break;
+        const int nl = __builtin_popcountll(live);
+        for (int j = 0; j <= m; j++) {
+            const uint64_t run = sysmask(F_RUN);
+            if (!run) break;
+            const int nr = __builtin_popcountll(run);
+            const bool hw = j < m;
+            if (hw) stencil(a.u, a.w, nullptr, run, false);
+            __syncthreads();
+            for (int q = wv; q < a.nsys; q += kNW)
+                if ((run >> q) & 1ull) reduce_step(q, j, hw);
+            clk(0);
+            grid.sync();
+            clk(1);
+            if (owner && ((run >> sys) & 1ull)) phaseB(j);
+            clk(2);
+            grid.sync();
+            clk(3);
+            const uint64_t cont = sysmask(F_RUN) & run;
+            update(j, cont);
+            clk(4);
+            const int nc = __builtin_popcountll(cont);
+            count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * sR * (j + (hw ? 2 : 1)) +
+                      nc * n * sR * (j + 4),
+                  1);
+            grid.sync();
+            clk(5);
+        }
+        if (owner && ((live >> sys) & 1ull)) backsub();
+        grid.sync();
+        xupdate(live);
+        grid.sync();
+        stencil(a.x, a.u, a.eta, live, true);
+        __syncthreads();
+        reduce_norm(live, 2 * m, a.u);
+        count(links_bytes + nl * n * sR * 6, 0);
+        grid.sync();
+        if (owner && ((live >> sys) & 1ull)) {
+            const double err = sqrt(total1(sys, 2 * m).x);
+            if (tid == 0) {
+                a.nrm[2 * sys + 1] = err;
+                int fl = F_LIVE | F_RUN;
+                if (err < a.tol * a.nrm[2 * sys]) {
+                    a.ctrl[sys].iters = k * m + a.ctrl[sys].maxit;
+                    fl = 0;
+                }
+                set_flags(sys, fl);
+            }
+        }
+        grid.sync();
+        clk(6);
+    }
+}
+
+template <typename R, int DOF>
+void launch_coarsest(const CArgs<R>& a, int G, size_t sh, hipStream_t s) {
+    void* args[] = {const_cast<CArgs<R>*>(&a)};
+    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, DOF>), dim3(G), dim3(kCT), args,
+                                      (unsigned)sh, s));
+}
+
+template <typename R, int DOF>
+int max_coresident(size_t sh) {
+    int dev = 0, ncu = 0, per = 0;
+    DD_HIP(hipGetDevice(&dev));
+    DD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    DD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_coarsest<R, DOF>),
+                                                        kCT, sh));
+    return per * ncu;
+}
+
+template <typename R>
+size_t coarsest_lds(int m) {
+    const size_t state = sizeof(cplx<double>) * (size_t)(7 * m + 5) + 64;
+    const size_t scratch = std::max(sizeof(cplx<R>) * kNW * 2 * (size_t)(m + 1), sizeof(cplx<double>) * kBB * kBB);
+    return state + scratch;
+}
+
+}  // namespace
+
+template <typename R>
+CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap)
+    : s_(s), Nx_(Nx), Nt_(Nt), dof_(dof), m_(m), restarts_(restarts), cap_(cap), tol_(tol) {
+    if (m < 1 || m > kMaxM || restarts < 1 || cap < 1 || cap > 64) return;
+    const int V = Nx * Nt;
+    const size_t sh = coarsest_lds<R>(m);
+    if (sh > 160 * 1024) return;
+    int maxg = 0;
+    switch (dof) {
+#define DD_CASE(D)                                                                                         \
+    case D:                                                                                                \
+        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarsest<R, D>),                       \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
+        maxg = max_coresident<R, D>(sh);                                                                   \
+        break;
+        DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
+#undef DD_CASE
+        default:
+            return;
+    }
+    // slices of spw sites: at least max(cap, 128) work-groups (one per system for the
+    // scalar state, enough to stream the basis), at most kSpwMax sites each
+    spw_ = std::max(1, std::min(kSpwMax, V / std::max(cap, 128)));
+    G_ = (V + spw_ - 1) / spw_;
+    if (G_ > maxg || G_ < cap) return;
+    const size_t n = (size_t)V * dof, vb = sizeof(cplx<R>) * n * cap;
+    Q_.alloc(vb * (m + 1));
+    u_.alloc(vb);
+    w_.alloc(vb);
+    part_.alloc(sizeof(cplx<double>) * (size_t)cap * (2 * m + 2) * G_);
+    Hrot_.alloc(sizeof(cplx<double>) * (size_t)cap * (m + 1) * m);
+    etals_.alloc(sizeof(cplx<double>) * (size_t)cap * m);
+    coef_.alloc(sizeof(cplx<double>) * (size_t)cap * (2 * m + 2));
+    nrm_.alloc(sizeof(double) * (2 * cap + 10));
+    ctrl_.alloc(sizeof(Ctrl) * cap);
+    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 10);
+    ok_ = true;
+}
+
+template <typename R>
+void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask,
+                             int* iters) {
+    if (!ok_) throw BadShape("coarsest GMRES: persistent kernel not available for this level");
+    const int nsys = mask ? 64 - __builtin_clzll(mask) : 0;
+    if (nsys == 0) return;
+    if (nsys > cap_ || nsys > G_) throw BadShape("coarsest GMRES: batch larger than capacity");
+    CArgs<R> a;
+    a.L = L;
+    a.eta = eta;
+    a.x = psi;
+    a.Q = Q_.as<cplx<R>>();
+    a.u = u_.as<cplx<R>>();
+    a.w = w_.as<cplx<R>>();
+    a.part = part_.as<cplx<double>>();
+    a.Hrot = Hrot_.as<cplx<double>>();
+    a.etals = etals_.as<cplx<double>>();
+    a.coef = coef_.as<cplx<double>>();
+    a.nrm = nrm_.as<double>();
+    a.ctrl = ctrl_.as<Ctrl>();
+    a.stats = nrm_.as<double>() + 2 * cap_;
+    a.mask = mask;
+    a.Nx = Nx_;
+    a.Nt = Nt_;
+    a.nsys = nsys;
+    a.m = m_;
+    a.restarts = restarts_;
+    a.spw = spw_;
+    a.m0 = (R)m0;
+    a.tol = tol_;
+    const size_t sh = coarsest_lds<R>(m_);
+    timer_begin(s_, K_COARSEST);
+    switch (dof_) {
+#define DD_CASE(D) \
+    case D: launch_coarsest<R, D>(a, G_, sh, s_); break;
+        DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
+#undef DD_CASE
+        default:
+            throw BadShape("coarsest GMRES: unsupported dof");
+    }
+    timer_end(s_, K_COARSEST, 0.0);
+    Ctrl* ch = ctrlh_.as<Ctrl>();
+    double* st = reinterpret_cast<double*>(ch + cap_);
+    DD_HIP(hipMemcpyAsync(ch, a.ctrl, sizeof(Ctrl) * nsys, hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipMemcpyAsync(st, a.stats, sizeof(double) * 10, hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipStreamSynchronize(s_));
+    for (int k = 0; k < 7; k++) phase_clk[k] += st[2 + k];
+    timer_add_bytes(K_COARSEST, st[0]);
+    steps += (long long)st[1];
+    for (int q = 0; q < nsys; q++)
+        if ((mask >> q) & 1ull) iters[q] = ch[q].iters;
+}
+
+template class CoarsestGmres<float>;
+template class CoarsestGmres<double>;
+
+}  // namespace dd

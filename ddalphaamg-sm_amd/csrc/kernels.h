@@ -17,6 +17,8 @@ void timer_reset();
 void timer_begin(hipStream_t s, int id);
 // `bytes`: algorithmic HBM bytes of the launch (each operand counted once)
 void timer_end(hipStream_t s, int id, double bytes);
+// adds algorithmic bytes to kernel `id` after the fact (kernels that count their own work)
+void timer_add_bytes(int id, double bytes);
 // total elapsed ms, number of timed launches and algorithmic bytes of kernel `id`
 // since the last reset (synchronises the events)
 double timer_read(int id, long long* count, double* bytes);

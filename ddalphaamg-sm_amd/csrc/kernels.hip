@@ -62,6 +62,10 @@ void timer_end(hipStream_t s, int id, double bytes) {
     t.used += 2;
     t.bytes += bytes;
 }
+void timer_add_bytes(int id, double bytes) {
+    TimerSlot& t = g_timers[id];
+    if (t.on) t.bytes += bytes;
+}
 double timer_read(int id, long long* count, double* bytes) {
     TimerSlot& t = g_timers[id];
     double tot = 0;
@@ -188,48 +192,9 @@ template <typename R, int DOF, bool RES>
 __global__ __launch_bounds__(256) void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
                                                 int Nt, R m0, uint64_t mask, size_t ld, int nrhs) {
-    using acc_t = typename Mfma<R>::acc_t;
-    constexpr int MT = DOF / 8, KS = DOF / 2;
-    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
     const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (n >= Nx * Nt) return;   // uniform per wave
-    const int j = blockIdx.y * 16 + i;
-    const bool on = j < nrhs && rhs_on(mask, j);
-    const int x = n / Nt, t = n - x * Nt;
-    const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
-                       (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
-    const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
-    const LinkLane ll = LinkLane::make<R, DOF>(lane);
-    const R* Ln = reinterpret_cast<const R*>(L + (size_t)n * 5 * DOF * DOF);
-    const R* vin = reinterpret_cast<const R*>(in + (size_t)(on ? j : 0) * ld);
-    acc_t acc[MT];
-#pragma unroll
-    for (int tt = 0; tt < MT; tt++) acc[tt] = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int h = 0; h < 5; h++) {
-        // B operand: part (g & 1) of column 2s + (g >> 1) -> real index 4s + g of the site
-        const R* vb = vin + (size_t)nb[h] * DOF * 2 + g;
-        R bv[KS];
-#pragma unroll
-        for (int s2 = 0; s2 < KS; s2++) bv[s2] = on ? sg[h] * vb[4 * s2] : R(0);
-        mfma_hop<R, DOF>(acc, Ln + (size_t)h * DOF * DOF * 2, ll, bv);
-    }
-    if (!on) return;
-    const R mass = m0 + R(2);
-    const size_t base = (size_t)j * ld + (size_t)n * DOF;
-#pragma unroll
-    for (int tt = 0; tt < MT; tt++) {
-        const int r0 = 8 * tt + 2 * g;
-        const cplx<R> v0 = in[base + r0], v1 = in[base + r0 + 1];
-        cplx<R> o0 = mk<R>(mass * v0.x - acc[tt][0], mass * v0.y - acc[tt][1]);
-        cplx<R> o1 = mk<R>(mass * v1.x - acc[tt][2], mass * v1.y - acc[tt][3]);
-        if (RES) {
-            o0 = b[base + r0] - o0;
-            o1 = b[base + r0 + 1] - o1;
-        }
-        out[base + r0] = o0;
-        out[base + r0 + 1] = o1;
-    }
+    coarse_site<R, DOF, RES>(L, in, out, b, Nx, Nt, m0, mask, ld, nrhs, n, blockIdx.y, threadIdx.x & 63);
 }
 
 #define DD_DOF_LIST(X) X(8) X(16) X(24) X(32)

@@ -9,55 +9,11 @@
 #include <vector>
 
 #include "../../include/ddamg.h"
+#include "coarsest.h"
+#include "devmem.h"
 #include "kernels.h"
 
 namespace dd {
-
-// ---- memory ------------------------------------------------------------------------
-class DevMem {
-   public:
-    DevMem() = default;
-    explicit DevMem(size_t bytes) { alloc(bytes); }
-    ~DevMem() { release(); }
-    DevMem(const DevMem&) = delete;
-    DevMem& operator=(const DevMem&) = delete;
-    DevMem(DevMem&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
-    DevMem& operator=(DevMem&& o) noexcept {
-        if (this != &o) {
-            release();
-            p_ = o.p_;
-            bytes_ = o.bytes_;
-            o.p_ = nullptr;
-            o.bytes_ = 0;
-        }
-        return *this;
-    }
-    void alloc(size_t bytes);
-    void release();
-    template <typename T>
-    T* as() const { return reinterpret_cast<T*>(p_); }
-    size_t bytes() const { return bytes_; }
-
-   private:
-    void* p_ = nullptr;
-    size_t bytes_ = 0;
-};
-
-class PinMem {
-   public:
-    PinMem() = default;
-    explicit PinMem(size_t bytes) { alloc(bytes); }
-    ~PinMem() { release(); }
-    PinMem(const PinMem&) = delete;
-    PinMem& operator=(const PinMem&) = delete;
-    void alloc(size_t bytes);
-    void release();
-    template <typename T>
-    T* as() const { return reinterpret_cast<T*>(p_); }
-
-   private:
-    void* p_ = nullptr;
-};
 
 inline uint64_t full_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
 inline int mask_count(uint64_t m) { return m ? 64 - __builtin_clzll(m) : 0; }   // highest set bit + 1
@@ -148,6 +104,13 @@ class Multigrid {
         *ms = (l >= 0 && l < (int)sapc_ms_.size()) ? sapc_ms_[l] : 0.0;
     }
     std::vector<long long> coarse_iters;   // coarsest GMRES iterations per call (stats)
+    // Arnoldi steps of the coarsest solver, its per-phase clocks (persistent kernel only,
+    // else nullptr) and the number of coarsest solves
+    void coarsest_stats(const long long** steps, const double** clk, long long* calls) const {
+        *steps = coarse_dev_ ? &coarse_dev_->steps : (coarse_ ? &coarse_->steps : nullptr);
+        *clk = coarse_dev_ ? coarse_dev_->phase_clk : nullptr;
+        *calls = (long long)coarse_iters.size();
+    }
 
    private:
     hipStream_t s_;
@@ -163,6 +126,7 @@ class Multigrid {
     std::vector<DevMem> part_;
     std::vector<std::unique_ptr<Krylov<R>>> kry_;   // K-cycle FGMRES of level l (index l)
     std::unique_ptr<Krylov<R>> coarse_;
+    std::unique_ptr<CoarsestGmres<R>> coarse_dev_;   // persistent kernel (ortho_coarse = 2)
     std::vector<int> iters_scratch_;
     std::vector<int> sapc_var_;
     std::vector<double> sapc_ms_;

@@ -325,8 +325,15 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
         }
         if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
     }
-    coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap, false,
-                                p.ortho_coarse));
+    if (p.ortho_coarse == 2) {
+        const LevelGeom& Gc = g_[L - 1];
+        coarse_dev_.reset(new CoarsestGmres<R>(s, Gc.Nx, Gc.Nt, Gc.dof, p.coarse_m, p.coarse_restarts, p.coarse_tol,
+                                               cap));
+        if (!coarse_dev_->supported()) coarse_dev_.reset();
+    }
+    if (!coarse_dev_)
+        coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap, false,
+                                    p.ortho_coarse == 0 ? 0 : 1));
     iters_scratch_.resize(64);
 }
 
@@ -386,9 +393,13 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
 template <typename R>
 void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
     const int L = levels();
-    auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
-    typename Krylov<R>::Func fA = A;
-    coarse_->solve(eta, eta, psi, mask, fA, nullptr, iters_scratch_.data(), nullptr);
+    if (coarse_dev_) {
+        coarse_dev_->solve(links(L - 1), p_.m0, eta, psi, mask, iters_scratch_.data());
+    } else {
+        auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
+        typename Krylov<R>::Func fA = A;
+        coarse_->solve(eta, eta, psi, mask, fA, nullptr, iters_scratch_.data(), nullptr);
+    }
     int mx = 0;
     for (int q = 0; q < mask_count(mask); q++)
         if ((mask >> q) & 1) mx = std::max(mx, iters_scratch_[q]);

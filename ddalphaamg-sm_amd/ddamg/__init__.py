@@ -252,6 +252,14 @@ class Context:
         _chk(lib.dd_get_tuning(self._h, level, C.byref(v), C.byref(t)))
         return v.value, t.value
 
+    def coarsest_stats(self) -> dict:
+        """Coarsest-solver counters since setup (solves, Arnoldi steps, phase ms)."""
+        out = (C.c_double * 10)()
+        _chk(lib.dd_coarsest_stats(self._h, out))
+        keys = ['solves', 'steps', 'ms_stencil_reduce', 'ms_barrier1', 'ms_scalar', 'ms_barrier2', 'ms_update',
+                'ms_barrier3', 'ms_cycle_end', 'persistent']
+        return dict(zip(keys, list(out)))
+
     def level_shape(self, level: int):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _chk(lib.dd_level_shape(self._h, level, C.byref(a), C.byref(b), C.byref(c)))

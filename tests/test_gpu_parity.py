@@ -225,3 +225,42 @@ def test_coarse_sap_matrix_cores(blk):
     e32, e2 = rel(out[32], ref), rel(out[2], ref)
     print(f'blocks {blk}: FP32 error vs FP64 oracle: matrix cores {e32:.2e}, vector kernel {e2:.2e}')
     assert e32 < max(2 * e2, 1e-5) and e32 < 1e-3
+
+
+@pytest.mark.parametrize('precision', [0, 1])
+def test_coarsest_persistent(precision):
+    """Coarsest-level GMRES in one persistent kernel (CGS with delayed
+    re-orthogonalisation, the default ortho_coarse = 2): (a) run to a tight tolerance it
+    equals the dense solve of the device's coarsest operator (FP64 1e-9, FP32 1e-4
+    relative); (b) a 17-system batch (two 16-system tiles) gives every system the
+    result it gets alone (systems share no arithmetic); (c) at the reference tolerance
+    0.1 every system meets ||eta - D x|| < 0.1 ||eta||, as the launch-per-step engine's
+    solution does."""
+    rng = np.random.default_rng(5)
+    N = 64
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.5)
+    base = dict(agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[8, 8], m0=-0.2, precision=precision,
+                max_rhs=17, rng_ranks=(2, 2))
+    tight = 1e-12 if precision == 0 else 1e-6
+    with ddamg.Context(N, N) as ctx:
+        ctx.set_gauge(U)
+        ctx.setup(ddamg.params(3, coarse_tol=tight, **base))
+        shape = ctx.level_shape(2)
+        v = rng.standard_normal((17,) + shape) + 1j * rng.standard_normal((17,) + shape)
+        x = ctx.cycle(2, v, np.zeros_like(v), kcycle=True, precision=precision)
+        A, B, Cm = ctx.links(2, precision=precision)
+        op = orc.Stencil(A, B, Cm, -0.2)
+        nn = int(np.prod(shape))
+        D = np.stack([op.apply(e.reshape(shape)).ravel() for e in np.eye(nn)], axis=1)
+        for r in (0, 7, 16):
+            xd = np.linalg.solve(D, v[r].ravel()).reshape(shape)
+            assert rel(x[r], xd) < (1e-9 if precision == 0 else 1e-4)
+            assert rel(ctx.cycle(2, v[r], np.zeros_like(v[r]), kcycle=True, precision=precision), x[r]) < 1e-12
+    for ortho in (2, 1):
+        with ddamg.Context(N, N) as ctx:
+            ctx.set_gauge(U)
+            ctx.setup(ddamg.params(3, ortho_coarse=ortho, **base))
+            x = ctx.cycle(2, v, np.zeros_like(v), kcycle=True, precision=precision)
+            for r in range(17):
+                res = v[r] - op.apply(x[r])
+                assert np.linalg.norm(res) < 0.1 * np.linalg.norm(v[r]) * (1 + 1e-5)
