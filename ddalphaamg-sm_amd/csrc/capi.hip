@@ -128,7 +128,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->rng_ranks_t = 2;
     p->ortho_outer = 0;
     p->ortho_inner = 0;
-    p->ortho_coarse = 2;
+    p->ortho_coarse = 3;
     p->max_rhs = 1;
     p->autotune = 1;
     p->sap_variant = -1;
@@ -330,7 +330,7 @@ int dd_coarsest_stats(dd_ctx* c, double* out) {
     if (check(c) || !out) return DD_ERR_ARG;
     return guard([&] {
         if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
-        for (int k = 0; k < 10; k++) out[k] = 0;
+        for (int k = 0; k < 12; k++) out[k] = 0;
         int dev = 0, khz = 0;
         DD_HIP(hipGetDevice(&dev));
         DD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -341,8 +341,8 @@ int dd_coarsest_stats(dd_ctx* c, double* out) {
             m->coarsest_stats(&st, &clk, &calls);
             out[0] = (double)calls;
             out[1] = st ? (double)*st : 0.0;
-            for (int k = 0; k < 7 && clk; k++) out[2 + k] = khz > 0 ? clk[k] / khz : 0.0;
-            out[9] = clk ? 1.0 : 0.0;
+            for (int k = 0; k < 9 && clk; k++) out[2 + k] = khz > 0 ? clk[k] / khz : 0.0;
+            out[11] = clk ? 1.0 : 0.0;
         };
         if (c->s->mg64) fill(c->s->mg64.get());
         else fill(c->s->mg32.get());

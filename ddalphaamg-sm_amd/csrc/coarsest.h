@@ -24,13 +24,20 @@ class CoarsestGmres {
     // psi = D_c^-1 eta for the systems in mask (x0 = eta); iters[r] per system
     void solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int* iters);
     long long steps = 0;   // Arnoldi steps taken (cumulative)
-    // wall-clock ticks of work-group 0 per phase (cumulative): stencil + reduction,
-    // barrier, scalar phase, barrier, update, barrier, cycle end
-    double phase_clk[7] = {0, 0, 0, 0, 0, 0, 0};
+    // restart length (at most the constructed one), cycles and tolerance of later solves
+    void set_limits(int m, int restarts, double tol) {
+        m_ = std::max(1, std::min(m, m_max_));
+        restarts_ = restarts;
+        tol_ = tol;
+    }
+    // wall-clock ticks of work-group 0 per phase (cumulative): reduction, barrier,
+    // scalar phase (after the totals), barrier, update, barrier, cycle end, stencil,
+    // totals of the scalar phase
+    double phase_clk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
    private:
     hipStream_t s_;
-    int Nx_, Nt_, dof_, m_, restarts_, cap_, G_ = 0, spw_ = 0;
+    int Nx_, Nt_, dof_, m_, restarts_, cap_, G_ = 0, spw_ = 0, m_max_ = 0;
     double tol_;
     bool ok_ = false;
     DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_;

@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "coarsest.h"
 #include "mfma.h"
@@ -40,7 +42,7 @@ namespace {
 constexpr int kCT = 512;     // threads per work-group
 constexpr int kNW = kCT / 64;
 constexpr int kSpwMax = 8;   // sites per work-group at most
-constexpr int kMaxM = 448;   // restart length at most
+constexpr int kMaxM = 432;   // restart length at most (LDS state below 64 KB)
 constexpr int kBB = 32;      // block of the triangular solve
 
 enum : int { F_LIVE = 1, F_RUN = 2 };
@@ -59,8 +61,8 @@ struct CArgs {
     cplx<R>* w;             // [nsys][n]
     cplx<double>* part;     // [nsys][2m+2][G] per-work-group partial sums
     cplx<double>* Hrot;     // [nsys][m+1][m] rotated (triangular) Hessenberg columns
-    cplx<double>* etals;    // [nsys][m] least-squares solution
-    cplx<double>* coef;     // [nsys][2m+2]: a_i (i<j) | h1_{i,j} (i<=j) | 1/beta
+    cplx<R>* etals;         // [nsys][m] least-squares solution (working precision)
+    cplx<R>* coef;          // [nsys][2m+2]: a_i (i<j) | h1_{i,j} (i<=j) | 1/beta
     double* nrm;            // [nsys][2]: ||eta||, ||r||
     Ctrl* ctrl;             // [nsys]
     double* stats;          // [2 + 8]: algorithmic bytes, Arnoldi steps, phase clocks of
@@ -119,9 +121,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     double* sc = reinterpret_cast<double*>(hcol + (m + 1));            // [4]
     int* si = reinterpret_cast<int*>(sc + 4);                          // [4]
     uint64_t* smask = reinterpret_cast<uint64_t*>(si + 4);             // [1]
-    unsigned char* scr = reinterpret_cast<unsigned char*>(smask + 2);  // scratch (phase C, x update, solve)
-    cplx<R>* scoef = reinterpret_cast<cplx<R>*>(scr);                  // [kNW][2(m+1)]
-    cplx<double>* blk = reinterpret_cast<cplx<double>*>(scr);          // [kBB][kBB]
+    cplx<double>* blk = reinterpret_cast<cplx<double>*>(smask + 2);    // [kBB][kBB] triangular solve
 
     const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int V = a.Nx * a.Nt;
@@ -134,22 +134,39 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     const int sys = wg;   // the system whose scalars this work-group owns (G >= nsys)
     const bool owner = sys < a.nsys && ((a.mask >> sys) & 1ull);
 
-    auto sysmask = [&](int flag) -> uint64_t {   // uniform: read after a grid barrier
+    const double links_bytes = (double)V * 5 * DOF * DOF * sR;
+    auto count = [&](double bytes, double steps) {
+        if (wg == 0 && tid == 0) {
+            a.stats[0] += bytes;
+            a.stats[1] += steps;
+        }
+    };
+    uint64_t tclk = wall_clock64();
+    auto clk = [&](int slot) {   // wall-clock time of work-group 0 per phase
+        if (wg == 0 && tid == 0) {
+            const uint64_t t = wall_clock64();
+            a.stats[2 + slot] += (double)(t - tclk);
+            tclk = t;
+        }
+    };
+
+    // systems whose control word has `flag`, read by the 64 lanes of wave 0 at once and
+    // broadcast through LDS.  Control words are written only between grid barriers and
+    // read after them (the barrier orders and publishes them), so plain loads and stores
+    // suffice and the result is the same in every work-group.
+    auto sysmask = [&](int flag) -> uint64_t {
         __syncthreads();
-        if (tid == 0) {
-            uint64_t mk2 = 0;
-            for (int q = 0; q < a.nsys; q++)
-                if (((a.mask >> q) & 1ull) &&
-                    (__hip_atomic_load(&a.ctrl[q].flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & flag))
-                    mk2 |= 1ull << q;
-            *smask = mk2;
+        if (wv == 0) {
+            const int q = lane;
+            bool on = false;
+            if (q < a.nsys && ((a.mask >> q) & 1ull)) on = (a.ctrl[q].flags & flag) != 0;
+            const uint64_t mk2 = __ballot(on);
+            if (lane == 0) *smask = mk2;
         }
         __syncthreads();
         return *smask;
     };
-    auto set_flags = [&](int q, int fl) {
-        __hip_atomic_store(&a.ctrl[q].flags, fl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
+    auto set_flags = [&](int q, int fl) { a.ctrl[q].flags = fl; };
     auto stencil = [&](const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mk2, bool res) {
         for (int site = s0 + wv; site < s1; site += kNW)
             for (int tl = 0; tl < ntile; tl++) {
@@ -190,15 +207,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         if (hw) put1(q, 2 * m + 1, d);
         double* pd = reinterpret_cast<double*>(a.part);
         for (int i0 = 0; i0 < j; i0 += kCH) {
+            // unconditional loads at clamped indices (no branches between the loads), the
+            // terms outside the basis or the own rows masked afterwards
             cplx<R> qv[kCH][RPL];
 #pragma unroll
             for (int ii = 0; ii < kCH; ii++) {
-                const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+                const cplx<R>* qi = a.Q + (size_t)min(i0 + ii, j - 1) * vs + (size_t)q * n + row0;
 #pragma unroll
-                for (int k = 0; k < RPL; k++) {
-                    const int row = lane + 64 * k;
-                    qv[ii][k] = (i0 + ii < j && row < nrow) ? qi[row] : czero<R>();
-                }
+                for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
             }
             R F[NVAL];
 #pragma unroll
@@ -206,8 +222,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
                 cplx<R> pa = czero<R>(), pb = czero<R>();
 #pragma unroll
                 for (int k = 0; k < RPL; k++) {
-                    pa = pa + cmul(qv[ii][k], ur[k]);
-                    pb = pb + cmul(qv[ii][k], wr[k]);
+                    const bool ok = i0 + ii < j && lane + 64 * k < nrow;
+                    const cplx<R> qq = ok ? qv[ii][k] : czero<R>();
+                    pa = pa + cmul(qq, ur[k]);
+                    pb = pb + cmul(qq, wr[k]);
                 }
                 F[4 * ii] = pa.x;
                 F[4 * ii + 1] = pa.y;
@@ -274,6 +292,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             }
         }
         __syncthreads();
+        clk(8);
         if (wv == 0) {
             double sa = 0;
             cplx<double> ab = czero<double>();
@@ -335,63 +354,57 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             for (int i = tid; i < j; i += kCT) Ho[(size_t)i * m + (j - 1)] = hcol[i];
         if (!si[0]) {   // first-pass coefficients of column j and the update coefficients
             const double ib = 1.0 / beta;
-            cplx<double>* cf = a.coef + (size_t)q * CST;
+            cplx<R>* cf = a.coef + (size_t)q * CST;
             for (int i = tid; i <= j; i += kCT) {
                 const cplx<double> h1 =
                     i < j ? ib * tot[m + i] : (ib * ib) * (tot[2 * m + 1] - mk<double>(sc[1], sc[2]));
                 h1p[i] = h1;
-                cf[m + i] = h1;
-                if (i < j) cf[i] = tot[i];
+                cf[m + i] = cast<R>(h1);
+                if (i < j) cf[i] = cast<R>(tot[i]);
             }
-            if (tid == 0) cf[2 * m + 1] = mk<double>(ib, 0.0);
+            if (tid == 0) cf[2 * m + 1] = mk<R>((R)ib, R(0));
         }
         __syncthreads();
     };
-    // ---- phase C: q_j and u_{j+1} on the own rows; systems in rounds of kNW (one per wave)
+    // ---- phase C: q_j and u_{j+1} on the own rows (wave per system); the update
+    // coefficients are read per chunk, next to the basis vectors they multiply
     auto update = [&](int j, uint64_t mk2) {
-        cplx<R>* sco = scoef + (size_t)wv * 2 * (m + 1);
-        for (int r0 = 0; r0 < a.nsys; r0 += kNW) {
-            const int q = r0 + wv;
-            const bool act = q < a.nsys && ((mk2 >> q) & 1ull);
-            __syncthreads();
-            if (act) {
-                const cplx<double>* cf = a.coef + (size_t)q * CST;
-                for (int i = lane; i <= j; i += 64) {
-                    sco[i] = i < j ? cast<R>(cf[i]) : czero<R>();
-                    sco[m + 1 + i] = cast<R>(cf[m + i]);
-                }
-            }
-            __syncthreads();
-            if (!act) continue;
-            const R ib = (R)a.coef[(size_t)q * CST + 2 * m + 1].x;
+        for (int q = wv; q < a.nsys; q += kNW) {
+            if (!((mk2 >> q) & 1ull)) continue;
+            const cplx<R>* cf = a.coef + (size_t)q * CST;
+            const R ib = cf[2 * m + 1].x;
             cplx<R> qj[RPL], un[RPL];
             load_rows(qj, a.u + (size_t)q * n + row0);
             load_rows(un, a.w + (size_t)q * n + row0);
 #pragma unroll
             for (int k = 0; k < RPL; k++) un[k] = ib * un[k];
-            for (int i0 = 0; i0 < j; i0 += kCH) {
-                cplx<R> qv[kCH][RPL];
+            // newest vectors first: the reduction pass just read the basis oldest-first, so
+            // its most recent vectors are still in the last-level cache
+            const int nch = (j + kCH - 1) / kCH;
+            for (int ch = nch - 1; ch >= 0; ch--) {
+                const int i0 = ch * kCH;
+                cplx<R> qv[kCH][RPL], ai[kCH], ci[kCH];
 #pragma unroll
                 for (int ii = 0; ii < kCH; ii++) {
-                    const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+                    const int ic = min(i0 + ii, j - 1);
+                    ai[ii] = cf[ic];
+                    ci[ii] = cf[m + ic];
+                    const cplx<R>* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0;
+#pragma unroll
+                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
+                }
+#pragma unroll
+                for (int ii = 0; ii < kCH; ii++) {
+                    const bool in = i0 + ii < j;
+                    const cplx<R> aa = in ? ai[ii] : czero<R>(), cc = in ? ci[ii] : czero<R>();
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
-                        const int row = lane + 64 * k;
-                        qv[ii][k] = (i0 + ii < j && row < nrow) ? qi[row] : czero<R>();
+                        qj[k] = qj[k] - aa * qv[ii][k];
+                        un[k] = un[k] - cc * qv[ii][k];
                     }
                 }
-#pragma unroll
-                for (int ii = 0; ii < kCH; ii++) {
-                    const cplx<R> ai = sco[min(i0 + ii, m)], ci = sco[m + 1 + min(i0 + ii, m)];
-                    if (i0 + ii < j)
-#pragma unroll
-                        for (int k = 0; k < RPL; k++) {
-                            qj[k] = qj[k] - ai * qv[ii][k];
-                            un[k] = un[k] - ci * qv[ii][k];
-                        }
-                }
             }
-            const cplx<R> cj = sco[m + 1 + j];
+            const cplx<R> cj = cf[m + j];
             cplx<R>* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0;
             cplx<R>* uo = a.u + (size_t)q * n + row0;
 #pragma unroll
@@ -440,40 +453,32 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             }
             __syncthreads();
         }
-        cplx<double>* et = a.etals + (size_t)q * m;
-        for (int i = tid; i < nI; i += kCT) et[i] = gls[i];
+        cplx<R>* et = a.etals + (size_t)q * m;
+        for (int i = tid; i < nI; i += kCT) et[i] = cast<R>(gls[i]);
     };
-    // ---- x += sum_{i < maxit} eta_i q_i on the own rows (rounds of kNW systems)
+    // ---- x += sum_{i < maxit} eta_i q_i on the own rows (wave per system)
     auto xupdate = [&](uint64_t mk2) {
-        cplx<R>* sco = scoef + (size_t)wv * 2 * (m + 1);
-        for (int r0 = 0; r0 < a.nsys; r0 += kNW) {
-            const int q = r0 + wv;
-            const bool act = q < a.nsys && ((mk2 >> q) & 1ull);
-            const int nI = act ? a.ctrl[q].maxit : 0;
-            __syncthreads();
-            if (act)
-                for (int i = lane; i < nI; i += 64) sco[i] = cast<R>(a.etals[(size_t)q * m + i]);
-            __syncthreads();
-            if (!act) continue;
+        for (int q = wv; q < a.nsys; q += kNW) {
+            if (!((mk2 >> q) & 1ull)) continue;
+            const int nI = a.ctrl[q].maxit;
+            const cplx<R>* et = a.etals + (size_t)q * m;
             cplx<R> xr[RPL];
             load_rows(xr, a.x + (size_t)q * n + row0);
             for (int i0 = 0; i0 < nI; i0 += kCH) {
-                cplx<R> qv[kCH][RPL];
+                cplx<R> qv[kCH][RPL], e[kCH];
 #pragma unroll
                 for (int ii = 0; ii < kCH; ii++) {
-                    const cplx<R>* qi = a.Q + (size_t)(i0 + ii) * vs + (size_t)q * n + row0;
+                    const int ic = min(i0 + ii, nI - 1);
+                    e[ii] = et[ic];
+                    const cplx<R>* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0;
 #pragma unroll
-                    for (int k = 0; k < RPL; k++) {
-                        const int row = lane + 64 * k;
-                        qv[ii][k] = (i0 + ii < nI && row < nrow) ? qi[row] : czero<R>();
-                    }
+                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
                 }
 #pragma unroll
                 for (int ii = 0; ii < kCH; ii++) {
-                    const cplx<R> e = sco[min(i0 + ii, m)];
-                    if (i0 + ii < nI)
+                    const cplx<R> ee = i0 + ii < nI ? e[ii] : czero<R>();
 #pragma unroll
-                        for (int k = 0; k < RPL; k++) xr[k] = xr[k] + e * qv[ii][k];
+                    for (int k = 0; k < RPL; k++) xr[k] = xr[k] + ee * qv[ii][k];
                 }
             }
             cplx<R>* xo = a.x + (size_t)q * n + row0;
@@ -484,25 +489,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             }
         }
     };
-    const double links_bytes = (double)V * 5 * DOF * DOF * sR;
-    auto count = [&](double bytes, double steps) {
-        if (wg == 0 && tid == 0) {
-            a.stats[0] += bytes;
-            a.stats[1] += steps;
-        }
-    };
-    uint64_t tclk = wall_clock64();
-    auto clk = [&](int slot) {   // wall-clock time of work-group 0 per phase
-        if (wg == 0 && tid == 0) {
-            const uint64_t t = wall_clock64();
-            a.stats[2 + slot] += (double)(t - tclk);
-            tclk = t;
-        }
-    };
-
     // ---- x0 = eta, r = eta - A x0, ||eta||, ||r||
     if (wg == 0 && tid == 0)
-        for (int k = 0; k < 10; k++) a.stats[k] = 0;
+        for (int k = 0; k < 12; k++) a.stats[k] = 0;
     for (int q = wv; q < a.nsys; q += kNW) {
         if (!((a.mask >> q) & 1ull)) continue;
         for (int row = lane; row < nrow; row += 64) a.x[(size_t)q * n + row0 + row] = a.eta[(size_t)q * n + row0 + row];
@@ -538,6 +527,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             const bool hw = j < m;
             if (hw) stencil(a.u, a.w, nullptr, run, false);
             __syncthreads();
+            clk(7);
             for (int q = wv; q < a.nsys; q += kNW)
                 if ((run >> q) & 1ull) reduce_step(q, j, hw);
             clk(0);
@@ -590,49 +580,38 @@ void launch_coarsest(const CArgs<R>& a, int G, size_t sh, hipStream_t s) {
                                       (unsigned)sh, s));
 }
 
-template <typename R, int DOF>
-int max_coresident(size_t sh) {
-    int dev = 0, ncu = 0, per = 0;
+// Work-groups that are certainly co-resident: the kernel is bounded to 512 threads
+// (at most 256 VGPRs per lane, two waves per SIMD) and its LDS request is checked to be
+// at most 64 KB, so every CU holds one work-group.  0 when the device cannot launch
+// cooperatively.
+int coresident_workgroups() {
+    int dev = 0, ncu = 0, coop = 0;
     DD_HIP(hipGetDevice(&dev));
     DD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    DD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_coarsest<R, DOF>),
-                                                        kCT, sh));
-    return per * ncu;
+    DD_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    return coop ? ncu : 0;
 }
 
-template <typename R>
-size_t coarsest_lds(int m) {
-    const size_t state = sizeof(cplx<double>) * (size_t)(7 * m + 5) + 64;
-    const size_t scratch = std::max(sizeof(cplx<R>) * kNW * 2 * (size_t)(m + 1), sizeof(cplx<double>) * kBB * kBB);
-    return state + scratch;
-}
+size_t coarsest_lds(int m) { return sizeof(cplx<double>) * ((size_t)(7 * m + 5) + kBB * kBB) + 64; }
 
 }  // namespace
 
 template <typename R>
 CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap)
-    : s_(s), Nx_(Nx), Nt_(Nt), dof_(dof), m_(m), restarts_(restarts), cap_(cap), tol_(tol) {
+    : s_(s), Nx_(Nx), Nt_(Nt), dof_(dof), m_(m), restarts_(restarts), cap_(cap), m_max_(m), tol_(tol) {
     if (m < 1 || m > kMaxM || restarts < 1 || cap < 1 || cap > 64) return;
     const int V = Nx * Nt;
-    const size_t sh = coarsest_lds<R>(m);
-    if (sh > 160 * 1024) return;
-    int maxg = 0;
-    switch (dof) {
-#define DD_CASE(D)                                                                                         \
-    case D:                                                                                                \
-        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarsest<R, D>),                       \
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));                  \
-        maxg = max_coresident<R, D>(sh);                                                                   \
-        break;
-        DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
-#undef DD_CASE
-        default:
-            return;
-    }
+    const size_t sh = coarsest_lds(m);
+    if (sh > 64 * 1024) return;
+    const int maxg = coresident_workgroups();
+    if (dof != 8 && dof != 16 && dof != 24 && dof != 32) return;
     // slices of spw sites: at least max(cap, 128) work-groups (one per system for the
     // scalar state, enough to stream the basis), at most kSpwMax sites each
     spw_ = std::max(1, std::min(kSpwMax, V / std::max(cap, 128)));
     G_ = (V + spw_ - 1) / spw_;
+    if (std::getenv("DDAMG_VERBOSE"))
+        fprintf(stderr, "coarsest persistent kernel: V %d dof %d cap %d m %d lds %zu co-resident %d -> %d x %d sites\n",
+                V, dof, cap, m, sh, maxg, G_, spw_);
     if (G_ > maxg || G_ < cap) return;
     const size_t n = (size_t)V * dof, vb = sizeof(cplx<R>) * n * cap;
     Q_.alloc(vb * (m + 1));
@@ -642,9 +621,9 @@ CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, i
     Hrot_.alloc(sizeof(cplx<double>) * (size_t)cap * (m + 1) * m);
     etals_.alloc(sizeof(cplx<double>) * (size_t)cap * m);
     coef_.alloc(sizeof(cplx<double>) * (size_t)cap * (2 * m + 2));
-    nrm_.alloc(sizeof(double) * (2 * cap + 10));
+    nrm_.alloc(sizeof(double) * (2 * cap + 12));
     ctrl_.alloc(sizeof(Ctrl) * cap);
-    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 10);
+    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 12);
     ok_ = true;
 }
 
@@ -664,8 +643,8 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     a.w = w_.as<cplx<R>>();
     a.part = part_.as<cplx<double>>();
     a.Hrot = Hrot_.as<cplx<double>>();
-    a.etals = etals_.as<cplx<double>>();
-    a.coef = coef_.as<cplx<double>>();
+    a.etals = etals_.as<cplx<R>>();
+    a.coef = coef_.as<cplx<R>>();
     a.nrm = nrm_.as<double>();
     a.ctrl = ctrl_.as<Ctrl>();
     a.stats = nrm_.as<double>() + 2 * cap_;
@@ -678,7 +657,7 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     a.spw = spw_;
     a.m0 = (R)m0;
     a.tol = tol_;
-    const size_t sh = coarsest_lds<R>(m_);
+    const size_t sh = coarsest_lds(m_);
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
 #define DD_CASE(D) \
@@ -692,9 +671,9 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     Ctrl* ch = ctrlh_.as<Ctrl>();
     double* st = reinterpret_cast<double*>(ch + cap_);
     DD_HIP(hipMemcpyAsync(ch, a.ctrl, sizeof(Ctrl) * nsys, hipMemcpyDeviceToHost, s_));
-    DD_HIP(hipMemcpyAsync(st, a.stats, sizeof(double) * 10, hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipMemcpyAsync(st, a.stats, sizeof(double) * 12, hipMemcpyDeviceToHost, s_));
     DD_HIP(hipStreamSynchronize(s_));
-    for (int k = 0; k < 7; k++) phase_clk[k] += st[2 + k];
+    for (int k = 0; k < 9; k++) phase_clk[k] += st[2 + k];
     timer_add_bytes(K_COARSEST, st[0]);
     steps += (long long)st[1];
     for (int q = 0; q < nsys; q++)

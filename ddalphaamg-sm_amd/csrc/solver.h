@@ -37,12 +37,18 @@ class Krylov {
                    const Prec* M, int* iters, double* relres);
     size_t n() const { return n_; }
     long long steps = 0;   // Arnoldi steps taken (max over the batch per step), cumulative
+    // restart length (at most the constructed one), cycles and tolerance of later solves
+    void set_limits(int m, int restarts, double tol) {
+        m_ = std::max(1, std::min(m, m_max_));
+        restarts_ = restarts;
+        tol_ = tol;
+    }
 
    private:
     void ortho(int j, cplx<R>* w, uint64_t run, int nr);
     hipStream_t s_;
     size_t n_;
-    int m_, restarts_, cap_, ortho_;
+    int m_, restarts_, cap_, ortho_, m_max_;
     double tol_;
     bool flex_;
     DevMem V_, Z_, r_, w_, H_, nrm_, eta_, nit_, part_;
@@ -107,8 +113,8 @@ class Multigrid {
     // Arnoldi steps of the coarsest solver, its per-phase clocks (persistent kernel only,
     // else nullptr) and the number of coarsest solves
     void coarsest_stats(const long long** steps, const double** clk, long long* calls) const {
-        *steps = coarse_dev_ ? &coarse_dev_->steps : (coarse_ ? &coarse_->steps : nullptr);
-        *clk = coarse_dev_ ? coarse_dev_->phase_clk : nullptr;
+        *steps = use_dev_ ? &coarse_dev_->steps : (coarse_ ? &coarse_->steps : nullptr);
+        *clk = use_dev_ ? coarse_dev_->phase_clk : nullptr;
         *calls = (long long)coarse_iters.size();
     }
 
@@ -126,7 +132,9 @@ class Multigrid {
     std::vector<DevMem> part_;
     std::vector<std::unique_ptr<Krylov<R>>> kry_;   // K-cycle FGMRES of level l (index l)
     std::unique_ptr<Krylov<R>> coarse_;
-    std::unique_ptr<CoarsestGmres<R>> coarse_dev_;   // persistent kernel (ortho_coarse = 2)
+    std::unique_ptr<CoarsestGmres<R>> coarse_dev_;   // persistent kernel (ortho_coarse 2, 3)
+    bool use_dev_ = false;                           // coarsest solves run on coarse_dev_
+    double coarse_ms_[2] = {0, 0};                   // setup timing: launch-per-step, persistent
     std::vector<int> iters_scratch_;
     std::vector<int> sapc_var_;
     std::vector<double> sapc_ms_;

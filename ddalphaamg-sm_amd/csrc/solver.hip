@@ -62,7 +62,7 @@ __global__ void k_fill_hash(cplx<R>* p, size_t n, uint64_t seed) {
 // ---------------------------------------------------------------------------------------
 template <typename R>
 Krylov<R>::Krylov(hipStream_t s, size_t n, int m, int restarts, double tol, int cap, bool flexible, int ortho)
-    : s_(s), n_(n), m_(m), restarts_(restarts), cap_(cap), ortho_(ortho), tol_(tol), flex_(flexible) {
+    : s_(s), n_(n), m_(m), restarts_(restarts), cap_(cap), ortho_(ortho), m_max_(m), tol_(tol), flex_(flexible) {
     if (m < 1 || restarts < 1 || cap < 1 || cap > 64) throw BadShape("Krylov: bad m/restarts/batch");
     const size_t vb = sizeof(cplx<R>) * n * cap;
     V_.alloc(vb * (m + 1));
@@ -325,15 +325,17 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
         }
         if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
     }
-    if (p.ortho_coarse == 2) {
+    if (p.ortho_coarse >= 2) {
         const LevelGeom& Gc = g_[L - 1];
         coarse_dev_.reset(new CoarsestGmres<R>(s, Gc.Nx, Gc.Nt, Gc.dof, p.coarse_m, p.coarse_restarts, p.coarse_tol,
                                                cap));
         if (!coarse_dev_->supported()) coarse_dev_.reset();
     }
-    if (!coarse_dev_)
+    // launch-per-step engine: always for 0/1, as the alternative the setup times for 3
+    if (!coarse_dev_ || p.ortho_coarse == 3)
         coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap, false,
                                     p.ortho_coarse == 0 ? 0 : 1));
+    use_dev_ = (bool)coarse_dev_;
     iters_scratch_.resize(64);
 }
 
@@ -393,9 +395,23 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
 template <typename R>
 void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
     const int L = levels();
-    if (coarse_dev_) {
-        coarse_dev_->solve(links(L - 1), p_.m0, eta, psi, mask, iters_scratch_.data());
-    } else {
+    if (use_dev_) {
+        try {
+            coarse_dev_->solve(links(L - 1), p_.m0, eta, psi, mask, iters_scratch_.data());
+        } catch (const HipFailure& e) {
+            // the runtime refused the cooperative launch: continue with the
+            // launch-per-step engine for the rest of this hierarchy's life
+            (void)hipGetLastError();
+            if (std::getenv("DDAMG_VERBOSE"))
+                fprintf(stderr, "coarsest: persistent kernel unavailable (%s), launch-per-step engine\n",
+                        hipGetErrorString(e.err));
+            use_dev_ = false;
+            if (!coarse_)
+                coarse_.reset(new Krylov<R>(s_, g_[L - 1].n(), p_.coarse_m, p_.coarse_restarts, p_.coarse_tol,
+                                            std::max(1, p_.max_rhs), false, 1));
+        }
+    }
+    if (!use_dev_) {
         auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
         typename Krylov<R>::Func fA = A;
         coarse_->solve(eta, eta, psi, mask, fA, nullptr, iters_scratch_.data(), nullptr);
@@ -536,6 +552,41 @@ void Multigrid<R>::autotune() {
         }
         sapc_var_[l] = bestv;
         sapc_ms_[l] = best;
+    }
+    // coarsest engine (ortho_coarse = 3): the same 160 Arnoldi steps (one cycle, no
+    // convergence test) on max_rhs pseudo-random systems with each engine; keep the faster
+    if (coarse_dev_ && coarse_) {
+        const LevelGeom& G = g_[L - 1];
+        const size_t n = G.n();
+        cplx<R>* e = eta_[L - 1].template as<cplx<R>>();
+        cplx<R>* x = psi_[L - 1].template as<cplx<R>>();
+        k_fill_hash<R><<<(unsigned)((n * nr + 255) / 256), 256, 0, s_>>>(e, n * nr, 777);
+        DD_HIP(hipGetLastError());
+        double t[2] = {0, 0};
+        const int msteps = std::min(160, p_.coarse_m);
+        coarse_->set_limits(msteps, 1, 0.0);
+        coarse_dev_->set_limits(msteps, 1, 0.0);
+        for (int eng = 0; eng < 2; eng++) {
+            use_dev_ = eng == 1;
+            DD_HIP(hipEventRecord(e0, s_));
+            coarsest(e, x, full_mask(nr));
+            DD_HIP(hipEventRecord(e1, s_));
+            DD_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            t[eng] = ms;
+        }
+        coarse_->set_limits(p_.coarse_m, p_.coarse_restarts, p_.coarse_tol);
+        coarse_dev_->set_limits(p_.coarse_m, p_.coarse_restarts, p_.coarse_tol);
+        coarse_->steps = 0;
+        coarse_dev_->steps = 0;
+        use_dev_ = t[1] <= t[0];
+        coarse_ms_[0] = t[0];
+        coarse_ms_[1] = t[1];
+        coarse_iters.clear();
+        if (std::getenv("DDAMG_VERBOSE"))
+            fprintf(stderr, "autotune coarsest: launch-per-step %.3f ms, persistent %.3f ms -> %s\n", t[0], t[1],
+                    use_dev_ ? "persistent" : "launch-per-step");
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -78,6 +78,7 @@ _SIGS = {
     'dd_get_links': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
     'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
+    'dd_coarsest_stats': (C.c_int, [_ctx, _dp]),
     'dd_last_error': (C.c_char_p, []),
     'dd_timer_enable': (C.c_int, [C.c_int, C.c_int]),
     'dd_timer_reset': (C.c_int, []),
@@ -254,10 +255,10 @@ class Context:
 
     def coarsest_stats(self) -> dict:
         """Coarsest-solver counters since setup (solves, Arnoldi steps, phase ms)."""
-        out = (C.c_double * 10)()
+        out = (C.c_double * 12)()
         _chk(lib.dd_coarsest_stats(self._h, out))
-        keys = ['solves', 'steps', 'ms_stencil_reduce', 'ms_barrier1', 'ms_scalar', 'ms_barrier2', 'ms_update',
-                'ms_barrier3', 'ms_cycle_end', 'persistent']
+        keys = ['solves', 'steps', 'ms_reduce', 'ms_barrier1', 'ms_scalar', 'ms_barrier2', 'ms_update',
+                'ms_barrier3', 'ms_cycle_end', 'ms_stencil', 'ms_gather', 'persistent']
         return dict(zip(keys, list(out)))
 
     def level_shape(self, level: int):

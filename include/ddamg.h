@@ -62,7 +62,8 @@ typedef struct dd_params {
     int ortho_inner;               /* same, for the K-cycle FGMRES                  */
     int ortho_coarse;              /* coarsest GMRES: 0, 1 as above (launch per step),
                                       2 = CGS with delayed re-orthogonalisation in one
-                                      persistent kernel (default)                   */
+                                      persistent kernel, 3 = the faster of 1 and 2,
+                                      timed at setup (default; 2 without autotune)  */
     int max_rhs;                   /* largest batch solve() will be asked for (1)   */
     int autotune;                  /* 1: time the coarse-SAP kernel variants on the
                                       built hierarchy at setup and keep the fastest  */
@@ -153,10 +154,10 @@ int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
  * (2*systems_per_workgroup + batched_link_loads) and its measured half-sweep time. */
 int dd_get_tuning(dd_ctx* ctx, int level, int* variant, double* ms);
 /* Coarsest-solver statistics since setup (no reference counterpart; diagnostics):
- * out[0] coarsest solves, out[1] Arnoldi steps (max over the batch per step), out[2..8]
- * milliseconds spent by the persistent kernel's first work-group in: stencil and
- * reductions, barrier, per-system scalar phase, barrier, basis update, barrier, cycle
- * end; out[9] = 1 when the persistent kernel is in use. */
+ * out[0] coarsest solves, out[1] Arnoldi steps (max over the batch per step), out[2..10]
+ * milliseconds spent by the persistent kernel's first work-group in: reduction pass,
+ * barrier, per-system scalar phase, barrier, basis update, barrier, cycle end, stencil,
+ * gathering of the partial sums; out[11] = 1 when the persistent kernel is in use. */
 int dd_coarsest_stats(dd_ctx* ctx, double* out);
 /* Shape of level l: Nx, Nt, dof. */
 int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);

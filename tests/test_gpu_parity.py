@@ -244,7 +244,8 @@ def test_coarsest_persistent(precision):
     tight = 1e-12 if precision == 0 else 1e-6
     with ddamg.Context(N, N) as ctx:
         ctx.set_gauge(U)
-        ctx.setup(ddamg.params(3, coarse_tol=tight, **base))
+        ctx.setup(ddamg.params(3, coarse_tol=tight, ortho_coarse=2, **base))
+        assert ctx.coarsest_stats()['persistent'] == 1
         shape = ctx.level_shape(2)
         v = rng.standard_normal((17,) + shape) + 1j * rng.standard_normal((17,) + shape)
         x = ctx.cycle(2, v, np.zeros_like(v), kcycle=True, precision=precision)
