@@ -50,6 +50,8 @@ def main():
     ap.add_argument('--cpu-ranks', type=str, default='4x4')
     ap.add_argument('--skip-plain', action='store_true')
     ap.add_argument('--skip-mixed', action='store_true')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='only run three device copies of 256 MiB (scale of the HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only (the multi-GPU domain decomposition is '
@@ -57,6 +59,15 @@ def main():
         sys.exit(2)
     import torch
     import ddamg
+
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log('calibration copy: 3 x (268435456 bytes read + 268435456 bytes written)')
+        return
 
     N, nrhs = args.N, args.nrhs
     m0 = -0.18
@@ -113,7 +124,7 @@ def main():
     peak = 8000.0
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': peak, 'unit': 'GB/s',
-                'frac': achieved / peak, 'traffic': None,
+                'frac': achieved / peak, 'traffic': measured_traffic(dom, N, nrhs),
                 'avg_launch_us': dom_ms * 1e3 / max(dom_n, 1), 'launches': dom_n,
                 'algorithmic_bytes_per_launch': dom_bytes / max(dom_n, 1)}
     log(f'roofline {dom}: {achieved:.1f} GB/s over {dom_n} launches')
@@ -186,6 +197,17 @@ def main():
         'coarsest_solver_plain': coarsest_plain,
     }
     print(json.dumps(line))
+
+
+def measured_traffic(kernel, N, nrhs):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    measurement (profiles/), when it covers this workload; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                        f'{kernel}_{N}_mixed{nrhs}_pmc.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)['traffic_bytes_per_dispatch']
 
 
 def cpu_reference(theta, b, N, m0, ranks_s):
