@@ -50,6 +50,8 @@ def main():
     ap.add_argument('--cpu-ranks', type=str, default='4x4')
     ap.add_argument('--skip-plain', action='store_true')
     ap.add_argument('--skip-mixed', action='store_true')
+    ap.add_argument('--no-secondary', action='store_true',
+                    help='skip the secondary rows (256^2 plain FP64, 256^2 32-RHS and the ragged batch)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='only run three device copies of 256 MiB (scale of the HBM byte counters)')
     args = ap.parse_args()
@@ -167,6 +169,8 @@ def main():
         log('coarsest solver per plain solve: ' + json.dumps(coarsest_plain))
         ctx.close()
 
+    secondary = None if args.no_secondary else secondary_rows(ddamg, torch, args.sweeps, args.seed)
+
     # ---- CPU baseline: the reference's own solve on the host cores --------------------------
     cpu = None
     if args.cpu_baseline == 'reference':
@@ -195,8 +199,75 @@ def main():
         'kernel_breakdown_ms_per_solve': breakdown,
         'coarsest_solver_mixed': coarsest_mixed,
         'coarsest_solver_plain': coarsest_plain,
+        'secondary': secondary,
     }
     print(json.dumps(line))
+
+
+def secondary_rows(ddamg, torch, sweeps, seed):
+    """Further workloads: the 256^2 plain FP64 single-RHS solve, 32 RHS on one 256^2 field
+    (mixed, batched), and a ragged batch of 8 fields of sizes {64,96,128,128,192,256,256,256}^2
+    with m0 drawn from [-0.18, -0.10] and 4 RHS each, all solved at the same time
+    (dd_solve_many_dev).  Gauge fields: seeded quenched Metropolis, beta = 2."""
+    out = {}
+    rng = np.random.default_rng(seed + 7)
+
+    def field_ctx(N, s):
+        ctx = ddamg.Context(N, N)
+        ctx.generate_gauge(2.0, sweeps, s)
+        return ctx
+
+    def rhs(nr, N):
+        B = rng.standard_normal((nr, N, N, 2)) + 1j * rng.standard_normal((nr, N, N, 2))
+        return torch.from_numpy(B.view(np.float64).copy()).cuda(), B
+
+    # 256^2 plain FP64, 1 RHS, and mixed 32 RHS on the same field
+    ctx = field_ctx(256, seed + 11)
+    b, B = rhs(32, 256)
+    x = torch.zeros_like(b)
+    for prec, nr in ((0, 1), (1, 32)):
+        ctx.setup(hierarchy_params(ddamg, 256, prec, nr, -0.18))
+        ctx.solve_dev(b.data_ptr(), x.data_ptr(), nr)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its, info = ctx.solve_dev(b.data_ptr(), x.data_ptr(), nr)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        key = 'plain_fp64_256_1rhs' if prec == 0 else 'mixed_256_32rhs'
+        out[key] = {'ms_per_rhs': t * 1e3 / nr, 'iterations': [int(its.min()), int(its.max())],
+                    'max_relres': info.relres}
+        log(f'{key}: {t * 1e3 / nr:.2f} ms/RHS, its {its.min()}..{its.max()}')
+    ctx.close()
+
+    # ragged batch: 8 fields, 4 RHS each, mixed precision
+    sizes = [64, 96, 128, 128, 192, 256, 256, 256]
+    m0s = rng.uniform(-0.18, -0.10, len(sizes))
+    ctxs, bs, xs = [], [], []
+    t0 = time.perf_counter()
+    for i, (N, m0) in enumerate(zip(sizes, m0s)):
+        c = field_ctx(N, seed + 100 + i)
+        c.setup(hierarchy_params(ddamg, N, 1, 4, float(m0)))
+        bb, _ = rhs(4, N)
+        ctxs.append(c)
+        bs.append(bb)
+        xs.append(torch.zeros_like(bb))
+    t_setup = time.perf_counter() - t0
+    args = ([4] * len(sizes), [v.data_ptr() for v in bs], [v.data_ptr() for v in xs])
+    ddamg.solve_many_dev(ctxs, *args)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = ddamg.solve_many_dev(ctxs, *args)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out['ragged_8_fields_4rhs'] = {'sizes': sizes, 'm0': [float(v) for v in m0s], 'ms_total': t * 1e3,
+                                   'ms_per_rhs': t * 1e3 / (4 * len(sizes)), 'setup_s_all': t_setup,
+                                   'iterations': [i.iterations for i in infos],
+                                   'max_relres': max(i.relres for i in infos)}
+    log(f'ragged batch: {t * 1e3:.1f} ms for 8 fields x 4 RHS ({t * 1e3 / 32:.2f} ms/RHS), '
+        f'its {[i.iterations for i in infos]}')
+    for c in ctxs:
+        c.close()
+    return out
 
 
 def measured_traffic(kernel, N, nrhs):

@@ -3,6 +3,7 @@
 // interior order) and the device layouts, and maps exceptions to status codes.
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "solver.h"
 
@@ -242,6 +243,42 @@ double dd_setup_seconds(dd_ctx* c) { return (c && c->s) ? c->s->setup_seconds : 
 int dd_solve_dev(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
     if (check(c) || !b || !x) return DD_ERR_ARG;
     return guard([&] { return c->s->solve(nrhs, (const cplx<double>*)b, (cplx<double>*)x, iters, info); });
+}
+
+int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* const* b, double* const* x,
+                      dd_solve_info* info) {
+    if (n < 1 || !ctx || !nrhs || !b || !x) return DD_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (check(ctx[i]) || !b[i] || !x[i] || nrhs[i] < 1) return DD_ERR_ARG;
+    std::vector<int> st(n, DD_OK);
+    std::vector<std::string> err(n);
+    std::vector<std::thread> th;
+    for (int i = 0; i < n; i++)
+        th.emplace_back([&, i] {
+            try {
+                DD_HIP(hipSetDevice(ctx[i]->s->device));
+                dd_solve_info inf{};
+                st[i] = ctx[i]->s->solve(nrhs[i], (const cplx<double>*)b[i], (cplx<double>*)x[i], nullptr, &inf);
+                if (info) info[i] = inf;
+            } catch (const dd::BadShape& e) {
+                st[i] = DD_ERR_BAD_SHAPE;
+                err[i] = e.what;
+            } catch (const dd::HipFailure& e) {
+                st[i] = DD_ERR_HIP;
+                err[i] = hipGetErrorString(e.err);
+            } catch (...) {
+                st[i] = DD_ERR_STATE;
+                err[i] = "unexpected error";
+            }
+        });
+    for (auto& t : th) t.join();
+    int rc = DD_OK;
+    for (int i = 0; i < n; i++)
+        if (st[i] != DD_OK && (rc == DD_OK || rc == DD_ERR_NOT_CONVERGED)) {
+            rc = st[i];
+            if (!err[i].empty()) set_err("context " + std::to_string(i) + ": " + err[i]);
+        }
+    return rc;
 }
 
 int dd_solve(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {

@@ -26,6 +26,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -658,6 +659,10 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     a.m0 = (R)m0;
     a.tol = tol_;
     const size_t sh = coarsest_lds(m_);
+    // one cooperative grid in flight per process: two grids that each hold part of the
+    // device could wait on each other's barriers (contexts solving from several threads)
+    static std::mutex coop_mutex;
+    std::lock_guard<std::mutex> lock(coop_mutex);
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
 #define DD_CASE(D) \

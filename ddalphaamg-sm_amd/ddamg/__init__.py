@@ -13,6 +13,14 @@ import os
 
 import numpy as np
 
+try:
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7); whichever
+    # is loaded first serves the whole process.  Loading torch's first keeps torch working
+    # next to this library (the reverse order leaves torch without devices).
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get('DDAMG_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libddamg.so'))
 HEADER = os.path.join(os.path.dirname(os.path.dirname(_HERE)), 'include', 'ddamg.h')
@@ -79,6 +87,8 @@ _SIGS = {
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
     'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
     'dd_coarsest_stats': (C.c_int, [_ctx, _dp]),
+    'dd_solve_many_dev': (C.c_int, [C.c_int, C.POINTER(_ctx), _ip, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                    C.POINTER(SolveInfo)]),
     'dd_last_error': (C.c_char_p, []),
     'dd_timer_enable': (C.c_int, [C.c_int, C.c_int]),
     'dd_timer_reset': (C.c_int, []),
@@ -319,6 +329,20 @@ class Context:
         out = np.zeros((Nx, Nt, 5, d, d), complex)
         _chk(lib.dd_get_links(self._h, level, _cptr(out), precision))
         return out[:, :, 0], np.stack([out[:, :, 1], out[:, :, 2]]), np.stack([out[:, :, 3], out[:, :, 4]])
+
+
+def solve_many_dev(contexts, nrhs, b_ptrs, x_ptrs):
+    """Ragged batch (dd_solve_many_dev): context i solves nrhs[i] systems with device
+    pointers b_ptrs[i], x_ptrs[i]; all contexts at the same time.  Returns the SolveInfo
+    list."""
+    n = len(contexts)
+    ctxs = (C.c_void_p * n)(*[c._h.value for c in contexts])
+    nr = (C.c_int * n)(*nrhs)
+    bp = (C.c_void_p * n)(*b_ptrs)
+    xp = (C.c_void_p * n)(*x_ptrs)
+    info = (SolveInfo * n)()
+    _chk(lib.dd_solve_many_dev(n, ctxs, nr, bp, xp, info), allow=(DD_OK, DD_ERR_NOT_CONVERGED))
+    return list(info)
 
 
 def header_symbols(path: str = HEADER) -> list[str]:

@@ -119,6 +119,14 @@ double dd_setup_seconds(dd_ctx* ctx);
 int dd_solve(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 int dd_solve_dev(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 
+/* Ragged batch: solves the systems of n contexts (each its own lattice, gauge field,
+ * hierarchy and nrhs[i] right-hand sides; device pointers b[i], x[i]) at the same time,
+ * each context's solve on its own stream from its own host thread.  The reference has no
+ * batched or multi-lattice solve.  info (nullable) receives one entry per context.  The
+ * kernel timers are not thread-safe: leave them off during this call. */
+int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* const* b, double* const* x,
+                      dd_solve_info* info);
+
 /* Same with the initial guess taken from x (x0 = x on entry; the reference's
  * fgmres(phi, x0, x) signature, src/fgmres.cpp:5-10). */
 int dd_solve_x0(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);

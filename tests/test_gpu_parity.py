@@ -265,3 +265,36 @@ def test_coarsest_persistent(precision):
             for r in range(17):
                 res = v[r] - op.apply(x[r])
                 assert np.linalg.norm(res) < 0.1 * np.linalg.norm(v[r]) * (1 + 1e-5)
+
+
+def test_ragged_batch_equals_separate_solves():
+    """dd_solve_many_dev: several lattices of different sizes (and masses), each with its
+    own hierarchy and batch of right-hand sides, solved at the same time give every system
+    the result of solving its context alone (contexts share no arithmetic), and every
+    system reaches the outer tolerance with the FP64 operator."""
+    torch = pytest.importorskip('torch')
+    rng = np.random.default_rng(17)
+    sizes, m0s, nr = [32, 48, 64], [-0.12, -0.15, -0.1], 3
+    ctxs, bs, xs, U = [], [], [], []
+    for N, m0 in zip(sizes, m0s):
+        c = ddamg.Context(N, N)
+        th = c.generate_gauge(2.0, 200, seed=N)
+        U.append(np.exp(1j * th))
+        c.setup(ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[8, 8], m0=m0, precision=1,
+                             max_rhs=nr, rng_ranks=(2, 2)))
+        B = rng.standard_normal((nr, N, N, 2)) + 1j * rng.standard_normal((nr, N, N, 2))
+        bs.append(torch.from_numpy(B.view(np.float64).copy()).cuda())
+        xs.append(torch.zeros_like(bs[-1]))
+        ctxs.append(c)
+    infos = ddamg.solve_many_dev(ctxs, [nr] * 3, [b.data_ptr() for b in bs], [x.data_ptr() for x in xs])
+    torch.cuda.synchronize()
+    for i, (c, N, m0) in enumerate(zip(ctxs, sizes, m0s)):
+        xi = xs[i].cpu().numpy().view(np.complex128).reshape(nr, N, N, 2)
+        Bi = bs[i].cpu().numpy().view(np.complex128).reshape(nr, N, N, 2)
+        xa, its, info = c.solve(Bi)
+        assert rel(xi, xa) < 1e-12
+        assert infos[i].iterations == info.iterations
+        for r in range(nr):
+            res = np.linalg.norm(Bi[r] - orc.D_phi(U[i], xi[r], m0)) / np.linalg.norm(Bi[r])
+            assert res < 1e-10
+        c.close()
