@@ -9,6 +9,8 @@
 // passes over the basis per step); the Hessenberg matrix, Givens rotations and the
 // triangular solve run on the device, so one launch performs the whole solve.
 #pragma once
+#include <vector>
+
 #include "devmem.h"
 #include "kernels.h"
 
@@ -40,7 +42,8 @@ class CoarsestGmres {
     int Nx_, Nt_, dof_, m_, restarts_, cap_, G_ = 0, spw_ = 0, m_max_ = 0;
     double tol_;
     bool ok_ = false;
-    DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_;
+    DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_, bar_, wgclk_;
+    long long solves_ = 0;
     PinMem ctrlh_;
 };
 

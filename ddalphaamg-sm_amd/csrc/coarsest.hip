@@ -23,7 +23,6 @@
 // holds to working precision.  The break of column j-1 is seen at step j, one stencil
 // application later than with modified Gram-Schmidt; the solution only uses the
 // columns before the break.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <mutex>
@@ -34,7 +33,6 @@
 #include "coarsest.h"
 #include "mfma.h"
 
-namespace cg = cooperative_groups;
 
 namespace dd {
 
@@ -52,6 +50,59 @@ struct Ctrl {
     int flags, maxit, iters, pad;
 };
 
+// Grid barrier of the persistent kernel.  Work-groups arrive on the counter of their
+// group (wg mod kBarGroups), the last of a group on the counter of the groups, and the
+// last group advances the generation that every work-group waits for.  Thread 0 of each
+// work-group carries the device-scope release (before arriving) and acquire (after the
+// generation moved) for the whole work-group, between two work-group barriers.  Waits
+// are bounded: a wait that exceeds kBarSpins polls (seconds) raises `fail`, which every
+// waiting work-group also watches, and the kernel ends; the host reports it.  All fields
+// are zero when a launch starts.
+constexpr int kBarGroups = 8;
+constexpr unsigned kBarSpins = 1u << 25;
+
+struct GridBar {
+    unsigned cnt[kBarGroups + 1];
+    unsigned gen;
+    unsigned fail;
+    unsigned pad[5];
+};
+
+// returns false when the kernel must end (a barrier timed out somewhere)
+__device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, int G, int* sflag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int grp = wg % kBarGroups;
+        const unsigned gsz = (unsigned)((G - grp + kBarGroups - 1) / kBarGroups);
+        const unsigned ngrp = (unsigned)min(G, kBarGroups);
+        int ok = 1;
+        const unsigned old = __hip_atomic_fetch_add(&b->cnt[grp], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == gsz - 1) {
+            __hip_atomic_store(&b->cnt[grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned o2 =
+                __hip_atomic_fetch_add(&b->cnt[kBarGroups], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (o2 == ngrp - 1) {
+                __hip_atomic_store(&b->cnt[kBarGroups], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        unsigned spins = 0;
+        while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            if (__hip_atomic_load(&b->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kBarSpins) {
+                __hip_atomic_store(&b->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        gen++;
+        *sflag = ok;
+    }
+    __syncthreads();
+    return *sflag != 0;
+}
+
 template <typename R>
 struct CArgs {
     const cplx<R>* L;       // links of the coarsest level
@@ -68,6 +119,8 @@ struct CArgs {
     Ctrl* ctrl;             // [nsys]
     double* stats;          // [2 + 8]: algorithmic bytes, Arnoldi steps, phase clocks of
                             // work-group 0 (A, barrier, B, barrier, C, barrier, cycle end)
+    GridBar* bar;           // grid barrier state (zero at launch)
+    double* wgclk;          // [G][9] per-work-group phase clocks (diagnostics), or null
     uint64_t mask;
     int Nx, Nt, nsys, m, restarts, spw;
     R m0;
@@ -109,7 +162,7 @@ template <typename R, int DOF>
 __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     constexpr int RPL = (kSpwMax * DOF + 63) / 64;   // own rows per lane (at most)
     constexpr int kCH = Chunk<R>::CH, NVAL = 4 * kCH, LGV = kCH == 8 ? 5 : 4;
-    cg::grid_group grid = cg::this_grid();
+    unsigned bar_gen = 0;
     extern __shared__ unsigned char smem_raw[];
     const int m = a.m, VALS = 2 * m + 2, CST = 2 * m + 2;
     // LDS: per-system state of the work-group's system, then scratch
@@ -143,10 +196,11 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         }
     };
     uint64_t tclk = wall_clock64();
-    auto clk = [&](int slot) {   // wall-clock time of work-group 0 per phase
-        if (wg == 0 && tid == 0) {
+    auto clk = [&](int slot) {   // wall-clock time per phase: work-group 0 and every work-group
+        if (tid == 0) {
             const uint64_t t = wall_clock64();
-            a.stats[2 + slot] += (double)(t - tclk);
+            if (wg == 0) a.stats[2 + slot] += (double)(t - tclk);
+            if (a.wgclk) a.wgclk[wg * 9 + slot] += (double)(t - tclk);
             tclk = t;
         }
     };
@@ -168,15 +222,24 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         return *smask;
     };
     auto set_flags = [&](int q, int fl) { a.ctrl[q].flags = fl; };
+    // sites of the slice, wave by wave; tiles of 16 systems in pairs (one link load for both)
     auto stencil = [&](const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mk2, bool res) {
         for (int site = s0 + wv; site < s1; site += kNW)
-            for (int tl = 0; tl < ntile; tl++) {
-                if (!((mk2 >> (16 * tl)) & 0xffffull)) continue;
-                if (res)
-                    coarse_site<R, DOF, true>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
-                else
-                    coarse_site<R, DOF, false>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
-                                               lane);
+            for (int tl = 0; tl < ntile; tl += 2) {
+                if (!((mk2 >> (16 * tl)) & 0xffffffffull)) continue;
+                if (tl + 1 < ntile) {
+                    if (res)
+                        coarse_site<R, DOF, true, 2>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
+                    else
+                        coarse_site<R, DOF, false, 2>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site,
+                                                      tl, lane);
+                } else {
+                    if (res)
+                        coarse_site<R, DOF, true, 1>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
+                    else
+                        coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site,
+                                                      tl, lane);
+                }
             }
     };
     auto load_rows = [&](cplx<R> (&r)[RPL], const cplx<R>* vec) {   // vec: system base + row0
@@ -497,14 +560,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         if (!((a.mask >> q) & 1ull)) continue;
         for (int row = lane; row < nrow; row += 64) a.x[(size_t)q * n + row0 + row] = a.eta[(size_t)q * n + row0 + row];
     }
-    grid.sync();
+    if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
     stencil(a.x, a.u, a.eta, a.mask, true);
     __syncthreads();
     reduce_norm(a.mask, 2 * m, a.u);
     reduce_norm(a.mask, 2 * m + 1, a.eta);
     const int nmask = __builtin_popcountll(a.mask);
     count(links_bytes + nmask * n * sR * 6, 0);
-    grid.sync();
+    if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
     if (owner) {
         const double err = sqrt(total1(sys, 2 * m).x), nphi = sqrt(total1(sys, 2 * m + 1).x);
         if (tid == 0) {
@@ -515,7 +578,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             set_flags(sys, err == 0.0 ? 0 : (F_LIVE | F_RUN));
         }
     }
-    grid.sync();
+    if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
 
     for (int k = 0; k < a.restarts; k++) {
         const uint64_t live = sysmask(F_LIVE);
@@ -532,11 +595,11 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             for (int q = wv; q < a.nsys; q += kNW)
                 if ((run >> q) & 1ull) reduce_step(q, j, hw);
             clk(0);
-            grid.sync();
+            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(1);
             if (owner && ((run >> sys) & 1ull)) phaseB(j);
             clk(2);
-            grid.sync();
+            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(3);
             const uint64_t cont = sysmask(F_RUN) & run;
             update(j, cont);
@@ -545,18 +608,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * sR * (j + (hw ? 2 : 1)) +
                       nc * n * sR * (j + 4),
                   1);
-            grid.sync();
+            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(5);
         }
         if (owner && ((live >> sys) & 1ull)) backsub();
-        grid.sync();
+        if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         xupdate(live);
-        grid.sync();
+        if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         stencil(a.x, a.u, a.eta, live, true);
         __syncthreads();
         reduce_norm(live, 2 * m, a.u);
         count(links_bytes + nl * n * sR * 6, 0);
-        grid.sync();
+        if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         if (owner && ((live >> sys) & 1ull)) {
             const double err = sqrt(total1(sys, 2 * m).x);
             if (tid == 0) {
@@ -569,7 +632,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
                 set_flags(sys, fl);
             }
         }
-        grid.sync();
+        if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         clk(6);
     }
 }
@@ -624,7 +687,8 @@ CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, i
     coef_.alloc(sizeof(cplx<double>) * (size_t)cap * (2 * m + 2));
     nrm_.alloc(sizeof(double) * (2 * cap + 12));
     ctrl_.alloc(sizeof(Ctrl) * cap);
-    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 12);
+    bar_.alloc(sizeof(GridBar));
+    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 12 + sizeof(GridBar));
     ok_ = true;
 }
 
@@ -649,6 +713,13 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     a.nrm = nrm_.as<double>();
     a.ctrl = ctrl_.as<Ctrl>();
     a.stats = nrm_.as<double>() + 2 * cap_;
+    a.bar = bar_.as<GridBar>();
+    static const bool verbose = std::getenv("DDAMG_VERBOSE") != nullptr;
+    if (verbose && !wgclk_.bytes()) {
+        wgclk_.alloc(sizeof(double) * 9 * G_);
+        DD_HIP(hipMemsetAsync(wgclk_.as<void>(), 0, sizeof(double) * 9 * G_, s_));
+    }
+    a.wgclk = verbose ? wgclk_.as<double>() : nullptr;
     a.mask = mask;
     a.Nx = Nx_;
     a.Nt = Nt_;
@@ -663,6 +734,7 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     // device could wait on each other's barriers (contexts solving from several threads)
     static std::mutex coop_mutex;
     std::lock_guard<std::mutex> lock(coop_mutex);
+    DD_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s_));
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
 #define DD_CASE(D) \
@@ -677,7 +749,34 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     double* st = reinterpret_cast<double*>(ch + cap_);
     DD_HIP(hipMemcpyAsync(ch, a.ctrl, sizeof(Ctrl) * nsys, hipMemcpyDeviceToHost, s_));
     DD_HIP(hipMemcpyAsync(st, a.stats, sizeof(double) * 12, hipMemcpyDeviceToHost, s_));
+    GridBar* bh = reinterpret_cast<GridBar*>(st + 12);
+    DD_HIP(hipMemcpyAsync(bh, a.bar, sizeof(GridBar), hipMemcpyDeviceToHost, s_));
     DD_HIP(hipStreamSynchronize(s_));
+    if (bh->fail) {
+        fprintf(stderr, "coarsest GMRES: grid barrier timed out\n");
+        throw HipFailure(hipErrorLaunchTimeOut);
+    }
+    if (verbose && ++solves_ % 19 == 0) {   // spread of the phase times over the work-groups
+        std::vector<double> h((size_t)9 * G_);
+        DD_HIP(hipMemcpy(h.data(), a.wgclk, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+        const char* nm[9] = {"reduce", "barrier1", "scalar", "barrier2", "update", "barrier3", "cycle", "stencil",
+                             "gather"};
+        for (int k = 0; k < 9; k++) {
+            double mn = 1e300, mx = 0, sum = 0;
+            int imx = 0;
+            for (int g = 0; g < G_; g++) {
+                const double v = h[(size_t)g * 9 + k];
+                mn = std::min(mn, v);
+                if (v > mx) {
+                    mx = v;
+                    imx = g;
+                }
+                sum += v;
+            }
+            fprintf(stderr, "coarsest wg clocks %-8s min %.3g mean %.3g max %.3g (wg %d)\n", nm[k], mn, sum / G_, mx,
+                    imx);
+        }
+    }
     for (int k = 0; k < 9; k++) phase_clk[k] += st[2 + k];
     timer_add_bytes(K_COARSEST, st[0]);
     steps += (long long)st[1];

@@ -188,13 +188,13 @@ void dirac_links(hipStream_t s, const cplx<R>* U, cplx<R>* L, int Nx, int Nt) {
 // layout in mfma.h).  The neighbour spinors are read straight from the vectors (four
 // lanes per system share each 16-byte segment).
 // ======================================================================================
-template <typename R, int DOF, bool RES>
+template <typename R, int DOF, bool RES, int NT>
 __global__ __launch_bounds__(256) void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
                                                 int Nt, R m0, uint64_t mask, size_t ld, int nrhs) {
     const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (n >= Nx * Nt) return;   // uniform per wave
-    coarse_site<R, DOF, RES>(L, in, out, b, Nx, Nt, m0, mask, ld, nrhs, n, blockIdx.y, threadIdx.x & 63);
+    coarse_site<R, DOF, RES, NT>(L, in, out, b, Nx, Nt, m0, mask, ld, nrhs, n, blockIdx.y * NT, threadIdx.x & 63);
 }
 
 #define DD_DOF_LIST(X) X(8) X(16) X(24) X(32)
@@ -205,12 +205,20 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
     timer_begin(s, K_COARSE);
     constexpr int kWaves = 4;   // sites per work-group
     const int top = mask ? 64 - __builtin_clzll(mask) : 1;   // systems up to the highest one set
-    dim3 grid(cdivu((size_t)Nx * Nt, kWaves), cdivu((size_t)std::min(top, nrhs), 16));
+    const int tiles = (int)cdivu((size_t)std::min(top, nrhs), 16);
+    // two 16-system tiles per wave share every link load once there are more than 16 systems
+    const bool two = tiles >= 2;
+    dim3 grid(cdivu((size_t)Nx * Nt, kWaves), two ? cdivu(tiles, 2) : tiles);
     switch (dof) {
-#define DD_CASE(D)                                                                                        \
-    case D:                                                                                               \
-        if (b) k_coarse<R, D, true><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
-        else k_coarse<R, D, false><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+#define DD_CASE(D)                                                                                            \
+    case D:                                                                                                   \
+        if (two) {                                                                                            \
+            if (b) k_coarse<R, D, true, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+            else k_coarse<R, D, false, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+        } else {                                                                                              \
+            if (b) k_coarse<R, D, true, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+            else k_coarse<R, D, false, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+        }                                                                                                     \
         break;
         DD_DOF_LIST(DD_CASE)
 #undef DD_CASE

@@ -86,55 +86,104 @@ __device__ __forceinline__ void mfma_hop(typename Mfma<R>::acc_t (&acc)[DOF / 8]
     }
 }
 
-// One site of the generic level stencil for a tile of 16 systems (one wave):
+// acc[k][t] += the same hop for NT tiles of 16 systems: each link operand is loaded once
+// and used for the NT tiles' products.
+template <typename R, int DOF, int NT>
+__device__ __forceinline__ void mfma_hop_nt(typename Mfma<R>::acc_t (&acc)[NT][DOF / 8], const R* __restrict__ Lh,
+                                            const LinkLane& ll, const R (&bv)[NT][DOF / 2]) {
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        R a[MT];
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            a[t] = Lh[ll.aoff + link_step<DOF>(s, t)];
+            a[t] = ll.aneg ? -a[t] : a[t];
+        }
+#pragma unroll
+        for (int k = 0; k < NT; k++)
+#pragma unroll
+            for (int t = 0; t < MT; t++) acc[k][t] = Mfma<R>::mac(a[t], bv[k][s], acc[k][t]);
+    }
+}
+
+// One site of the generic level stencil for NT tiles of 16 systems (one wave):
 //   out = (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))      (RES: b - that)
-// for site n and systems 16*tile + (lane&15) that are in `mask` and below nrhs;
-// links [V][5][DOF][DOF] column-major, vectors [system][V][DOF] with stride ld.
-template <typename R, int DOF, bool RES>
+// for site n and systems 16*(tile0 + k) + (lane&15), k < NT, that are in `mask` and below
+// nrhs; links [V][5][DOF][DOF] column-major, vectors [system][V][DOF] with stride ld.
+// Every load is unconditional (indices of systems that do not take part are replaced by
+// system 0) and masked afterwards, so no branch separates the loads of a hop.
+template <typename R, int DOF, bool RES, int NT = 1>
 __device__ __forceinline__ void coarse_site(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                             cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx, int Nt,
-                                            R m0, uint64_t mask, size_t ld, int nrhs, int n, int tile, int lane) {
+                                            R m0, uint64_t mask, size_t ld, int nrhs, int n, int tile0, int lane) {
     using acc_t = typename Mfma<R>::acc_t;
     constexpr int MT = DOF / 8, KS = DOF / 2;
     const int g = lane >> 4;
-    const int j = tile * 16 + (lane & 15);
-    const bool on = j < nrhs && ((mask >> j) & 1ull);
+    int jj[NT];
+    bool on[NT];
+    const R* vin[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        jj[k] = (tile0 + k) * 16 + (lane & 15);
+        on[k] = jj[k] < nrhs && ((mask >> jj[k]) & 1ull);
+        vin[k] = reinterpret_cast<const R*>(in + (size_t)(on[k] ? jj[k] : 0) * ld);
+    }
     const int x = n / Nt, t = n - x * Nt;
-    const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
-                       (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
-    const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+    // neighbour of hop h (0: self, 1: t+1, 2: x+1, 3: t-1, 4: x-1) and its boundary sign,
+    // computed per hop (no private array indexed by the rolled hop loop)
+    auto nbr = [&](int h) {
+        return h == 0 ? n
+             : h == 1 ? ((t == Nt - 1) ? n - (Nt - 1) : n + 1)
+             : h == 2 ? ((x == Nx - 1) ? t : n + Nt)
+             : h == 3 ? ((t == 0) ? n + (Nt - 1) : n - 1)
+                      : ((x == 0) ? (Nx - 1) * Nt + t : n - Nt);
+    };
+    auto sgn = [&](int h) { return (h == 1 && t == Nt - 1) || (h == 3 && t == 0) ? R(-1) : R(1); };
     const LinkLane ll = LinkLane::make<R, DOF>(lane);
     const R* Ln = reinterpret_cast<const R*>(L + (size_t)n * 5 * DOF * DOF);
-    const R* vin = reinterpret_cast<const R*>(in + (size_t)(on ? j : 0) * ld);
-    acc_t acc[MT];
+    acc_t acc[NT][MT];
 #pragma unroll
-    for (int tt = 0; tt < MT; tt++) acc[tt] = acc_t{0, 0, 0, 0};
+    for (int k = 0; k < NT; k++)
+#pragma unroll
+        for (int tt = 0; tt < MT; tt++) acc[k][tt] = acc_t{0, 0, 0, 0};
     // one hop at a time: its link and vector loads are issued together, hops are not
     // interleaved (bounded register use inside larger kernels)
 #pragma unroll 1
     for (int h = 0; h < 5; h++) {
         // B operand: part (g & 1) of column 2s + (g >> 1) -> real index 4s + g of the site
-        const R* vb = vin + (size_t)nb[h] * DOF * 2 + g;
-        R bv[KS];
+        R bv[NT][KS];
+        const int nh = nbr(h);
+        const R sh = sgn(h);
 #pragma unroll
-        for (int s2 = 0; s2 < KS; s2++) bv[s2] = on ? sg[h] * vb[4 * s2] : R(0);
-        mfma_hop<R, DOF>(acc, Ln + (size_t)h * DOF * DOF * 2, ll, bv);
-    }
-    if (!on) return;
-    const R mass = m0 + R(2);
-    const size_t base = (size_t)j * ld + (size_t)n * DOF;
+        for (int k = 0; k < NT; k++) {
+            const R* vb = vin[k] + (size_t)nh * DOF * 2 + g;
 #pragma unroll
-    for (int tt = 0; tt < MT; tt++) {
-        const int r0 = 8 * tt + 2 * g;
-        const cplx<R> v0 = in[base + r0], v1 = in[base + r0 + 1];
-        cplx<R> o0 = mk<R>(mass * v0.x - acc[tt][0], mass * v0.y - acc[tt][1]);
-        cplx<R> o1 = mk<R>(mass * v1.x - acc[tt][2], mass * v1.y - acc[tt][3]);
-        if (RES) {
-            o0 = b[base + r0] - o0;
-            o1 = b[base + r0 + 1] - o1;
+            for (int s2 = 0; s2 < KS; s2++) {
+                const R v = vb[4 * s2];
+                bv[k][s2] = on[k] ? sh * v : R(0);
+            }
         }
-        out[base + r0] = o0;
-        out[base + r0 + 1] = o1;
+        mfma_hop_nt<R, DOF, NT>(acc, Ln + (size_t)h * DOF * DOF * 2, ll, bv);
+    }
+    const R mass = m0 + R(2);
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        if (!on[k]) continue;
+        const size_t base = (size_t)jj[k] * ld + (size_t)n * DOF;
+#pragma unroll
+        for (int tt = 0; tt < MT; tt++) {
+            const int r0 = 8 * tt + 2 * g;
+            const cplx<R> v0 = in[base + r0], v1 = in[base + r0 + 1];
+            cplx<R> o0 = mk<R>(mass * v0.x - acc[k][tt][0], mass * v0.y - acc[k][tt][1]);
+            cplx<R> o1 = mk<R>(mass * v1.x - acc[k][tt][2], mass * v1.y - acc[k][tt][3]);
+            if (RES) {
+                o0 = b[base + r0] - o0;
+                o1 = b[base + r0 + 1] - o1;
+            }
+            out[base + r0] = o0;
+            out[base + r0 + 1] = o1;
+        }
     }
 }
 
