@@ -32,9 +32,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+COARSE_BASIS = None   # None: the library default; 0 FP32, 1 binary16 (--coarse-basis)
+
+
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
-    return ddamg.params(3, agg=[(4, 4), (4, 4)], sap_block=[(4, 4), (4, 4)], nv=[12, 12], m0=m0, nu2=2,
-                        fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
+    p = ddamg.params(3, agg=[(4, 4), (4, 4)], sap_block=[(4, 4), (4, 4)], nv=[12, 12], m0=m0, nu2=2,
+                     fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
+    if COARSE_BASIS is not None:
+        p.coarse_basis = COARSE_BASIS
+    return p
 
 
 def main():
@@ -54,7 +60,11 @@ def main():
                     help='skip the secondary rows (256^2 plain FP64, 256^2 32-RHS and the ragged batch)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='only run three device copies of 256 MiB (scale of the HBM byte counters)')
+    ap.add_argument('--coarse-basis', type=int, choices=[0, 1], default=None,
+                    help='storage of the coarsest GMRES basis in the FP32 hierarchy (0 FP32, 1 binary16)')
     args = ap.parse_args()
+    global COARSE_BASIS
+    COARSE_BASIS = args.coarse_basis
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only (the multi-GPU domain decomposition is '
               'design-only); run with --gpus 1', file=sys.stderr)

@@ -133,6 +133,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->max_rhs = 1;
     p->autotune = 1;
     p->sap_variant = -1;
+    p->coarse_basis = 1;
     return DD_OK;
 }
 

@@ -19,7 +19,9 @@ namespace dd {
 template <typename R>
 class CoarsestGmres {
    public:
-    CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap);
+    // half_basis (FP32 engine only): the Krylov basis is stored in IEEE binary16
+    CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap,
+                  bool half_basis = false);
     // false when the lattice is too large for one co-resident grid (the caller then
     // uses the launch-per-step Krylov engine)
     bool supported() const { return ok_; }
@@ -38,10 +40,12 @@ class CoarsestGmres {
     double phase_clk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
    private:
+    template <typename S>
+    void run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int nsys, int* iters);
     hipStream_t s_;
     int Nx_, Nt_, dof_, m_, restarts_, cap_, G_ = 0, spw_ = 0, m_max_ = 0;
     double tol_;
-    bool ok_ = false;
+    bool half_ = false, ok_ = false;
     DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_, bar_, wgclk_;
     long long solves_ = 0;
     PinMem ctrlh_;

@@ -18,6 +18,11 @@
 //      rotation of that column and the break test |g_j| < tol ||eta||; first-pass
 //      coefficients of the next column h1_{i,j} = b_i/beta, h1_{j,j} = (d - a^H b)/beta^2;
 //   C  q_j = (u_j - Q a)/beta,  u_{j+1} = w/beta - Q_j h1_{.,j}  (one pass).
+// The basis may be stored in IEEE binary16 (compressed-basis GMRES, Aliaga, Anzt,
+// Gruetzmacher, Quintana-Orti, Tomas 2022): q_j is rounded once when it is written and
+// every product is formed in the working precision from the stored values, so the
+// orthogonalisation and the Arnoldi relation refer to the stored basis; the passes then
+// move half the bytes.
 // Using w/beta for A q_j leaves an error A Q a/beta inside span(Q_j); the next step's
 // projection removes it and adds it to the Hessenberg column, so the Arnoldi relation
 // holds to working precision.  The break of column j-1 is seen at step j, one stencil
@@ -31,6 +36,7 @@
 #include <cstdlib>
 
 #include "coarsest.h"
+#include "lanes.h"
 #include "mfma.h"
 
 
@@ -103,15 +109,35 @@ __device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, 
     return *sflag != 0;
 }
 
+// complex number of two IEEE binary16 values (basis storage)
+struct chalf {
+    _Float16 x, y;
+};
 template <typename R>
+__device__ __forceinline__ cplx<R> widen(cplx<R> v) { return v; }
+__device__ __forceinline__ cplx<float> widen(chalf v) { return mk<float>((float)v.x, (float)v.y); }
+template <typename R>
+__device__ __forceinline__ cplx<R> narrow(cplx<R>* d, cplx<R> v) {   // store, return the stored value
+    *d = v;
+    return v;
+}
+__device__ __forceinline__ cplx<float> narrow(chalf* d, cplx<float> v) {
+    chalf h;
+    h.x = (_Float16)v.x;
+    h.y = (_Float16)v.y;
+    *d = h;
+    return widen(h);
+}
+
+template <typename R, typename S>
 struct CArgs {
     const cplx<R>* L;       // links of the coarsest level
     const cplx<R>* eta;     // right-hand sides [nsys][n]
     cplx<R>* x;             // solutions [nsys][n]
-    cplx<R>* Q;             // basis [m+1][nsys][n]
+    S* Q;                   // basis [m+1][nsys][n] (storage type S)
     cplx<R>* u;             // [nsys][n]
     cplx<R>* w;             // [nsys][n]
-    cplx<double>* part;     // [nsys][2m+2][G] per-work-group partial sums
+    cplx<double>* part;     // [G][nsys][2m+2] per-work-group partial sums
     cplx<double>* Hrot;     // [nsys][m+1][m] rotated (triangular) Hessenberg columns
     cplx<R>* etals;         // [nsys][m] least-squares solution (working precision)
     cplx<R>* coef;          // [nsys][2m+2]: a_i (i<j) | h1_{i,j} (i<=j) | 1/beta
@@ -129,39 +155,60 @@ struct CArgs {
 
 __device__ __forceinline__ cplx<double> cdivr(cplx<double> a, double d) { return mk<double>(a.x / d, a.y / d); }
 
-// Sums NV values (a power of two <= 32) over the 64 lanes of a wave with about NV
-// shuffles (each step hands half of the values to the partner lane): afterwards the
-// lanes with f = lane >> (6 - log2 NV) hold the total of value f.
+// Sums NV values (a power of two, 4 <= NV <= 32) over the 64 lanes of a wave with NV
+// lane exchanges (each step hands half of the values to the partner lane, through the
+// DPP / permlane networks): afterwards the lanes with f = lane >> (6 - log2 NV) hold the
+// total of value f.  The stages are instantiated per offset, so every array index is a
+// constant and the values stay in registers.
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+template <typename R, int NV, int ST>
+__device__ __forceinline__ void bfly_stage(R (&F)[NV], int lane) {
+    constexpr int O = 32 >> ST, HALF = (NV / 2) >> ST;
+#pragma unroll
+    for (int k = 0; k < HALF; k++) {
+        R A = F[k], B = F[k + HALF];
+        pair_swap<O>(A, B, lane);
+        F[k] = A + B;
+    }
+    if constexpr (ST + 1 < ilog2c(NV)) bfly_stage<R, NV, ST + 1>(F, lane);
+}
+template <typename R, int O>
+__device__ __forceinline__ R bfly_tail(R t, int lane) {   // t + partners at offsets O, O/2, ..., 1
+    if constexpr (O >= 1) return bfly_tail<R, O / 2>(t + lane_xor<O>(t, lane), lane);
+    else return t;
+}
 template <typename R, int NV>
 __device__ __forceinline__ R butterfly(R (&F)[NV], int lane) {
-    constexpr int LG = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : NV == 4 ? 2 : NV == 2 ? 1 : 0;
-    static_assert((1 << LG) == NV, "NV must be a power of two <= 32");
-#pragma unroll
-    for (int st = 0; st < LG; st++) {
-        const int o = 32 >> st, half = (NV / 2) >> st;
-        const bool upper = (lane & o) != 0;
-#pragma unroll
-        for (int k = 0; k < half; k++) {
-            const R send = upper ? F[k] : F[k + half];
-            const R keep = upper ? F[k + half] : F[k];
-            F[k] = keep + __shfl_xor(send, o, 64);
-        }
-    }
-    R t = F[0];
-#pragma unroll
-    for (int o = (32 >> LG); o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    return t;
+    static_assert(NV >= 4 && NV <= 32 && (NV & (NV - 1)) == 0, "NV must be a power of two in 4..32");
+    bfly_stage<R, NV, 0>(F, lane);
+    return bfly_tail<R, (32 >> ilog2c(NV))>(F[0], lane);
 }
 
-template <typename R>
-struct Chunk {   // basis vectors in flight per pass
-    static constexpr int CH = sizeof(R) == 4 ? 8 : 4;
+constexpr int pow2_floor(int v) {
+    int p = 1;
+    while (2 * p <= v) p *= 2;
+    return p;
+}
+// basis vectors in flight per wave for storage S and RPL rows per lane: the reduction
+// pass holds up to 96 registers of basis values (binary16, FP32; 4 vectors for FP64), the
+// update passes half as many next to their coefficients
+template <typename S, int RPL>
+struct Chunk {
+    static constexpr int words = (int)sizeof(S) / 4;
+    static constexpr int RED = sizeof(S) == 16 ? 4 : pow2_floor(96 / (RPL * words) < 32 ? 96 / (RPL * words) : 32);
+    static constexpr int UPD = sizeof(S) == 16 ? 4 : (RED / 2 < 4 ? 4 : RED / 2);
 };
 
-template <typename R, int DOF>
-__global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
-    constexpr int RPL = (kSpwMax * DOF + 63) / 64;   // own rows per lane (at most)
-    constexpr int kCH = Chunk<R>::CH, NVAL = 4 * kCH, LGV = kCH == 8 ? 5 : 4;
+template <typename R, typename S, int DOF>
+__global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
+    // RPL consecutive rows of the slice per lane; RPL divides DOF, so a lane's rows lie
+    // all inside the slice or all outside
+    constexpr int RPL = (kSpwMax * DOF + 63) / 64;
+    static_assert(DOF % RPL == 0, "rows per lane must divide the site's rows");
+    // kCH basis vectors loaded per pass step, reduced in groups of kGV (NVAL sums each)
+    constexpr int kCH = Chunk<S, RPL>::RED, kCU = Chunk<S, RPL>::UPD, kGV = kCH < 8 ? kCH : 8, NVAL = 4 * kGV,
+                  LGV = kGV == 8 ? 5 : 4;
+    static_assert(kCH % kGV == 0 && (kGV == 8 || kGV == 4), "chunk of 4, 8 or a multiple of 8 vectors");
     unsigned bar_gen = 0;
     extern __shared__ unsigned char smem_raw[];
     const int m = a.m, VALS = 2 * m + 2, CST = 2 * m + 2;
@@ -176,6 +223,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     int* si = reinterpret_cast<int*>(sc + 4);                          // [4]
     uint64_t* smask = reinterpret_cast<uint64_t*>(si + 4);             // [1]
     cplx<double>* blk = reinterpret_cast<cplx<double>*>(smask + 2);    // [kBB][kBB] triangular solve
+    cplx<double>* red = blk;   // [kNW][64] wave sums of the totals (phase B; blk is used after it)
+    static_assert(kBB * kBB >= kNW * 64, "LDS scratch of the totals");
 
     const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int V = a.Nx * a.Nt;
@@ -184,9 +233,15 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     const int nrow = max(0, s1 - s0) * DOF;
     const size_t row0 = (size_t)s0 * DOF;
     const int ntile = (a.nsys + 15) / 16;
-    const double sR = sizeof(cplx<R>);
+    const double sR = sizeof(cplx<R>), sQ = sizeof(S);
+    // the lane's rows lb..lb+RPL-1 (lanes past the slice read its last rows, masked)
+    const bool lrow = lane * RPL < nrow;
+    const int lb = lrow ? lane * RPL : nrow - RPL;
     const int sys = wg;   // the system whose scalars this work-group owns (G >= nsys)
     const bool owner = sys < a.nsys && ((a.mask >> sys) & 1ull);
+    // partial sums: work-group-major [G][nsys][VALS], each work-group writes its own
+    // contiguous block
+    auto pidx = [&](int q, int v, int w) -> size_t { return ((size_t)w * a.nsys + q) * VALS + v; };
 
     const double links_bytes = (double)V * 5 * DOF * DOF * sR;
     auto count = [&](double bytes, double steps) {
@@ -196,7 +251,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         }
     };
     uint64_t tclk = wall_clock64();
-    auto clk = [&](int slot) {   // wall-clock time per phase: work-group 0 and every work-group
+    // wall-clock time per phase: work-group 0, and every work-group in the diagnostic
+    // mode, where a phase ends when all waves of the work-group are done with it
+    auto clk = [&](int slot) {
+        if (a.wgclk) __syncthreads();
         if (tid == 0) {
             const uint64_t t = wall_clock64();
             if (wg == 0) a.stats[2 + slot] += (double)(t - tclk);
@@ -245,13 +303,13 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     auto load_rows = [&](cplx<R> (&r)[RPL], const cplx<R>* vec) {   // vec: system base + row0
 #pragma unroll
         for (int k = 0; k < RPL; k++) {
-            const int row = lane + 64 * k;
-            r[k] = row < nrow ? vec[row] : czero<R>();
+            const cplx<R> v = vec[lb + k];
+            r[k] = lrow ? v : czero<R>();
         }
     };
     auto put1 = [&](int q, int v, cplx<R> p) {   // single work-group partial
         p = wave_sum(p, 64);
-        if (lane == 0) a.part[((size_t)q * VALS + v) * G + wg] = mk<double>((double)p.x, (double)p.y);
+        if (lane == 0) a.part[pidx(q, v, wg)] = mk<double>((double)p.x, (double)p.y);
     };
     // ---- phase A reduction: a_i, c and (hw) b_i, d of system q over the own rows
     auto reduce_step = [&](int q, int j, bool hw) {
@@ -273,34 +331,39 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         for (int i0 = 0; i0 < j; i0 += kCH) {
             // unconditional loads at clamped indices (no branches between the loads), the
             // terms outside the basis or the own rows masked afterwards
-            cplx<R> qv[kCH][RPL];
+            S qv[kCH][RPL];
 #pragma unroll
             for (int ii = 0; ii < kCH; ii++) {
-                const cplx<R>* qi = a.Q + (size_t)min(i0 + ii, j - 1) * vs + (size_t)q * n + row0;
+                const S* qi = a.Q + (size_t)min(i0 + ii, j - 1) * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
-                for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
+                for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
             }
-            R F[NVAL];
+            // groups of kGV vectors: 4 kGV <= 32 sums per butterfly
 #pragma unroll
-            for (int ii = 0; ii < kCH; ii++) {
-                cplx<R> pa = czero<R>(), pb = czero<R>();
+            for (int g0 = 0; g0 < kCH; g0 += kGV) {
+                R F[NVAL];
 #pragma unroll
-                for (int k = 0; k < RPL; k++) {
-                    const bool ok = i0 + ii < j && lane + 64 * k < nrow;
-                    const cplx<R> qq = ok ? qv[ii][k] : czero<R>();
-                    pa = pa + cmul(qq, ur[k]);
-                    pb = pb + cmul(qq, wr[k]);
+                for (int gi = 0; gi < kGV; gi++) {
+                    const int ii = g0 + gi;
+                    cplx<R> pa = czero<R>(), pb = czero<R>();
+                    const bool ok = i0 + ii < j && lrow;
+#pragma unroll
+                    for (int k = 0; k < RPL; k++) {
+                        const cplx<R> qq = ok ? widen(qv[ii][k]) : czero<R>();
+                        pa = pa + cmul(qq, ur[k]);
+                        pb = pb + cmul(qq, wr[k]);
+                    }
+                    F[4 * gi] = pa.x;
+                    F[4 * gi + 1] = pa.y;
+                    F[4 * gi + 2] = pb.x;
+                    F[4 * gi + 3] = pb.y;
                 }
-                F[4 * ii] = pa.x;
-                F[4 * ii + 1] = pa.y;
-                F[4 * ii + 2] = pb.x;
-                F[4 * ii + 3] = pb.y;
-            }
-            const R tsum = butterfly<R, NVAL>(F, lane);
-            const int f = lane >> (6 - LGV), ii = f >> 2, comp = f & 3;
-            if ((lane & ((1 << (6 - LGV)) - 1)) == 0 && i0 + ii < j && (hw || comp < 2)) {
-                const int v = (comp < 2 ? 0 : m) + i0 + ii;
-                pd[(((size_t)q * VALS + v) * G + wg) * 2 + (comp & 1)] = (double)tsum;
+                const R tsum = butterfly<R, NVAL>(F, lane);
+                const int f = lane >> (6 - LGV), ii = g0 + (f >> 2), comp = f & 3;
+                if ((lane & ((1 << (6 - LGV)) - 1)) == 0 && i0 + ii < j && (hw || comp < 2)) {
+                    const int v = (comp < 2 ? 0 : m) + i0 + ii;
+                    pd[pidx(q, v, wg) * 2 + (comp & 1)] = (double)tsum;
+                }
             }
         }
     };
@@ -318,9 +381,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     // total of value v of system q over the work-groups (whole work-group; every thread)
     auto total1 = [&](int q, int v) -> cplx<double> {
         if (wv == 0) {
-            const cplx<double>* pp = a.part + ((size_t)q * VALS + v) * G;
             cplx<double> t = czero<double>();
-            for (int k = lane; k < G; k += 64) t = t + pp[k];
+            for (int k = lane; k < G; k += 64) t = t + a.part[pidx(q, v, k)];
             t = wave_sum(t, 64);
             if (lane == 0) tot[v] = t;
         }
@@ -334,28 +396,30 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
         const int q = sys;
         const bool hw = j < m;
         const int nv = hw ? 2 * j + 2 : j + 1;
-        {   // totals: 16 lanes per value
-            const int grp = tid >> 4, sub = tid & 15;
-            for (int base = 0; base < nv; base += kCT / 16) {
-                const int idx = base + grp;
-                cplx<double> t = czero<double>();
+        {   // totals: a lane per value, wave w adds the work-groups w, w + 8, ...; the
+            // waves' sums are added in LDS in a fixed order
+            for (int base = 0; base < nv; base += 64) {
+                const int idx = base + lane;
                 int v = 0;
+                if (idx < j) v = idx;
+                else if (hw && idx < 2 * j) v = m + (idx - j);
+                else v = 2 * m + min(idx - (hw ? 2 * j : j), 1);
+                cplx<double> t = czero<double>();
                 if (idx < nv) {
-                    if (idx < j) v = idx;
-                    else if (hw && idx < 2 * j) v = m + (idx - j);
-                    else v = 2 * m + (idx - (hw ? 2 * j : j));
-                    const cplx<double>* pp = a.part + ((size_t)q * VALS + v) * G;
-                    for (int k = sub; k < G; k += 16) t = t + pp[k];
+#pragma unroll 8
+                    for (int k = wv; k < G; k += kNW) t = t + a.part[pidx(q, v, k)];
                 }
+                red[wv * 64 + lane] = t;
+                __syncthreads();
+                if (wv == 0 && idx < nv) {
+                    cplx<double> s2 = red[lane];
 #pragma unroll
-                for (int o = 8; o > 0; o >>= 1) {
-                    t.x += __shfl_xor(t.x, o, 64);
-                    t.y += __shfl_xor(t.y, o, 64);
+                    for (int w = 1; w < kNW; w++) s2 = s2 + red[w * 64 + lane];
+                    tot[v] = s2;
                 }
-                if (idx < nv && sub == 0) tot[v] = t;
+                __syncthreads();
             }
         }
-        __syncthreads();
         clk(8);
         if (wv == 0) {
             double sa = 0;
@@ -444,41 +508,41 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             for (int k = 0; k < RPL; k++) un[k] = ib * un[k];
             // newest vectors first: the reduction pass just read the basis oldest-first, so
             // its most recent vectors are still in the last-level cache
-            const int nch = (j + kCH - 1) / kCH;
+            const int nch = (j + kCU - 1) / kCU;
             for (int ch = nch - 1; ch >= 0; ch--) {
-                const int i0 = ch * kCH;
-                cplx<R> qv[kCH][RPL], ai[kCH], ci[kCH];
+                const int i0 = ch * kCU;
+                S qv[kCU][RPL];
+                cplx<R> ai[kCU], ci[kCU];
 #pragma unroll
-                for (int ii = 0; ii < kCH; ii++) {
+                for (int ii = 0; ii < kCU; ii++) {
                     const int ic = min(i0 + ii, j - 1);
                     ai[ii] = cf[ic];
                     ci[ii] = cf[m + ic];
-                    const cplx<R>* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0;
+                    const S* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
-                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
+                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
                 }
 #pragma unroll
-                for (int ii = 0; ii < kCH; ii++) {
+                for (int ii = 0; ii < kCU; ii++) {
                     const bool in = i0 + ii < j;
                     const cplx<R> aa = in ? ai[ii] : czero<R>(), cc = in ? ci[ii] : czero<R>();
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
-                        qj[k] = qj[k] - aa * qv[ii][k];
-                        un[k] = un[k] - cc * qv[ii][k];
+                        const cplx<R> qq = widen(qv[ii][k]);
+                        qj[k] = qj[k] - aa * qq;
+                        un[k] = un[k] - cc * qq;
                     }
                 }
             }
+            // u_{j+1} is orthogonalised against the stored q_j
             const cplx<R> cj = cf[m + j];
-            cplx<R>* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0;
-            cplx<R>* uo = a.u + (size_t)q * n + row0;
+            S* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0 + lb;
+            cplx<R>* uo = a.u + (size_t)q * n + row0 + lb;
+            if (lrow) {
 #pragma unroll
-            for (int k = 0; k < RPL; k++) {
-                const int row = lane + 64 * k;
-                if (row < nrow) {
-                    qj[k] = ib * qj[k];
-                    un[k] = un[k] - cj * qj[k];
-                    qo[row] = qj[k];
-                    uo[row] = un[k];
+                for (int k = 0; k < RPL; k++) {
+                    const cplx<R> qs = narrow(qo + k, ib * qj[k]);
+                    uo[k] = un[k] - cj * qs;
                 }
             }
         }
@@ -528,29 +592,28 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             const cplx<R>* et = a.etals + (size_t)q * m;
             cplx<R> xr[RPL];
             load_rows(xr, a.x + (size_t)q * n + row0);
-            for (int i0 = 0; i0 < nI; i0 += kCH) {
-                cplx<R> qv[kCH][RPL], e[kCH];
+            for (int i0 = 0; i0 < nI; i0 += kCU) {
+                S qv[kCU][RPL];
+                cplx<R> e[kCU];
 #pragma unroll
-                for (int ii = 0; ii < kCH; ii++) {
+                for (int ii = 0; ii < kCU; ii++) {
                     const int ic = min(i0 + ii, nI - 1);
                     e[ii] = et[ic];
-                    const cplx<R>* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0;
+                    const S* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
-                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[min(lane + 64 * k, nrow - 1)];
+                    for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
                 }
 #pragma unroll
-                for (int ii = 0; ii < kCH; ii++) {
+                for (int ii = 0; ii < kCU; ii++) {
                     const cplx<R> ee = i0 + ii < nI ? e[ii] : czero<R>();
 #pragma unroll
-                    for (int k = 0; k < RPL; k++) xr[k] = xr[k] + ee * qv[ii][k];
+                    for (int k = 0; k < RPL; k++) xr[k] = xr[k] + ee * widen(qv[ii][k]);
                 }
             }
-            cplx<R>* xo = a.x + (size_t)q * n + row0;
+            cplx<R>* xo = a.x + (size_t)q * n + row0 + lb;
+            if (lrow)
 #pragma unroll
-            for (int k = 0; k < RPL; k++) {
-                const int row = lane + 64 * k;
-                if (row < nrow) xo[row] = xr[k];
-            }
+                for (int k = 0; k < RPL; k++) xo[k] = xr[k];
         }
     };
     // ---- x0 = eta, r = eta - A x0, ||eta||, ||r||
@@ -605,8 +668,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
             update(j, cont);
             clk(4);
             const int nc = __builtin_popcountll(cont);
-            count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * sR * (j + (hw ? 2 : 1)) +
-                      nc * n * sR * (j + 4),
+            count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * (sQ * j + sR * (hw ? 2 : 1)) +
+                      nc * n * (sQ * (j + 1) + sR * 3),
                   1);
             if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(5);
@@ -637,10 +700,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R> a) {
     }
 }
 
-template <typename R, int DOF>
-void launch_coarsest(const CArgs<R>& a, int G, size_t sh, hipStream_t s) {
-    void* args[] = {const_cast<CArgs<R>*>(&a)};
-    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, DOF>), dim3(G), dim3(kCT), args,
+template <typename R, typename S, int DOF>
+void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
+    void* args[] = {const_cast<CArgs<R, S>*>(&a)};
+    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF>), dim3(G), dim3(kCT), args,
                                       (unsigned)sh, s));
 }
 
@@ -661,8 +724,10 @@ size_t coarsest_lds(int m) { return sizeof(cplx<double>) * ((size_t)(7 * m + 5) 
 }  // namespace
 
 template <typename R>
-CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap)
-    : s_(s), Nx_(Nx), Nt_(Nt), dof_(dof), m_(m), restarts_(restarts), cap_(cap), m_max_(m), tol_(tol) {
+CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, int restarts, double tol, int cap,
+                                bool half_basis)
+    : s_(s), Nx_(Nx), Nt_(Nt), dof_(dof), m_(m), restarts_(restarts), cap_(cap), m_max_(m), tol_(tol),
+      half_(half_basis && sizeof(R) == 4) {
     if (m < 1 || m > kMaxM || restarts < 1 || cap < 1 || cap > 64) return;
     const int V = Nx * Nt;
     const size_t sh = coarsest_lds(m);
@@ -678,7 +743,7 @@ CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, i
                 V, dof, cap, m, sh, maxg, G_, spw_);
     if (G_ > maxg || G_ < cap) return;
     const size_t n = (size_t)V * dof, vb = sizeof(cplx<R>) * n * cap;
-    Q_.alloc(vb * (m + 1));
+    Q_.alloc((half_ ? sizeof(chalf) * n * cap : vb) * (m + 1));
     u_.alloc(vb);
     w_.alloc(vb);
     part_.alloc(sizeof(cplx<double>) * (size_t)cap * (2 * m + 2) * G_);
@@ -699,11 +764,24 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     const int nsys = mask ? 64 - __builtin_clzll(mask) : 0;
     if (nsys == 0) return;
     if (nsys > cap_ || nsys > G_) throw BadShape("coarsest GMRES: batch larger than capacity");
-    CArgs<R> a;
+    if constexpr (sizeof(R) == 4) {
+        if (half_) {
+            run<chalf>(L, m0, eta, psi, mask, nsys, iters);
+            return;
+        }
+    }
+    run<cplx<R>>(L, m0, eta, psi, mask, nsys, iters);
+}
+
+template <typename R>
+template <typename S>
+void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int nsys,
+                           int* iters) {
+    CArgs<R, S> a;
     a.L = L;
     a.eta = eta;
     a.x = psi;
-    a.Q = Q_.as<cplx<R>>();
+    a.Q = Q_.as<S>();
     a.u = u_.as<cplx<R>>();
     a.w = w_.as<cplx<R>>();
     a.part = part_.as<cplx<double>>();
@@ -738,7 +816,7 @@ void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cp
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
 #define DD_CASE(D) \
-    case D: launch_coarsest<R, D>(a, G_, sh, s_); break;
+    case D: launch_coarsest<R, S, D>(a, G_, sh, s_); break;
         DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
 #undef DD_CASE
         default:

@@ -328,7 +328,7 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
     if (p.ortho_coarse >= 2) {
         const LevelGeom& Gc = g_[L - 1];
         coarse_dev_.reset(new CoarsestGmres<R>(s, Gc.Nx, Gc.Nt, Gc.dof, p.coarse_m, p.coarse_restarts, p.coarse_tol,
-                                               cap));
+                                               cap, p.coarse_basis == 1));
         if (!coarse_dev_->supported()) coarse_dev_.reset();
     }
     // launch-per-step engine: always for 0/1, as the alternative the setup times for 3
@@ -564,10 +564,14 @@ void Multigrid<R>::autotune() {
         DD_HIP(hipGetLastError());
         double t[2] = {0, 0};
         const int msteps = std::min(160, p_.coarse_m);
-        coarse_->set_limits(msteps, 1, 0.0);
-        coarse_dev_->set_limits(msteps, 1, 0.0);
         for (int eng = 0; eng < 2; eng++) {
             use_dev_ = eng == 1;
+            // a short untimed run first (loads the kernels' code), then the timed one
+            coarse_->set_limits(2, 1, 0.0);
+            coarse_dev_->set_limits(2, 1, 0.0);
+            coarsest(e, x, full_mask(nr));
+            coarse_->set_limits(msteps, 1, 0.0);
+            coarse_dev_->set_limits(msteps, 1, 0.0);
             DD_HIP(hipEventRecord(e0, s_));
             coarsest(e, x, full_mask(nr));
             DD_HIP(hipEventRecord(e1, s_));

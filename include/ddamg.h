@@ -70,6 +70,10 @@ typedef struct dd_params {
     int sap_variant;               /* coarse-SAP kernel variant for every coarse level
                                       (2*systems_per_workgroup + batched_link_loads;
                                       32 = matrix cores, FP32); -1: autotuned/default */
+    int coarse_basis;              /* storage of the persistent coarsest GMRES's Krylov
+                                      basis in the FP32 hierarchy: 0 = FP32, 1 = IEEE
+                                      binary16 (compressed basis, products in FP32;
+                                      default).  Ignored by the FP64 hierarchy.     */
 } dd_params;
 
 typedef struct dd_solve_info {
