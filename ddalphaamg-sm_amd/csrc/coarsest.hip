@@ -47,6 +47,7 @@ namespace {
 constexpr int kCT = 512;     // threads per work-group
 constexpr int kNW = kCT / 64;
 constexpr int kSpwMax = 8;   // sites per work-group at most
+constexpr int kGmax = 256;   // work-groups at most (one per compute unit)
 constexpr int kMaxM = 432;   // restart length at most (LDS state below 64 KB)
 constexpr int kBB = 32;      // block of the triangular solve
 
@@ -137,7 +138,8 @@ struct CArgs {
     S* Q;                   // basis [m+1][nsys][n] (storage type S)
     cplx<R>* u;             // [nsys][n]
     cplx<R>* w;             // [nsys][n]
-    cplx<double>* part;     // [G][nsys][2m+2] per-work-group partial sums
+    cplx<R>* part;          // [G][nsys][2m+2] per-work-group partial sums (working precision:
+                            // they are sums formed in it, the totals are taken in FP64)
     cplx<double>* Hrot;     // [nsys][m+1][m] rotated (triangular) Hessenberg columns
     cplx<R>* etals;         // [nsys][m] least-squares solution (working precision)
     cplx<R>* coef;          // [nsys][2m+2]: a_i (i<j) | h1_{i,j} (i<=j) | 1/beta
@@ -155,33 +157,34 @@ struct CArgs {
 
 __device__ __forceinline__ cplx<double> cdivr(cplx<double> a, double d) { return mk<double>(a.x / d, a.y / d); }
 
-// Sums NV values (a power of two, 4 <= NV <= 32) over the 64 lanes of a wave with NV
-// lane exchanges (each step hands half of the values to the partner lane, through the
-// DPP / permlane networks): afterwards the lanes with f = lane >> (6 - log2 NV) hold the
-// total of value f.  The stages are instantiated per offset, so every array index is a
-// constant and the values stay in registers.
+// Sums NV values (a power of two, 4 <= NV <= 32) over each group of LPS lanes (32 or 64)
+// with NV lane exchanges (each step hands half of the values to the partner lane,
+// through the DPP / permlane networks): afterwards lane li = lane mod LPS holds the
+// group's total of value li >> (log2 LPS - log2 NV).  The stages are instantiated per
+// offset, so every array index is a constant and the values stay in registers.
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
-template <typename R, int NV, int ST>
+template <typename R, int NV, int LPS, int ST>
 __device__ __forceinline__ void bfly_stage(R (&F)[NV], int lane) {
-    constexpr int O = 32 >> ST, HALF = (NV / 2) >> ST;
+    constexpr int O = (LPS / 2) >> ST, HALF = (NV / 2) >> ST;
 #pragma unroll
     for (int k = 0; k < HALF; k++) {
         R A = F[k], B = F[k + HALF];
         pair_swap<O>(A, B, lane);
         F[k] = A + B;
     }
-    if constexpr (ST + 1 < ilog2c(NV)) bfly_stage<R, NV, ST + 1>(F, lane);
+    if constexpr (ST + 1 < ilog2c(NV)) bfly_stage<R, NV, LPS, ST + 1>(F, lane);
 }
 template <typename R, int O>
 __device__ __forceinline__ R bfly_tail(R t, int lane) {   // t + partners at offsets O, O/2, ..., 1
     if constexpr (O >= 1) return bfly_tail<R, O / 2>(t + lane_xor<O>(t, lane), lane);
     else return t;
 }
-template <typename R, int NV>
+template <typename R, int NV, int LPS>
 __device__ __forceinline__ R butterfly(R (&F)[NV], int lane) {
     static_assert(NV >= 4 && NV <= 32 && (NV & (NV - 1)) == 0, "NV must be a power of two in 4..32");
-    bfly_stage<R, NV, 0>(F, lane);
-    return bfly_tail<R, (32 >> ilog2c(NV))>(F[0], lane);
+    static_assert(LPS == 32 || LPS == 64, "lane groups of 32 or 64");
+    bfly_stage<R, NV, LPS, 0>(F, lane);
+    return bfly_tail<R, ((LPS / 2) >> ilog2c(NV))>(F[0], lane);
 }
 
 constexpr int pow2_floor(int v) {
@@ -199,15 +202,16 @@ struct Chunk {
     static constexpr int UPD = sizeof(S) == 16 ? 4 : (RED / 2 < 4 ? 4 : RED / 2);
 };
 
-template <typename R, typename S, int DOF>
+// Lanes and rows: a system's rows of the slice (spw sites of DOF rows) are spread over a
+// group of LPS lanes, RPL = DOF / 8 consecutive rows per lane, so LPS = 32 holds up to 4
+// sites and LPS = 64 up to 8; with LPS = 32 a wave works on two systems at once.
+template <typename R, typename S, int DOF, int LPS>
 __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
-    // RPL consecutive rows of the slice per lane; RPL divides DOF, so a lane's rows lie
-    // all inside the slice or all outside
-    constexpr int RPL = (kSpwMax * DOF + 63) / 64;
-    static_assert(DOF % RPL == 0, "rows per lane must divide the site's rows");
+    constexpr int RPL = DOF / 8, SPI = 64 / LPS;
+    static_assert(DOF % 8 == 0, "rows per lane must divide the site's rows");
     // kCH basis vectors loaded per pass step, reduced in groups of kGV (NVAL sums each)
     constexpr int kCH = Chunk<S, RPL>::RED, kCU = Chunk<S, RPL>::UPD, kGV = kCH < 8 ? kCH : 8, NVAL = 4 * kGV,
-                  LGV = kGV == 8 ? 5 : 4;
+                  SH = ilog2c(LPS) - ilog2c(NVAL);
     static_assert(kCH % kGV == 0 && (kGV == 8 || kGV == 4), "chunk of 4, 8 or a multiple of 8 vectors");
     unsigned bar_gen = 0;
     extern __shared__ unsigned char smem_raw[];
@@ -226,17 +230,22 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     cplx<double>* red = blk;   // [kNW][64] wave sums of the totals (phase B; blk is used after it)
     static_assert(kBB * kBB >= kNW * 64, "LDS scratch of the totals");
 
-    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // wv is wave-uniform (read from the first lane, so that per-wave loops and addresses
+    // stay scalar)
+    const int G = gridDim.x, wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63,
+              wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int V = a.Nx * a.Nt;
     const size_t n = (size_t)V * DOF, vs = n * (size_t)a.nsys;
     const int s0 = wg * a.spw, s1 = min(V, s0 + a.spw);
-    const int nrow = max(0, s1 - s0) * DOF;
+    const int nrow = max(0, s1 - s0) * DOF;   // at most LPS * RPL
     const size_t row0 = (size_t)s0 * DOF;
     const int ntile = (a.nsys + 15) / 16;
     const double sR = sizeof(cplx<R>), sQ = sizeof(S);
-    // the lane's rows lb..lb+RPL-1 (lanes past the slice read its last rows, masked)
-    const bool lrow = lane * RPL < nrow;
-    const int lb = lrow ? lane * RPL : nrow - RPL;
+    // the lane's system within a wave's pair (h) and its rows lb..lb+RPL-1 (lanes past the
+    // slice read its last rows, masked)
+    const int h = lane / LPS, li = lane % LPS;
+    const bool lrow = li * RPL < nrow;
+    const int lb = lrow ? li * RPL : nrow - RPL;
     const int sys = wg;   // the system whose scalars this work-group owns (G >= nsys)
     const bool owner = sys < a.nsys && ((a.mask >> sys) & 1ull);
     // partial sums: work-group-major [G][nsys][VALS], each work-group writes its own
@@ -280,8 +289,22 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         return *smask;
     };
     auto set_flags = [&](int q, int fl) { a.ctrl[q].flags = fl; };
-    // sites of the slice, wave by wave; tiles of 16 systems in pairs (one link load for both)
+    // sites of the slice, wave by wave; tiles of 16 systems in pairs (one link load for
+    // both).  Slices of at most 4 sites give a wave one (site, tile) each.
     auto stencil = [&](const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mk2, bool res) {
+        if (LPS == 32) {
+            const int ns = s1 - s0;
+            for (int u = wv; u < ns * ntile; u += kNW) {
+                const int site = s0 + u % ns, tl = u / ns;
+                if (!((mk2 >> (16 * tl)) & 0xffffull)) continue;
+                if (res)
+                    coarse_site<R, DOF, true, 1>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
+                else
+                    coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                                  lane);
+            }
+            return;
+        }
         for (int site = s0 + wv; site < s1; site += kNW)
             for (int tl = 0; tl < ntile; tl += 2) {
                 if (!((mk2 >> (16 * tl)) & 0xffffffffull)) continue;
@@ -307,12 +330,17 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             r[k] = lrow ? v : czero<R>();
         }
     };
-    auto put1 = [&](int q, int v, cplx<R> p) {   // single work-group partial
-        p = wave_sum(p, 64);
-        if (lane == 0) a.part[pidx(q, v, wg)] = mk<double>((double)p.x, (double)p.y);
+    auto put1 = [&](int q, int v, cplx<R> p, bool act) {   // single work-group partial (lane group)
+        p = wave_sum(p, LPS);
+        if (li == 0 && act) a.part[pidx(q, v, wg)] = p;
     };
+    // the wave's pair of systems qb*SPI + h; loads use a valid system index
+    auto sys_of = [&](int qb) { return min(qb * SPI + h, a.nsys - 1); };
+    auto pair_mask = [&](uint64_t mk2, int qb) -> unsigned { return (unsigned)(mk2 >> (qb * SPI)) & ((1u << SPI) - 1); };
     // ---- phase A reduction: a_i, c and (hw) b_i, d of system q over the own rows
-    auto reduce_step = [&](int q, int j, bool hw) {
+    auto reduce_step = [&](int qb, int j, bool hw, unsigned pm) {
+        const int q = sys_of(qb);
+        const bool act = (pm >> h) & 1u;
         cplx<R> ur[RPL], wr[RPL];
         load_rows(ur, a.u + (size_t)q * n + row0);
         if (hw) load_rows(wr, a.w + (size_t)q * n + row0);
@@ -325,9 +353,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             c.x += norm2(ur[k]);
             d = d + cmul(ur[k], wr[k]);
         }
-        put1(q, 2 * m, c);
-        if (hw) put1(q, 2 * m + 1, d);
-        double* pd = reinterpret_cast<double*>(a.part);
+        put1(q, 2 * m, c, act);
+        if (hw) put1(q, 2 * m + 1, d, act);
+        R* pd = reinterpret_cast<R*>(a.part);
         for (int i0 = 0; i0 < j; i0 += kCH) {
             // unconditional loads at clamped indices (no branches between the loads), the
             // terms outside the basis or the own rows masked afterwards
@@ -358,31 +386,33 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                     F[4 * gi + 2] = pb.x;
                     F[4 * gi + 3] = pb.y;
                 }
-                const R tsum = butterfly<R, NVAL>(F, lane);
-                const int f = lane >> (6 - LGV), ii = g0 + (f >> 2), comp = f & 3;
-                if ((lane & ((1 << (6 - LGV)) - 1)) == 0 && i0 + ii < j && (hw || comp < 2)) {
+                const R tsum = butterfly<R, NVAL, LPS>(F, lane);
+                const int f = li >> SH, ii = g0 + (f >> 2), comp = f & 3;
+                if (act && (li & ((1 << SH) - 1)) == 0 && i0 + ii < j && (hw || comp < 2)) {
                     const int v = (comp < 2 ? 0 : m) + i0 + ii;
-                    pd[pidx(q, v, wg) * 2 + (comp & 1)] = (double)tsum;
+                    pd[pidx(q, v, wg) * 2 + (comp & 1)] = tsum;
                 }
             }
         }
     };
     auto reduce_norm = [&](uint64_t mk2, int v, const cplx<R>* vec) {
-        for (int q = wv; q < a.nsys; q += kNW) {
-            if (!((mk2 >> q) & 1ull)) continue;
+        for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
+            const unsigned pm = pair_mask(mk2, qb);
+            if (!pm) continue;
+            const int q = sys_of(qb);
             cplx<R> r[RPL];
             load_rows(r, vec + (size_t)q * n + row0);
             cplx<R> c = czero<R>();
 #pragma unroll
             for (int k = 0; k < RPL; k++) c.x += norm2(r[k]);
-            put1(q, v, c);
+            put1(q, v, c, (pm >> h) & 1u);
         }
     };
     // total of value v of system q over the work-groups (whole work-group; every thread)
     auto total1 = [&](int q, int v) -> cplx<double> {
         if (wv == 0) {
             cplx<double> t = czero<double>();
-            for (int k = lane; k < G; k += 64) t = t + a.part[pidx(q, v, k)];
+            for (int k = lane; k < G; k += 64) t = t + cast<double>(a.part[pidx(q, v, k)]);
             t = wave_sum(t, 64);
             if (lane == 0) tot[v] = t;
         }
@@ -396,8 +426,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         const int q = sys;
         const bool hw = j < m;
         const int nv = hw ? 2 * j + 2 : j + 1;
-        {   // totals: a lane per value, wave w adds the work-groups w, w + 8, ...; the
-            // waves' sums are added in LDS in a fixed order
+        {   // totals: a lane per value, wave w adds the work-groups w, w + 8, ... (all
+            // their partials loaded before the first addition); the waves' sums are added
+            // in LDS in a fixed order
+            constexpr int KW = kGmax / kNW, KB = sizeof(R) == 4 ? KW / 2 : KW / 4;
             for (int base = 0; base < nv; base += 64) {
                 const int idx = base + lane;
                 int v = 0;
@@ -405,10 +437,16 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 else if (hw && idx < 2 * j) v = m + (idx - j);
                 else v = 2 * m + min(idx - (hw ? 2 * j : j), 1);
                 cplx<double> t = czero<double>();
-                if (idx < nv) {
-#pragma unroll 8
-                    for (int k = wv; k < G; k += kNW) t = t + a.part[pidx(q, v, k)];
+#pragma unroll
+                for (int u0 = 0; u0 < KW; u0 += KB) {
+                    cplx<R> pv[KB];
+#pragma unroll
+                    for (int u = 0; u < KB; u++) pv[u] = a.part[pidx(q, v, min(wv + kNW * (u0 + u), G - 1))];
+#pragma unroll
+                    for (int u = 0; u < KB; u++)
+                        if (wv + kNW * (u0 + u) < G) t = t + cast<double>(pv[u]);
                 }
+                if (idx >= nv) t = czero<double>();
                 red[wv * 64 + lane] = t;
                 __syncthreads();
                 if (wv == 0 && idx < nv) {
@@ -497,8 +535,11 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     // ---- phase C: q_j and u_{j+1} on the own rows (wave per system); the update
     // coefficients are read per chunk, next to the basis vectors they multiply
     auto update = [&](int j, uint64_t mk2) {
-        for (int q = wv; q < a.nsys; q += kNW) {
-            if (!((mk2 >> q) & 1ull)) continue;
+        for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
+            const unsigned pm = pair_mask(mk2, qb);
+            if (!pm) continue;
+            const int q = sys_of(qb);
+            const bool act = (pm >> h) & 1u;
             const cplx<R>* cf = a.coef + (size_t)q * CST;
             const R ib = cf[2 * m + 1].x;
             cplx<R> qj[RPL], un[RPL];
@@ -538,7 +579,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             const cplx<R> cj = cf[m + j];
             S* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0 + lb;
             cplx<R>* uo = a.u + (size_t)q * n + row0 + lb;
-            if (lrow) {
+            if (lrow && act) {
 #pragma unroll
                 for (int k = 0; k < RPL; k++) {
                     const cplx<R> qs = narrow(qo + k, ib * qj[k]);
@@ -586,9 +627,17 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     };
     // ---- x += sum_{i < maxit} eta_i q_i on the own rows (wave per system)
     auto xupdate = [&](uint64_t mk2) {
-        for (int q = wv; q < a.nsys; q += kNW) {
-            if (!((mk2 >> q) & 1ull)) continue;
-            const int nI = a.ctrl[q].maxit;
+        for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
+            const unsigned pm = pair_mask(mk2, qb);
+            if (!pm) continue;
+            const int q = sys_of(qb);
+            const bool act = (pm >> h) & 1u;
+            // the pair's longer basis bounds the loop; the other system's terms past its
+            // own length are masked
+            const int nIo = a.ctrl[q].maxit;
+            int nI = nIo;
+#pragma unroll
+            for (int o = LPS; o < 64; o *= 2) nI = max(nI, __shfl_xor(nI, o, 64));
             const cplx<R>* et = a.etals + (size_t)q * m;
             cplx<R> xr[RPL];
             load_rows(xr, a.x + (size_t)q * n + row0);
@@ -597,7 +646,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 cplx<R> e[kCU];
 #pragma unroll
                 for (int ii = 0; ii < kCU; ii++) {
-                    const int ic = min(i0 + ii, nI - 1);
+                    const int ic = max(0, min(i0 + ii, nIo - 1));
                     e[ii] = et[ic];
                     const S* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
@@ -605,13 +654,13 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 }
 #pragma unroll
                 for (int ii = 0; ii < kCU; ii++) {
-                    const cplx<R> ee = i0 + ii < nI ? e[ii] : czero<R>();
+                    const cplx<R> ee = i0 + ii < nIo ? e[ii] : czero<R>();
 #pragma unroll
                     for (int k = 0; k < RPL; k++) xr[k] = xr[k] + ee * widen(qv[ii][k]);
                 }
             }
             cplx<R>* xo = a.x + (size_t)q * n + row0 + lb;
-            if (lrow)
+            if (lrow && act)
 #pragma unroll
                 for (int k = 0; k < RPL; k++) xo[k] = xr[k];
         }
@@ -655,8 +704,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (hw) stencil(a.u, a.w, nullptr, run, false);
             __syncthreads();
             clk(7);
-            for (int q = wv; q < a.nsys; q += kNW)
-                if ((run >> q) & 1ull) reduce_step(q, j, hw);
+            for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
+                const unsigned pm = pair_mask(run, qb);
+                if (pm) reduce_step(qb, j, hw, pm);
+            }
             clk(0);
             if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(1);
@@ -700,10 +751,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     }
 }
 
-template <typename R, typename S, int DOF>
+template <typename R, typename S, int DOF, int LPS>
 void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
     void* args[] = {const_cast<CArgs<R, S>*>(&a)};
-    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF>), dim3(G), dim3(kCT), args,
+    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS>), dim3(G), dim3(kCT), args,
                                       (unsigned)sh, s));
 }
 
@@ -734,14 +785,16 @@ CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, i
     if (sh > 64 * 1024) return;
     const int maxg = coresident_workgroups();
     if (dof != 8 && dof != 16 && dof != 24 && dof != 32) return;
-    // slices of spw sites: at least max(cap, 128) work-groups (one per system for the
-    // scalar state, enough to stream the basis), at most kSpwMax sites each
-    spw_ = std::max(1, std::min(kSpwMax, V / std::max(cap, 128)));
+    // slices of spw sites (a power of two, at most kSpwMax): the smallest that fits the
+    // lattice into the co-resident grid, so that every compute unit streams the basis;
+    // one work-group per system for the scalar state (G >= cap)
+    spw_ = 1;
+    while (spw_ < kSpwMax && (V + spw_ - 1) / spw_ > std::max(maxg, 1)) spw_ *= 2;
     G_ = (V + spw_ - 1) / spw_;
     if (std::getenv("DDAMG_VERBOSE"))
         fprintf(stderr, "coarsest persistent kernel: V %d dof %d cap %d m %d lds %zu co-resident %d -> %d x %d sites\n",
                 V, dof, cap, m, sh, maxg, G_, spw_);
-    if (G_ > maxg || G_ < cap) return;
+    if (G_ > maxg || G_ > kGmax || G_ < cap) return;
     const size_t n = (size_t)V * dof, vb = sizeof(cplx<R>) * n * cap;
     Q_.alloc((half_ ? sizeof(chalf) * n * cap : vb) * (m + 1));
     u_.alloc(vb);
@@ -784,7 +837,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     a.Q = Q_.as<S>();
     a.u = u_.as<cplx<R>>();
     a.w = w_.as<cplx<R>>();
-    a.part = part_.as<cplx<double>>();
+    a.part = part_.as<cplx<R>>();
     a.Hrot = Hrot_.as<cplx<double>>();
     a.etals = etals_.as<cplx<R>>();
     a.coef = coef_.as<cplx<R>>();
@@ -815,8 +868,11 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     DD_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s_));
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
-#define DD_CASE(D) \
-    case D: launch_coarsest<R, S, D>(a, G_, sh, s_); break;
+#define DD_CASE(D)                                                   \
+    case D:                                                          \
+        if (spw_ <= 4) launch_coarsest<R, S, D, 32>(a, G_, sh, s_);  \
+        else launch_coarsest<R, S, D, 64>(a, G_, sh, s_);            \
+        break;
         DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
 #undef DD_CASE
         default:
