@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,47 +52,43 @@ constexpr int kGmax = 256;   // work-groups at most (one per compute unit)
 constexpr int kMaxM = 432;   // restart length at most (LDS state below 64 KB)
 constexpr int kBB = 32;      // block of the triangular solve
 
-enum : int { F_LIVE = 1, F_RUN = 2 };
+// F_CONV: the least-squares residual met the tolerance (set by the owner after the
+// step's coefficients are published; the system stops at the next step)
+enum : int { F_LIVE = 1, F_RUN = 2, F_CONV = 4 };
 
 struct Ctrl {
     int flags, maxit, iters, pad;
 };
 
-// Grid barrier of the persistent kernel.  Work-groups arrive on the counter of their
-// group (wg mod kBarGroups), the last of a group on the counter of the groups, and the
-// last group advances the generation that every work-group waits for.  Thread 0 of each
-// work-group carries the device-scope release (before arriving) and acquire (after the
-// generation moved) for the whole work-group, between two work-group barriers.  Waits
-// are bounded: a wait that exceeds kBarSpins polls (seconds) raises `fail`, which every
-// waiting work-group also watches, and the kernel ends; the host reports it.  All fields
-// are zero when a launch starts.
-constexpr int kBarGroups = 8;
+// Grid barrier of the persistent kernel.  Every work-group arrives on one counter; the
+// last to arrive resets it and advances the generation that every work-group waits for.
+// (A flat counter measured faster on 256 work-groups than per-group counters: the
+// arrivals are device-scope atomics of the memory side, and one level fewer of them
+// shortens the chain.)  Thread 0 of each work-group carries the device-scope release
+// (before arriving) and acquire (after the generation moved) for the whole work-group,
+// between two work-group barriers.  Waits are bounded: a wait that exceeds kBarSpins
+// polls (seconds) raises `fail`, which every waiting work-group also watches, and the
+// kernel ends; the host reports it.  All fields are zero when a launch starts.
 constexpr unsigned kBarSpins = 1u << 25;
 
 struct GridBar {
-    unsigned cnt[kBarGroups + 1];
+    unsigned cnt;
+    unsigned pad0[15];
     unsigned gen;
     unsigned fail;
-    unsigned pad[5];
+    unsigned pad1[14];
 };
 
 // returns false when the kernel must end (a barrier timed out somewhere)
 __device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, int G, int* sflag) {
+    (void)wg;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int grp = wg % kBarGroups;
-        const unsigned gsz = (unsigned)((G - grp + kBarGroups - 1) / kBarGroups);
-        const unsigned ngrp = (unsigned)min(G, kBarGroups);
         int ok = 1;
-        const unsigned old = __hip_atomic_fetch_add(&b->cnt[grp], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == gsz - 1) {
-            __hip_atomic_store(&b->cnt[grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned o2 =
-                __hip_atomic_fetch_add(&b->cnt[kBarGroups], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (o2 == ngrp - 1) {
-                __hip_atomic_store(&b->cnt[kBarGroups], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+        const unsigned old = __hip_atomic_fetch_add(&b->cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == (unsigned)G - 1) {
+            __hip_atomic_store(&b->cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         unsigned spins = 0;
         while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
@@ -100,7 +97,7 @@ __device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, 
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         gen++;
@@ -273,15 +270,20 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     };
 
     // systems whose control word has `flag`, read by the 64 lanes of wave 0 at once and
-    // broadcast through LDS.  Control words are written only between grid barriers and
-    // read after them (the barrier orders and publishes them), so plain loads and stores
-    // suffice and the result is the same in every work-group.
-    auto sysmask = [&](int flag) -> uint64_t {
+    // broadcast through LDS.  Control words are written between grid barriers and read
+    // after them (the barrier orders and publishes them), so plain loads and stores
+    // suffice and the result is the same in every work-group.  The one write that
+    // overlaps a read, F_CONV set by an owner in the update phase, changes no bit that
+    // the update phase's own read (F_RUN) looks at.
+    auto sysmask = [&](int flag, int off = 0) -> uint64_t {
         __syncthreads();
         if (wv == 0) {
             const int q = lane;
             bool on = false;
-            if (q < a.nsys && ((a.mask >> q) & 1ull)) on = (a.ctrl[q].flags & flag) != 0;
+            if (q < a.nsys && ((a.mask >> q) & 1ull)) {
+                const int fl = a.ctrl[q].flags;
+                on = (fl & flag) != 0 && (fl & off) == 0;
+            }
             const uint64_t mk2 = __ballot(on);
             if (lane == 0) *smask = mk2;
         }
@@ -356,19 +358,21 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         put1(q, 2 * m, c, act);
         if (hw) put1(q, 2 * m + 1, d, act);
         R* pd = reinterpret_cast<R*>(a.part);
-        for (int i0 = 0; i0 < j; i0 += kCH) {
-            // unconditional loads at clamped indices (no branches between the loads), the
-            // terms outside the basis or the own rows masked afterwards
-            S qv[kCH][RPL];
+        // chunks of NC vectors starting at i0: unconditional loads at clamped indices (no
+        // branches between the loads), the terms outside the basis or the own rows masked
+        // afterwards.  Full chunks of kCH, the tail in groups of kGV.
+        auto chunk = [&](auto nc_, int i0) {
+            constexpr int NC = decltype(nc_)::value;
+            S qv[NC][RPL];
 #pragma unroll
-            for (int ii = 0; ii < kCH; ii++) {
+            for (int ii = 0; ii < NC; ii++) {
                 const S* qi = a.Q + (size_t)min(i0 + ii, j - 1) * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
                 for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
             }
             // groups of kGV vectors: 4 kGV <= 32 sums per butterfly
 #pragma unroll
-            for (int g0 = 0; g0 < kCH; g0 += kGV) {
+            for (int g0 = 0; g0 < NC; g0 += kGV) {
                 R F[NVAL];
 #pragma unroll
                 for (int gi = 0; gi < kGV; gi++) {
@@ -393,7 +397,13 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                     pd[pidx(q, v, wg) * 2 + (comp & 1)] = tsum;
                 }
             }
-        }
+        };
+        int i0 = 0;
+        for (; i0 + kCH <= j; i0 += kCH) chunk(std::integral_constant<int, kCH>{}, i0);
+        if (kCH > kGV)
+            for (; i0 < j; i0 += kGV) chunk(std::integral_constant<int, kGV>{}, i0);
+        else if (i0 < j)
+            chunk(std::integral_constant<int, kCH>{}, i0);
     };
     auto reduce_norm = [&](uint64_t mk2, int v, const cplx<R>* vec) {
         for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
@@ -477,60 +487,61 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         }
         __syncthreads();
         const double beta = sc[0];
-        if (j >= 1)   // column j-1 of the Arnoldi Hessenberg matrix
-            for (int i = tid; i <= j; i += kCT) hcol[i] = i < j ? h1p[i] + tot[i] : mk<double>(beta, 0.0);
-        __syncthreads();
+        // the last step of the cycle and a breakdown stop now (no update follows); the
+        // Givens rotation of column j-1 and the convergence test are left to givens(),
+        // after the coefficients are published
+        const bool stop_now = j == m || beta == 0.0;
         if (tid == 0) {
-            int stop = 0, maxit = a.ctrl[q].maxit;
-            if (j == 0) {
-                gls[0] = mk<double>(beta, 0.0);
-            } else {
-                // Givens rotations of column c = j-1 (src/fgmres.cpp:99-115)
-                const int c = j - 1;
-                cplx<double> carry = hcol[0];
-                for (int i = 0; i < c; i++) {
-                    const cplx<double> h1 = hcol[i + 1], cn = csl[i], sn = snl[i];
-                    hcol[i] = cmul(cn, carry) + cmul(sn, h1);
-                    carry = cn * h1 - sn * carry;
-                }
-                const cplx<double> hs = hcol[j];
-                const double den = sqrt(norm2(carry) + norm2(hs));
-                const cplx<double> sn = den > 0 ? cdivr(hs, den) : czero<double>();
-                const cplx<double> cn = den > 0 ? cdivr(carry, den) : mk<double>(1.0, 0.0);
-                snl[c] = sn;
-                csl[c] = cn;
-                hcol[c] = cmul(cn, carry) + cmul(sn, hs);
-                gls[c + 1] = -(sn * gls[c]);
-                gls[c] = cmul(cn, gls[c]);
-                if (sqrt(norm2(gls[c + 1])) < a.tol * a.nrm[2 * q]) {
-                    maxit = j;
-                    stop = 1;
-                }
-            }
-            if (j == m || beta == 0.0) stop = 1;
-            if (stop) {
-                a.ctrl[q].maxit = maxit;
+            if (j == 0) gls[0] = mk<double>(beta, 0.0);
+            if (stop_now) {
+                a.ctrl[q].maxit = j;
                 set_flags(q, a.ctrl[q].flags & ~F_RUN);
             }
-            si[0] = stop;
         }
-        __syncthreads();
-        cplx<double>* Ho = a.Hrot + (size_t)q * (m + 1) * m;
-        if (j >= 1)
-            for (int i = tid; i < j; i += kCT) Ho[(size_t)i * m + (j - 1)] = hcol[i];
-        if (!si[0]) {   // first-pass coefficients of column j and the update coefficients
-            const double ib = 1.0 / beta;
-            cplx<R>* cf = a.coef + (size_t)q * CST;
-            for (int i = tid; i <= j; i += kCT) {
+        // column j-1 of the Arnoldi Hessenberg matrix; then (same thread, same i) the
+        // first-pass coefficients of column j and the update coefficients
+        const double ib = 1.0 / beta;
+        cplx<R>* cf = a.coef + (size_t)q * CST;
+        for (int i = tid; i <= j; i += kCT) {
+            if (j >= 1) hcol[i] = i < j ? h1p[i] + tot[i] : mk<double>(beta, 0.0);
+            if (!stop_now) {
                 const cplx<double> h1 =
                     i < j ? ib * tot[m + i] : (ib * ib) * (tot[2 * m + 1] - mk<double>(sc[1], sc[2]));
                 h1p[i] = h1;
                 cf[m + i] = cast<R>(h1);
                 if (i < j) cf[i] = cast<R>(tot[i]);
             }
-            if (tid == 0) cf[2 * m + 1] = mk<R>((R)ib, R(0));
         }
+        if (tid == 0 && !stop_now) cf[2 * m + 1] = mk<R>((R)ib, R(0));
+        if (tid == 0) si[1] = stop_now;
         __syncthreads();
+    };
+    // ---- Givens rotation of column c = j-1 of the owned system (src/fgmres.cpp:99-115),
+    // its rotated entries to Hrot and the break test |g_j| < tol ||eta||, by one thread
+    // while the work-group's other waves already run the update pass
+    auto givens = [&](int j) {
+        const int q = sys, c = j - 1;
+        cplx<double> carry = hcol[0];
+        for (int i = 0; i < c; i++) {
+            const cplx<double> h1 = hcol[i + 1], cn = csl[i], sn = snl[i];
+            hcol[i] = cmul(cn, carry) + cmul(sn, h1);
+            carry = cn * h1 - sn * carry;
+        }
+        const cplx<double> hs = hcol[j];
+        const double den = sqrt(norm2(carry) + norm2(hs));
+        const cplx<double> sn = den > 0 ? cdivr(hs, den) : czero<double>();
+        const cplx<double> cn = den > 0 ? cdivr(carry, den) : mk<double>(1.0, 0.0);
+        snl[c] = sn;
+        csl[c] = cn;
+        hcol[c] = cmul(cn, carry) + cmul(sn, hs);
+        gls[c + 1] = -(sn * gls[c]);
+        gls[c] = cmul(cn, gls[c]);
+        cplx<double>* Ho = a.Hrot + (size_t)q * (m + 1) * m;
+        for (int i = 0; i < j; i++) Ho[(size_t)i * m + c] = hcol[i];
+        if (!si[1] && sqrt(norm2(gls[c + 1])) < a.tol * a.nrm[2 * q]) {
+            a.ctrl[q].maxit = j;
+            set_flags(q, a.ctrl[q].flags | F_CONV);
+        }
     };
     // ---- phase C: q_j and u_{j+1} on the own rows (wave per system); the update
     // coefficients are read per chunk, next to the basis vectors they multiply
@@ -549,14 +560,15 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             for (int k = 0; k < RPL; k++) un[k] = ib * un[k];
             // newest vectors first: the reduction pass just read the basis oldest-first, so
             // its most recent vectors are still in the last-level cache
-            const int nch = (j + kCU - 1) / kCU;
-            for (int ch = nch - 1; ch >= 0; ch--) {
-                const int i0 = ch * kCU;
-                S qv[kCU][RPL];
-                cplx<R> ai[kCU], ci[kCU];
+            // chunks [i0, i0 + NC) from the newest vectors down: full chunks of kCU, the
+            // oldest remainder in groups of 4 (indices below 0 masked)
+            auto chunk = [&](auto nc_, int i0) {
+                constexpr int NC = decltype(nc_)::value;
+                S qv[NC][RPL];
+                cplx<R> ai[NC], ci[NC];
 #pragma unroll
-                for (int ii = 0; ii < kCU; ii++) {
-                    const int ic = min(i0 + ii, j - 1);
+                for (int ii = 0; ii < NC; ii++) {
+                    const int ic = max(0, min(i0 + ii, j - 1));
                     ai[ii] = cf[ic];
                     ci[ii] = cf[m + ic];
                     const S* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0 + lb;
@@ -564,8 +576,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                     for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
                 }
 #pragma unroll
-                for (int ii = 0; ii < kCU; ii++) {
-                    const bool in = i0 + ii < j;
+                for (int ii = 0; ii < NC; ii++) {
+                    const bool in = i0 + ii >= 0 && i0 + ii < j;
                     const cplx<R> aa = in ? ai[ii] : czero<R>(), cc = in ? ci[ii] : czero<R>();
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
@@ -574,7 +586,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                         un[k] = un[k] - cc * qq;
                     }
                 }
-            }
+            };
+            int i1 = j;
+            for (; i1 >= kCU; i1 -= kCU) chunk(std::integral_constant<int, kCU>{}, i1 - kCU);
+            for (; i1 > 0; i1 -= 4) chunk(std::integral_constant<int, 4>{}, i1 - 4);
             // u_{j+1} is orthogonalised against the stored q_j
             const cplx<R> cj = cf[m + j];
             S* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0 + lb;
@@ -697,7 +712,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (!live) break;
         const int nl = __builtin_popcountll(live);
         for (int j = 0; j <= m; j++) {
-            const uint64_t run = sysmask(F_RUN);
+            const uint64_t run = sysmask(F_RUN, F_CONV);
             if (!run) break;
             const int nr = __builtin_popcountll(run);
             const bool hw = j < m;
@@ -716,6 +731,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(3);
             const uint64_t cont = sysmask(F_RUN) & run;
+            if (tid == 0 && j >= 1 && owner && ((run >> sys) & 1ull)) givens(j);
             update(j, cont);
             clk(4);
             const int nc = __builtin_popcountll(cont);

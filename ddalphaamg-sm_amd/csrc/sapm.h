@@ -16,7 +16,10 @@
 //     slot per wave (double-buffered), summed in a fixed order so every lane of a
 //     system sees the same bits;
 //   * the Givens/back-substitution scalars of system j live in LDS and are updated by
-//     lane j of wave 0; per-system flags are uniform 16-bit masks (ballots).
+//     lane j of wave 0; per-system flags are uniform 16-bit masks (ballots);
+//   * the work-groups of one block's system tiles are numbered 8 apart (1-D grid), so
+//     the round-robin dispatch puts them on the same XCD at the same time and the second
+//     tile reads the block's links from that XCD's L2.
 #pragma once
 #include "mfma.h"
 
@@ -38,7 +41,7 @@ template <int DOF, int SPW>
 __global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L, const cplx<float>* __restrict__ rr,
                                               cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
                                               int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
-                                              const int* __restrict__ gate, int nrhs) {
+                                              const int* __restrict__ gate, int nrhs, int ntile) {
     using R = float;
     using acc_t = Mfma<R>::acc_t;
     using Vec = SapmVec<DOF, SPW>;
@@ -60,9 +63,13 @@ __global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L,
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, j = lane & 15, g = lane >> 4;
     const bool lead = (wv == 0) && (g == 0);   // lane j of wave 0: scalars of system j
     const int nbx = Nx / bx, nbt = Nt / bt;
+    // work-group id -> (block, system tile): 8 consecutive blocks per group of 8 ntile ids
+    const int gid = blockIdx.x, grp = gid / (8 * ntile), rr8 = gid % (8 * ntile);
+    const int blk = grp * 8 + (rr8 & 7), tile = rr8 >> 3;
+    if (blk >= (nbx * nbt) / 2) return;
     int BX, BT;
-    colour_block(blockIdx.x, colour, nbx, nbt, BX, BT);
-    const int r0sys = blockIdx.y * 16;
+    colour_block(blk, colour, nbx, nbt, BX, BT);
+    const int r0sys = tile * 16;
     unsigned onm = 0;
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -334,8 +341,8 @@ static void launch_sapm(hipStream_t s, const cplx<float>* L, const cplx<float>* 
     if (S > 16) throw BadShape("matrix-core coarse SAP: at most 16 sites per block");
     const size_t sh = sapm_lds_bytes<DOF>(S);
     if (sh > 160 * 1024) throw BadShape("matrix-core coarse SAP: block does not fit the LDS");
-    const int nblk = (Nx / bx) * (Nt / bt) / 2;
-    dim3 grid(nblk, (nrhs + 15) / 16);
+    const int nblk = (Nx / bx) * (Nt / bt) / 2, ntile = (nrhs + 15) / 16;
+    const dim3 grid(((nblk + 7) / 8) * 8 * ntile);
     if (S > 8) {   // two sites per wave
         static bool attr = false;
         if (!attr) {
@@ -344,7 +351,7 @@ static void launch_sapm(hipStream_t s, const cplx<float>* L, const cplx<float>* 
             attr = true;
         }
         k_sapm<DOF, 2><<<grid, 64 * ((S + 1) / 2), sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts,
-                                                           (float)tol, mask, ld, gate, nrhs);
+                                                           (float)tol, mask, ld, gate, nrhs, ntile);
     } else {
         static bool attr = false;
         if (!attr) {
@@ -353,7 +360,7 @@ static void launch_sapm(hipStream_t s, const cplx<float>* L, const cplx<float>* 
             attr = true;
         }
         k_sapm<DOF, 1><<<grid, 64 * S, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
-                                                mask, ld, gate, nrhs);
+                                                mask, ld, gate, nrhs, ntile);
     }
 }
 
