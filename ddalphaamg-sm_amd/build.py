@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'lib', 'libddamg.so')
-SOURCES = (['kernels.hip', 'coarsest.hip', 'sap_f32.hip', 'sap_f64.hip', 'solver.hip', 'capi.hip'] +
+SOURCES = (['kernels.hip', 'coarsest.hip', 'sap_f32.hip', 'sap_f64.hip', 'solver.hip', 'methods.hip', 'capi.hip'] +
            [f'sap_{p}_d{d}.hip' for p in ('f32', 'f64') for d in (8, 16, 24, 32)])
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-Xclang', '-target-feature', '-Xclang',

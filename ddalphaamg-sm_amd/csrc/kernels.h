@@ -106,6 +106,11 @@ void scale_inv(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t l
 template <typename R>
 void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, double alpha, size_t n, size_t ld,
           int nrhs, uint64_t mask);
+// out = c[3r] x + c[3r+1] y + c[3r+2] z per system r (c on the device; null operands skipped;
+// out may alias an operand)
+template <typename R>
+void lincomb(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, const cplx<R>* z,
+             const cplx<double>* c, size_t n, size_t ld, int nrhs, uint64_t mask);
 // x += sum_{j < nit[r]} eta[r*m + j] * Z_j   (Z_j = Z + j*zstride)
 template <typename R>
 void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const cplx<double>* eta,

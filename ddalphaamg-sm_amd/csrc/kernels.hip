@@ -616,6 +616,34 @@ void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, doubl
     DD_LAUNCH_CHECK();
 }
 
+// out = c0[r] x + c1[r] y + c2[r] z per system r (coefficients c[3r..3r+2] in FP64 on
+// the device; a null operand contributes nothing)
+template <typename R>
+__global__ __launch_bounds__(256) void k_lincomb(cplx<R>* out, const cplx<R>* x, const cplx<R>* y, const cplx<R>* z,
+                                                 const cplx<double>* __restrict__ c, size_t n, size_t ld,
+                                                 uint64_t mask) {
+    const int r = blockIdx.y;
+    if (!rhs_on(mask, r)) return;
+    const cplx<double> a = c[3 * r], b = c[3 * r + 1], d = c[3 * r + 2];
+    const cplx<R> ca = mk<R>((R)a.x, (R)a.y), cb = mk<R>((R)b.x, (R)b.y), cd = mk<R>((R)d.x, (R)d.y);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t o = r * ld + i;
+        cplx<R> v = czero<R>();
+        if (x) v = v + ca * x[o];
+        if (y) v = v + cb * y[o];
+        if (z) v = v + cd * z[o];
+        out[o] = v;
+    }
+}
+
+template <typename R>
+void lincomb(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, const cplx<R>* z,
+             const cplx<double>* c, size_t n, size_t ld, int nrhs, uint64_t mask) {
+    dim3 grid(ew_blocks(n), nrhs);
+    k_lincomb<R><<<grid, 256, 0, s>>>(out, x, y, z, c, n, ld, mask);
+    DD_LAUNCH_CHECK();
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_update_x(cplx<R>* __restrict__ x, const cplx<R>* __restrict__ Z,
                                                   size_t zstride, const cplx<double>* __restrict__ eta,
@@ -954,6 +982,8 @@ void dtheta_contract(hipStream_t s, const cplx<double>* U, const cplx<double>* y
                                const cplx<double>*, int);                                                           \
     template void axpy<R>(hipStream_t, cplx<R>*, const cplx<R>*, const cplx<R>*, double, size_t, size_t, int,       \
                           uint64_t);                                                                                \
+    template void lincomb<R>(hipStream_t, cplx<R>*, const cplx<R>*, const cplx<R>*, const cplx<R>*,                 \
+                             const cplx<double>*, size_t, size_t, int, uint64_t);                                   \
     template void update_x<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, const cplx<double>*, const int*, int,  \
                               size_t, size_t, int, uint64_t);                                                       \
     template void copy<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, size_t, int, uint64_t);            \

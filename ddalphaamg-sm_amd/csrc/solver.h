@@ -152,6 +152,22 @@ class Solver {
     int solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info,
               bool use_x0 = false);
     void apply_D(const cplx<double>* in, cplx<double>* out, int nrhs, double m0, bool dagger, int precision);
+    // Comparison solvers of the reference on the level-0 FP64 operator (methods.hip), each
+    // batched over nrhs independent systems in lock-step; device pointers, iters[r] per
+    // system in the reference's count convention, DD_ERR_NOT_CONVERGED if any system
+    // did not reach its tolerance.
+    //   solve_cg:        conjugate_gradient on D^dagger D x = D^dagger b, x0 = D^dagger b
+    //                    (src/conjugate_gradient.cpp:4-84 with Methods::CG src/methods.cpp:43-59)
+    //   solve_bicgstab:  bi_cgstab on D x = b from x0 (src/conjugate_gradient.cpp:87-176)
+    //   solve_gmres:     FGMRES with the identity (FGMRES_fine_level, include/fgmres.h:153-234)
+    //                    or, with sap_prec, one level-0 SAP iteration of a zero initial guess
+    //                    as the flexible preconditioner (FGMRES_SAP, include/sap.h:236-333)
+    int solve_cg(int nrhs, const cplx<double>* b, cplx<double>* x, double m0, int max_iter, double tol, int* iters,
+                 dd_solve_info* info);
+    int solve_bicgstab(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0,
+                       int max_iter, double tol, int* iters, dd_solve_info* info);
+    int solve_gmres(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0, int m,
+                    int restarts, double tol, bool sap_prec, int* iters, dd_solve_info* info);
     int Nx, Nt, device;
     hipStream_t s = nullptr;
     DevMem U64, U32, theta;

@@ -87,6 +87,11 @@ _SIGS = {
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
     'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
     'dd_coarsest_stats': (C.c_int, [_ctx, _dp]),
+    'dd_solve_cg': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, _ip, C.POINTER(SolveInfo)]),
+    'dd_solve_bicgstab': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, _ip,
+                                    C.POINTER(SolveInfo)]),
+    'dd_solve_gmres': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 _ip, C.POINTER(SolveInfo)]),
     'dd_solve_many_dev': (C.c_int, [C.c_int, C.POINTER(_ctx), _ip, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                     C.POINTER(SolveInfo)]),
     'dd_last_error': (C.c_char_p, []),
@@ -233,6 +238,39 @@ class Context:
         _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
         its = np.array(list(its))
         return (x if batch else x[0]), its, info
+
+    def solve_method(self, method: str, b: np.ndarray, m0: float, x0: np.ndarray | None = None,
+                     tol: float = 1e-10, max_iter: int = 10000, m: int = 20, restarts: int = 50,
+                     check: bool = True):
+        """The reference's comparison solvers (src/methods.cpp) on the FP64 Wilson-Dirac
+        operator: method 'cg' (normal equations, from D^dagger b; x0 ignored), 'bicgstab',
+        'gmres' (no preconditioner) or 'fgmres_sap' (one level-0 SAP iteration per step;
+        needs setup() with the same m0).  b: [nrhs] + lattice shape or lattice shape.
+        Returns (x, iterations per system, info)."""
+        b = _c128(b)
+        batch = b.ndim == 4
+        bb = b if batch else b[None]
+        nr = bb.shape[0]
+        x = np.zeros_like(bb)
+        x0p = None
+        if x0 is not None:
+            x0c = _c128(x0)
+            x0c = x0c if x0c.ndim == 4 else x0c[None]
+            x0p = _cptr(x0c)
+        its = (C.c_int * nr)()
+        info = SolveInfo()
+        if method == 'cg':
+            code = lib.dd_solve_cg(self._h, nr, _cptr(bb), _cptr(x), m0, max_iter, tol, its, C.byref(info))
+        elif method == 'bicgstab':
+            code = lib.dd_solve_bicgstab(self._h, nr, _cptr(bb), x0p, _cptr(x), m0, max_iter, tol, its,
+                                         C.byref(info))
+        elif method in ('gmres', 'fgmres_sap'):
+            code = lib.dd_solve_gmres(self._h, nr, _cptr(bb), x0p, _cptr(x), m0, m, restarts, tol,
+                                      int(method == 'fgmres_sap'), its, C.byref(info))
+        else:
+            raise ValueError(f'unknown method {method!r}')
+        _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
+        return (x if batch else x[0]), np.array(list(its)), info
 
     def solve_dev(self, b_ptr: int, x_ptr: int, nrhs: int, adjoint: bool = False, check: bool = True):
         """Solve with device pointers (complex128 interleaved, nrhs systems back to back),

@@ -135,6 +135,32 @@ int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* 
  * fgmres(phi, x0, x) signature, src/fgmres.cpp:5-10). */
 int dd_solve_x0(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 
+/* Comparison solvers of the reference on the FP64 Wilson-Dirac operator D(m0) of the
+ * context's gauge field, batched over nrhs <= 64 independent systems (each follows the
+ * reference's recurrence and stopping test).  info->relres is the final
+ * ||b - D x|| / ||b||.  Returns DD_ERR_NOT_CONVERGED if any system missed its tolerance.
+ *
+ * Conjugate gradient on the normal equations D^dagger D x = D^dagger b, started from
+ * D^dagger b, stopping when the normal-equation residual is below tol times
+ * ||D^dagger b||.  Replaces Methods::CG (src/methods.cpp:43-59) with
+ * conjugate_gradient (src/conjugate_gradient.cpp:4-84). */
+int dd_solve_cg(dd_ctx* ctx, int nrhs, const double* b, double* x, double m0, int max_iter, double tol, int* iters,
+                dd_solve_info* info);
+/* BiCGStab on D x = b from x0 (nullable: zero).  Replaces bi_cgstab
+ * (src/conjugate_gradient.cpp:87-176) as Methods::BiCG calls it (src/methods.cpp:3-19);
+ * a system that does not converge reports max_iter. */
+int dd_solve_bicgstab(dd_ctx* ctx, int nrhs, const double* b, const double* x0, double* x, double m0, int max_iter,
+                      double tol, int* iters, dd_solve_info* info);
+/* GMRES(m) with `restarts` cycles on D x = b from x0 (nullable: zero), count k*m + maxIt.
+ * sap_preconditioner = 0: no preconditioner, replaces FGMRES_fine_level
+ * (include/fgmres.h:153-234, Methods::GMRES src/methods.cpp:21-41).
+ * sap_preconditioner = 1: flexible GMRES preconditioned by one level-0 SAP iteration
+ * from a zero guess with the setup's level-0 SAP blocks, replaces FGMRES_SAP
+ * (include/sap.h:236-333, Methods::FGMRES_sap src/methods.cpp:83-104); needs dd_setup,
+ * with m0 equal to the setup's. */
+int dd_solve_gmres(dd_ctx* ctx, int nrhs, const double* b, const double* x0, double* x, double m0, int m,
+                   int restarts, double tol, int sap_preconditioner, int* iters, dd_solve_info* info);
+
 /* Adjoint solve D^dagger y = eta on the same hierarchy through sigma3-hermiticity:
  * y = sigma3 D^-1 sigma3 eta.  The reference has D_dagger_phi only
  * (src/dirac_operator.cpp:56-85); no adjoint solver. */

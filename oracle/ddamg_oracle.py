@@ -25,6 +25,9 @@ Pinning (what checks each function, see tests/test_oracle.py):
                          iteration count for the whole FGMRES-AMG solve.
   setup_phase RNG        golden test vectors (std::mt19937 per virtual rank).
   gauge_metropolis       <cos theta_P> against I1(beta)/I0(beta).
+  cg_normal, bicgstab    golden iteration counts and solutions of the compiled
+                         reference (g32_methods), dense solve, and CG's finite
+                         termination in at most n steps on a 4x4 lattice.
 """
 from __future__ import annotations
 
@@ -297,6 +300,86 @@ def _dot(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 
 def _bc(s: np.ndarray, like: np.ndarray) -> np.ndarray:
     return s.reshape((-1,) + (1,) * (like.ndim - 1))
+
+
+def cg_normal(U: np.ndarray, b: np.ndarray, m0: float, max_iter: int, tol: float):
+    """Methods::CG (src/methods.cpp:43-59): conjugate_gradient
+    (src/conjugate_gradient.cpp:4-84) on D^dagger D x = D^dagger b for a batch b[nb].
+
+    x0 = D^dagger b (`sol = phi`); r = phi - D^dagger D x; alpha = (r,r)/(d,Ad);
+    stop when ||r|| < tol ||phi|| with count k+1; beta = (r',r')/(r,r); d = beta d + r.
+    Returns (x, iterations) with iterations 0 where the reference reports a failure.
+    """
+    DdD = lambda v: np.stack([D_dagger_phi(U, D_phi(U, vi, m0), m0) for vi in v])
+    phi = np.stack([D_dagger_phi(U, bi, m0) for bi in b])
+    x = phi.copy()
+    r = phi - DdD(x)
+    d = r.copy()
+    rn2 = _dot(r, r)
+    nphi = np.sqrt(np.real(_dot(phi, phi)))
+    its = np.zeros(b.shape[0], int)
+    run = np.ones(b.shape[0], bool)
+    for k in range(max_iter):
+        Ad = DdD(d)
+        alpha = np.where(run, rn2 / np.where(run, _dot(d, Ad), 1), 0)
+        x = x + _bc(alpha, x) * d
+        r = r - _bc(alpha, r) * Ad
+        e2 = np.real(_dot(r, r))
+        conv = run & (np.sqrt(e2) < tol * nphi)
+        its[conv] = k + 1
+        run &= ~conv
+        if not run.any():
+            break
+        beta = np.where(run, e2 / np.where(run, rn2, 1), 0)
+        d = np.where(_bc(run, d), _bc(beta, d) * d + r, d)
+        rn2 = np.where(run, e2, rn2)
+    return x, its
+
+
+def bicgstab(U: np.ndarray, b: np.ndarray, x0: np.ndarray, m0: float, max_iter: int, tol: float):
+    """bi_cgstab (src/conjugate_gradient.cpp:87-176) on D x = b for a batch b[nb].
+
+    r~ = r_0 = b - D x0; rho = (r, r~); d = r (k = 0) or r + beta (d - omega Ad) with
+    beta = alpha rho / (omega rho_prev); alpha = rho / (Ad, r~); s = r - alpha Ad; stop
+    when ||s|| < tol ||b|| after x += alpha d (count k+1); omega = (s,t)/(t,t), t = D s;
+    r = s - omega t; x += alpha d + omega s.  Count max_iter on failure.
+    """
+    Dm = lambda v: np.stack([D_phi(U, vi, m0) for vi in v])
+    nb = b.shape[0]
+    x = x0.copy()
+    r = b - Dm(x)
+    rt = r.copy()
+    nphi = np.sqrt(np.real(_dot(b, b)))
+    its = np.full(nb, max_iter)
+    run = np.ones(nb, bool)
+    alpha = np.zeros(nb, complex)
+    omega = np.zeros(nb, complex)
+    rho_prev = np.zeros(nb, complex)
+    d = np.zeros_like(r)
+    Ad = np.zeros_like(r)
+    for k in range(max_iter):
+        rho = _dot(r, rt)
+        if k == 0:
+            d = r.copy()
+        else:
+            beta = np.where(run, alpha * rho / np.where(run, omega * rho_prev, 1), 0)
+            d = np.where(_bc(run, d), r + _bc(beta, d) * (d - _bc(omega, d) * Ad), d)
+        Ad = np.where(_bc(run, d), Dm(d), Ad)
+        alpha = np.where(run, rho / np.where(run, _dot(Ad, rt), 1), alpha)
+        sv = r - _bc(alpha, r) * Ad
+        conv = run & (np.sqrt(np.real(_dot(sv, sv))) < tol * nphi)
+        x = np.where(_bc(conv, x), x + _bc(alpha, x) * d, x)
+        its[conv] = k + 1
+        run &= ~conv
+        if not run.any():
+            break
+        t = Dm(sv)
+        om = _dot(sv, t) / np.where(run, _dot(t, t), 1)
+        omega = np.where(run, om, omega)
+        r = np.where(_bc(run, r), sv - _bc(omega, sv) * t, r)
+        x = np.where(_bc(run, x), x + _bc(alpha, x) * d + _bc(omega, x) * sv, x)
+        rho_prev = np.where(run, rho, rho_prev)
+    return x, its
 
 
 @dataclass

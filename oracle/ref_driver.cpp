@@ -13,6 +13,9 @@
 //   SAP_C::SAP                      src/sap.cpp:81-165
 //   AlgebraicMG::setUpPhase         src/amg.cpp:4-62
 //   FGMRES_AMG_{k,v}_cycle          include/amg.h:240-270, src/fgmres.cpp:5-97
+//   conjugate_gradient, bi_cgstab   src/conjugate_gradient.cpp:4-176 (as Methods::CG/BiCG
+//                                   call them, src/methods.cpp:3-59)
+//   FGMRES_fine_level, FGMRES_SAP   include/fgmres.h:153-234, include/sap.h:236-333
 //
 // Access to the reference classes' private members (the level list of
 // AlgebraicMG) is needed to dump the hierarchy; the access specifiers are
@@ -37,6 +40,7 @@
 #define private public
 #define protected public
 #include "amg.h"
+#include "conjugate_gradient.h"
 #include "params.h"
 #include "io.h"
 #undef private
@@ -200,6 +204,36 @@ int main(int argc, char** argv) {
         char buf[512];
         snprintf(buf, sizeof(buf), "iterations %d\nrelres %.17g\nsetup_s %.6f\ntotal_s %.6f\nsolve_s %.6f\nranks %d\n",
                  its, rel, t_setup, t_total, t_total - t_setup, mpi::size);
+        write_scalar_file(out, buf);
+    } else if (mode == "methods") {
+        // The comparison solvers of src/methods.cpp on the same system, each from x0 = 0
+        // (CG from D^dagger b, as conjugate_gradient starts).
+        CG::max_iter = BiCG::max_iter = atoi(opt("max_iter", "10000").c_str());
+        CG::tol = BiCG::tol = tol;
+        const int gm = atoi(opt("gmres_m", "20").c_str()), gr = atoi(opt("gmres_restarts", "100").c_str());
+        spinor xc(mpi::maxSizeH), xb(mpi::maxSizeH), xg(mpi::maxSizeH), xs(mpi::maxSizeH), Ddag(mpi::maxSizeH);
+        D_dagger_phi(U, rhs, Ddag, m0);
+        const int it_cg = conjugate_gradient(U, Ddag, xc, m0, false);
+        const int it_bi = bi_cgstab(U, rhs, x0, xb, m0, false);
+        FGMRES_fine_level g(wx * wt, LV::dof, mpi::maxSizeH, 1, 1, wx, wt, gm, gr, tol, U, m0);
+        const int it_gm = g.fgmres(rhs, x0, xg, false);
+        FGMRES_SAP fs(wx * wt, LV::dof, mpi::maxSizeH, 1, 1, wx, wt, gm, gr, tol, U, m0);
+        const int it_fs = fs.fgmres(rhs, x0, xs, false);
+        auto rel = [&](const spinor& x) {
+            spinor Dx(mpi::maxSizeH), r(mpi::maxSizeH);
+            D_phi(U, x, Dx, m0);
+            for (int i = 0; i < mpi::maxSizeH; i++) r.val[i] = rhs.val[i] - Dx.val[i];
+            return global_norm(r, wx, wt, 2) / global_norm(rhs, wx, wt, 2);
+        };
+        dump(out, "x_cg", xc.val, wx, wt, 2);
+        dump(out, "x_bicgstab", xb.val, wx, wt, 2);
+        dump(out, "x_gmres", xg.val, wx, wt, 2);
+        dump(out, "x_fgmres_sap", xs.val, wx, wt, 2);
+        char buf[512];
+        snprintf(buf, sizeof(buf),
+                 "cg_iterations %d\nbicgstab_iterations %d\ngmres_iterations %d\nfgmres_sap_iterations %d\n"
+                 "cg_relres %.17g\nbicgstab_relres %.17g\ngmres_relres %.17g\nfgmres_sap_relres %.17g\n",
+                 it_cg, it_bi, it_gm, it_fs, rel(xc), rel(xb), rel(xg), rel(xs));
         write_scalar_file(out, buf);
     } else {
         if (mpi::rank == 0) fprintf(stderr, "unknown mode %s\n", mode.c_str());

@@ -2,8 +2,8 @@
 
 Run in the development container (where the reference sources compile):
 
-    make -C oracle ref N="16 64"
-    python tests/golden/make_golden.py
+    make -C oracle ref N="16 32 64"
+    python tests/golden/make_golden.py [fixture ...]      # all fixtures by default
 
 Each fixture is an .npz of inputs and the reference's outputs (complex128), plus a
 JSON sidecar with the parameters and scalar results.  The reference itself never
@@ -17,6 +17,10 @@ Fixtures:
               level-2 links, transfer ops, D_1, SAP at levels 0 and 1
   g64_solve   cfg1 shape (64^2, m0=-0.1, 2 levels, Nv=8, nu2=1, FGMRES(20)): K- and
               V-cycle solutions, iteration counts and residuals
+  g32_methods the comparison solvers of src/methods.cpp on a 32^2 field (m0=-0.1, tol
+              1e-10, 2x2 ranks so that FGMRES_SAP's 4x4 blocks per rank are 4x4 sites):
+              CG on the normal equations, BiCGStab, GMRES(20) and FGMRES(20)+SAP --
+              iteration counts, final residuals and solutions
 """
 from __future__ import annotations
 
@@ -56,7 +60,27 @@ def level_meta(levels):
     return [dict(agg=list(s.agg), sap_block=list(s.sap_block), nv=s.nv) for s in levels]
 
 
-def main():
+def g32_methods():
+    N, m0 = 32, -0.1
+    theta, U = field(N, seed=3201, sweeps=500)
+    b = gaussian(N, 3202)
+    levels = [orc.LevelSpec((4, 4), (4, 4), 4)]
+    wd, _ = run_ref.run('methods', U, b, levels, m0, ranks=RANKS, gmres_m=20, gmres_restarts=200, max_iter=20000)
+    res = run_ref.result(wd)
+    meta = dict(N=N, m0=m0, beta=2.0, sweeps=500, seed=3201, ranks=list(RANKS), tol=1e-10, gmres_m=20,
+                gmres_restarts=200, max_iter=20000, sap_block=[4, 4], **res)
+    save('g32_methods', meta, theta=theta, rhs=b, x_cg=run_ref.load(wd, 'x_cg', RANKS),
+         x_bicgstab=run_ref.load(wd, 'x_bicgstab', RANKS), x_gmres=run_ref.load(wd, 'x_gmres', RANKS),
+         x_fgmres_sap=run_ref.load(wd, 'x_fgmres_sap', RANKS))
+    run_ref.cleanup(wd)
+
+
+def main(names=()):
+    if names:
+        for nm in names:
+            {'g32_methods': g32_methods}[nm]()
+        return
+    g32_methods()
     # ---- 16^2 operator fixtures -------------------------------------------------------
     N, m0 = 16, -0.1
     theta, U = field(N, seed=1601)
@@ -113,4 +137,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(sys.argv[1:])

@@ -1,0 +1,62 @@
+"""The reference's comparison solvers on the GPU (dd_solve_cg, dd_solve_bicgstab,
+dd_solve_gmres): CG on the normal equations, BiCGStab, GMRES(20) and FGMRES(20)+SAP,
+against the compiled reference's iteration counts and solutions (g32_methods fixture),
+and batched systems against single ones."""
+import numpy as np
+import pytest
+
+from oracle import ddamg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+ddamg = pytest.importorskip('ddamg')
+
+# (method, fixture key, allowed difference of the iteration count).  The GPU forms the
+# dot products in another order than the reference's MPI sums; the counts of the
+# long CG/BiCGStab recurrences may move by a step or two with that rounding.
+METHODS = [('cg', 'cg', 2), ('bicgstab', 'bicgstab', 2), ('gmres', 'gmres', 1), ('fgmres_sap', 'fgmres_sap', 1)]
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _ctx(meta, a):
+    ctx = ddamg.Context(meta['N'], meta['N'])
+    ctx.set_gauge(np.exp(1j * a['theta']))
+    # the level-0 SAP blocks of FGMRES+SAP: 4x4 blocks per rank of the 2x2 grid = 4x4 sites
+    ctx.setup(ddamg.params(2, agg=[(4, 4)], sap_block=[tuple(meta['sap_block'])], nv=[4], m0=meta['m0'],
+                           rng_ranks=tuple(meta['ranks']), max_rhs=3))
+    return ctx
+
+
+@pytest.mark.parametrize('method,key,slack', METHODS)
+def test_methods_golden(golden, method, key, slack):
+    meta, a = golden('g32_methods')
+    with _ctx(meta, a) as ctx:
+        x, its, info = ctx.solve_method(method, a['rhs'], meta['m0'], tol=meta['tol'], max_iter=meta['max_iter'],
+                                        m=meta['gmres_m'], restarts=meta['gmres_restarts'])
+    U = np.exp(1j * a['theta'])
+    rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, meta['m0'])) / np.linalg.norm(a['rhs'])
+    print(f'{method}: {its[0]} iterations (reference {meta[key + "_iterations"]}), relres {rr:.3e} '
+          f'(reference {meta[key + "_relres"]:.3e}), error vs reference x {rel(x, a["x_" + key]):.2e}')
+    assert abs(int(its[0]) - meta[key + '_iterations']) <= slack
+    assert abs(info.relres - rr) <= 1e-3 * rr + 1e-14
+    assert rr < 2 * meta[key + '_relres'] + 1e-12
+    assert rel(x, a['x_' + key]) < 1e-7
+
+
+@pytest.mark.parametrize('method', ['cg', 'bicgstab', 'fgmres_sap'])
+def test_methods_batched_equal_single(golden, method):
+    """Three systems solved together give each the iteration count and the solution it
+    gets alone (systems share no arithmetic)."""
+    meta, a = golden('g32_methods')
+    rng = np.random.default_rng(7)
+    B = np.stack([a['rhs']] + [rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape)
+                               for _ in range(2)])
+    kw = dict(tol=meta['tol'], max_iter=meta['max_iter'], m=meta['gmres_m'], restarts=meta['gmres_restarts'])
+    with _ctx(meta, a) as ctx:
+        xb, itb, _ = ctx.solve_method(method, B, meta['m0'], **kw)
+        for r in range(3):
+            x1, it1, _ = ctx.solve_method(method, B[r], meta['m0'], **kw)
+            assert int(it1[0]) == int(itb[r])
+            assert rel(xb[r], x1) < 1e-12

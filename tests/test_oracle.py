@@ -232,3 +232,40 @@ def test_gradient_formula_dense_fd():
         assert abs(fd - g[ix, it, mu]) < 1e-7 * max(1, abs(fd))
     fd_m = (L(th, m0 + h) - L(th, m0 - h)) / (2 * h)
     assert abs(fd_m - (-np.real(np.vdot(y, x)))) < 1e-7 * max(1, abs(fd_m))
+
+
+def test_comparison_solvers_golden(golden):
+    """CG on the normal equations, BiCGStab, GMRES(20) and FGMRES(20)+SAP: the oracle's
+    restatements reproduce the compiled reference's iteration counts and solutions on
+    the g32_methods fixture."""
+    meta, a = golden('g32_methods')
+    U = np.exp(1j * a['theta'])
+    b = a['rhs'][None]
+    m0, tol = meta['m0'], meta['tol']
+    x, its = orc.cg_normal(U, b, m0, meta['max_iter'], tol)
+    assert int(its[0]) == meta['cg_iterations'] and rel(x[0], a['x_cg']) < 1e-10
+    x, its = orc.bicgstab(U, b, np.zeros_like(b), m0, meta['max_iter'], tol)
+    assert int(its[0]) == meta['bicgstab_iterations'] and rel(x[0], a['x_bicgstab']) < 1e-10
+    func = lambda v: np.stack([orc.D_phi(U, vi, m0) for vi in v])
+    res = orc.fgmres(func, None, b, np.zeros_like(b), meta['gmres_m'], meta['gmres_restarts'], tol)
+    assert int(res.iterations[0]) == meta['gmres_iterations'] and rel(res.x[0], a['x_gmres']) < 1e-10
+    op = orc.dirac_stencil(U, m0)
+    blocks = orc.SapBlocks(meta['N'], meta['N'], *meta['sap_block'])
+    pre = lambda v, z: np.stack([orc.sap(op, blocks, vi, np.zeros_like(vi), 1, 1e-10)[0] for vi in v])
+    res = orc.fgmres(func, pre, b, np.zeros_like(b), meta['gmres_m'], meta['gmres_restarts'], tol)
+    assert int(res.iterations[0]) == meta['fgmres_sap_iterations'] and rel(res.x[0], a['x_fgmres_sap']) < 1e-10
+
+
+def test_cg_bicgstab_dense():
+    """Against a dense solve on 4x4: both reach the solution; CG on the (Hermitian
+    positive definite) normal equations stops within the dimension, 32, up to rounding."""
+    rng = np.random.default_rng(11)
+    N, m0 = 4, -0.3
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)))
+    b = (rng.standard_normal((2, N, N, 2)) + 1j * rng.standard_normal((2, N, N, 2)))
+    D = orc.dense_D(U, m0)
+    xd = np.stack([np.linalg.solve(D, bi.ravel()).reshape(N, N, 2) for bi in b])
+    x, its = orc.cg_normal(U, b, m0, 500, 1e-12)
+    assert rel(x, xd) < 1e-9 and (its > 0).all() and (its <= 48).all()
+    x, its = orc.bicgstab(U, b, np.zeros_like(b), m0, 500, 1e-12)
+    assert rel(x, xd) < 1e-9 and (its < 500).all()
