@@ -294,13 +294,13 @@ int dd_solve(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_sol
     });
 }
 
-int dd_solve_cg(dd_ctx* c, int nrhs, const double* b, double* x, double m0, int max_iter, double tol, int* iters,
+int dd_solve_cg(dd_ctx* c, int nrhs, const double* phi, double* x, double m0, int max_iter, double tol, int* iters,
                 dd_solve_info* info) {
-    if (check(c) || !b || !x || nrhs < 1 || nrhs > 64 || max_iter < 1 || !(tol > 0)) return DD_ERR_ARG;
+    if (check(c) || !phi || !x || nrhs < 1 || nrhs > 64 || max_iter < 1 || !(tol > 0)) return DD_ERR_ARG;
     return guard([&] {
         hipStream_t s = c->s->s;
         const size_t n = vsize(c) * nrhs;
-        Staged bb(b, n, s), xx(nullptr, n, s);
+        Staged bb(phi, n, s), xx(nullptr, n, s);
         const int st = c->s->solve_cg(nrhs, bb.p(), xx.p(), m0, max_iter, tol, iters, info);
         download(x, xx.p(), n, s);
         return st;
@@ -322,15 +322,16 @@ int dd_solve_bicgstab(dd_ctx* c, int nrhs, const double* b, const double* x0, do
 }
 
 int dd_solve_gmres(dd_ctx* c, int nrhs, const double* b, const double* x0, double* x, double m0, int m,
-                   int restarts, double tol, int sap_preconditioner, int* iters, dd_solve_info* info) {
-    if (check(c) || !b || !x || nrhs < 1 || nrhs > 64 || m < 1 || restarts < 1 || !(tol > 0)) return DD_ERR_ARG;
+                   int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info) {
+    if (check(c) || !b || !x || nrhs < 1 || nrhs > 64 || m < 1 || restarts < 1 || !(tol > 0) || sap_bx < 0 ||
+        (sap_bx > 0 && sap_bt < 1))
+        return DD_ERR_ARG;
     return guard([&] {
         hipStream_t s = c->s->s;
         const size_t n = vsize(c) * nrhs;
         Staged bb(b, n, s), xx(x0, n, s);
         if (!x0) DD_HIP(hipMemsetAsync(xx.p(), 0, sizeof(double) * 2 * n, s));
-        const int st =
-            c->s->solve_gmres(nrhs, bb.p(), xx.p(), xx.p(), m0, m, restarts, tol, sap_preconditioner != 0, iters, info);
+        const int st = c->s->solve_gmres(nrhs, bb.p(), xx.p(), xx.p(), m0, m, restarts, tol, sap_bx, sap_bt, iters, info);
         download(x, xx.p(), n, s);
         return st;
     });

@@ -90,27 +90,25 @@ void finish(int nrhs, const std::vector<int>& its, const std::vector<double>& re
 
 }  // namespace
 
-// conjugate_gradient (src/conjugate_gradient.cpp:4-84) on D^dagger D, right-hand side
-// D^dagger b as Methods::CG prepares it (src/methods.cpp:43-59).  sol = phi initially;
-// converged when ||r|| < tol ||phi||, returning k+1; the reference returns 0 on
-// max_iter, here the count is max_iter and the status DD_ERR_NOT_CONVERGED.  relres is
-// the final ||b - D x|| / ||b||.
-int Solver::solve_cg(int nrhs, const cplx<double>* b, cplx<double>* x, double m0, int max_iter, double tol,
+// conjugate_gradient (src/conjugate_gradient.cpp:4-84) on D^dagger D x = phi (Methods::CG,
+// src/methods.cpp:43-59, passes phi = D^dagger b).  sol = phi initially; converged when
+// ||r|| < tol ||phi||, returning k+1; the reference returns 0 on max_iter, here the count
+// is max_iter and the status DD_ERR_NOT_CONVERGED.  relres is ||phi - D^dagger D x|| / ||phi||.
+int Solver::solve_cg(int nrhs, const cplx<double>* phi, cplx<double>* x, double m0, int max_iter, double tol,
                      int* iters, dd_solve_info* info) {
     if (!have_gauge) throw BadShape("CG before the gauge field");
     if (nrhs < 1 || nrhs > 64) throw BadShape("CG: 1..64 systems");
     auto t0 = std::chrono::steady_clock::now();
     const size_t n = (size_t)Nx * Nt * 2;
     const uint64_t all = full_mask(nrhs);
-    Scratch w(n, nrhs, 5);
-    cplx<double> *phi = w.v(0), *r = w.v(1), *d = w.v(2), *Ad = w.v(3), *tmp = w.v(4);
+    Scratch w(n, nrhs, 4);
+    cplx<double> *r = w.v(0), *d = w.v(1), *Ad = w.v(2), *tmp = w.v(3);
     const cplx<double>* U = U64.as<cplx<double>>();
     auto DdD = [&](const cplx<double>* in, cplx<double>* out, uint64_t m) {
         wilson<double>(s, U, in, tmp, nullptr, Nx, Nt, m0, false, nrhs, m, n);
         wilson<double>(s, U, tmp, out, nullptr, Nx, Nt, m0, true, nrhs, m, n);
     };
-    wilson<double>(s, U, b, phi, nullptr, Nx, Nt, m0, true, nrhs, all, n);   // phi = D^dagger b
-    copy<double>(s, x, phi, n, n, n, nrhs, all);                              // sol = phi
+    if (x != phi) copy<double>(s, x, phi, n, n, n, nrhs, all);                // sol = phi
     DdD(x, Ad, all);
     const std::vector<C> one(nrhs, C(1, 0)), mone(nrhs, C(-1, 0)), zero(nrhs, C(0, 0));
     combine(s, w, r, phi, Ad, nullptr, one, mone, zero, all);                 // r = phi - D^+D x
@@ -124,6 +122,7 @@ int Solver::solve_cg(int nrhs, const cplx<double>* b, cplx<double>* x, double m0
     for (int q = 0; q < nrhs; q++) {
         rn2[q] = h[q][0];
         nphi[q] = std::sqrt(h[q][1].real());
+        err[q] = std::sqrt(h[q][0].real());
     }
     uint64_t run = all;
     for (int k = 0; k < max_iter && run; k++) {
@@ -157,13 +156,8 @@ int Solver::solve_cg(int nrhs, const cplx<double>* b, cplx<double>* x, double m0
         run = next;
         if (run) combine(s, w, d, r, d, nullptr, one, beta, zero, run);   // d = r + beta d
     }
-    // true residual of D x = b for the report
-    wilson<double>(s, U, x, tmp, b, Nx, Nt, m0, false, nrhs, all, n);
-    dots(s, w, tmp, tmp, 0, all);
-    dots(s, w, b, b, 1, all);
-    h = fetch(s, w);
     std::vector<double> rel(nrhs);
-    for (int q = 0; q < nrhs; q++) rel[q] = h[q][1].real() > 0 ? std::sqrt(h[q][0].real() / h[q][1].real()) : 0.0;
+    for (int q = 0; q < nrhs; q++) rel[q] = nphi[q] > 0 ? err[q] / nphi[q] : 0.0;
     finish(nrhs, its, rel, iters, info, t0);
     return run ? DD_ERR_NOT_CONVERGED : DD_OK;
 }
@@ -259,41 +253,42 @@ int Solver::solve_bicgstab(int nrhs, const cplx<double>* b, const cplx<double>* 
     return run ? DD_ERR_NOT_CONVERGED : DD_OK;
 }
 
-// FGMRES_fine_level (identity preconditioner: GMRES) or FGMRES_SAP (one level-0 SAP
-// iteration, tolerance sap_tol of the setup, from a zero guess) with the Krylov engine;
-// iteration count k*m + maxIt as FGMRES::fgmres.
+// FGMRES_fine_level (identity preconditioner: GMRES) or FGMRES_SAP, whose preconditioner
+// is SAP_fine_level::SAP(in, out = 0, nu = 1, tolerance 1e-10) (include/sap.h:274-284):
+// one red-black sweep over sap_bx x sap_bt site blocks, each block solved by GMRES(5) x 5
+// to 1e-3 from the block right-hand side (the SAPV defaults, src/variables.cpp:100-105).
+// Iteration count k*m + maxIt as FGMRES::fgmres.
 int Solver::solve_gmres(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0, int m,
-                        int restarts, double tol, bool sap_prec, int* iters, dd_solve_info* info) {
+                        int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info) {
     if (!have_gauge) throw BadShape("GMRES before the gauge field");
     if (nrhs < 1 || nrhs > 64) throw BadShape("GMRES: 1..64 systems");
-    if (sap_prec && !have_setup) throw BadShape("FGMRES+SAP needs dd_setup (its level-0 SAP blocks)");
-    if (sap_prec && nrhs > std::max(1, prm.max_rhs)) throw BadShape("FGMRES+SAP: nrhs exceeds max_rhs of the setup");
-    if (sap_prec && m0 != prm.m0) throw BadShape("FGMRES+SAP: m0 differs from the setup's");
+    const bool sap_prec = sap_bx > 0;
+    if (sap_prec) {
+        if (sap_bt < 1 || Nx % sap_bx || Nt % sap_bt) throw BadShape("FGMRES+SAP: block size must divide the lattice");
+        const int nbx = Nx / sap_bx, nbt = Nt / sap_bt, S = sap_bx * sap_bt;
+        if ((nbx * nbt) % 2 || ((nbt & 1) && (nbx & 1))) throw BadShape("FGMRES+SAP: SAP needs an even number of blocks");
+        if (S > 64 || (S & (S - 1))) throw BadShape("FGMRES+SAP: blocks of a power-of-two number of sites, at most 64");
+    }
     auto t0 = std::chrono::steady_clock::now();
     const size_t n = (size_t)Nx * Nt * 2;
     const uint64_t all = full_mask(nrhs);
-    Krylov<double> kry(s, n, m, restarts, tol, nrhs, sap_prec, prm.ortho_outer);
-    Krylov<double>::Func A = [this, n, m0](const cplx<double>* in, cplx<double>* out, uint64_t mk) {
-        wilson<double>(s, U64.as<cplx<double>>(), in, out, nullptr, Nx, Nt, m0, false, mask_count(mk), mk, n);
+    const cplx<double>* U = U64.as<cplx<double>>();
+    Krylov<double> kry(s, n, m, restarts, tol, nrhs, sap_prec, 0);
+    Krylov<double>::Func A = [this, n, m0, U](const cplx<double>* in, cplx<double>* out, uint64_t mk) {
+        wilson<double>(s, U, in, out, nullptr, Nx, Nt, m0, false, mask_count(mk), mk, n);
     };
-    DevMem a32, y32;
-    if (sap_prec && !mg64) {
-        a32.alloc(sizeof(cplx<float>) * n * nrhs);
-        y32.alloc(sizeof(cplx<float>) * n * nrhs);
-    }
-    // SAP_fine_level::SAP(in, out, nu = 1, sap_tolerance) after out = 0 (include/sap.h:274-284)
-    Krylov<double>::Prec M = [&](const cplx<double>* in, cplx<double>* io, uint64_t mk) {
+    DevMem rbuf;
+    if (sap_prec) rbuf.alloc(sizeof(cplx<double>) * n * nrhs);
+    // out = SAP(v, 0, nu = 1): the residual before each colour is v - D out; with one
+    // iteration the tolerance test of SAP_C::SAP (src/sap.cpp:150-156) stops nothing
+    Krylov<double>::Prec M = [&](const cplx<double>* v, cplx<double>* out, uint64_t mk) {
         const int nr = mask_count(mk);
-        if (mg64) {
-            zero<double>(s, io, n, n, nr, mk);
-            mg64->sap(0, in, io, 1, 1e-10, mk);
-        } else {
-            cplx<float>* av = a32.as<cplx<float>>();
-            cplx<float>* yv = y32.as<cplx<float>>();
-            convert<float, double>(s, av, in, n, n, n, nr, mk);
-            zero<float>(s, yv, n, n, nr, mk);
-            mg32->sap(0, av, yv, 1, 1e-10, mk);
-            convert<double, float>(s, io, yv, n, n, n, nr, mk);
+        cplx<double>* r = rbuf.as<cplx<double>>();
+        zero<double>(s, out, n, n, nr, mk);
+        copy<double>(s, r, v, n, n, n, nr, mk);
+        for (int colour = 0; colour < 2; colour++) {
+            sap_blocks0<double>(s, U, r, out, Nx, Nt, sap_bx, sap_bt, colour, m0, 5, 5, 1e-3, nr, mk, n, nullptr);
+            wilson<double>(s, U, out, r, v, Nx, Nt, m0, false, nr, mk, n);
         }
     };
     std::vector<int> its(64, 0);

@@ -156,18 +156,19 @@ class Solver {
     // batched over nrhs independent systems in lock-step; device pointers, iters[r] per
     // system in the reference's count convention, DD_ERR_NOT_CONVERGED if any system
     // did not reach its tolerance.
-    //   solve_cg:        conjugate_gradient on D^dagger D x = D^dagger b, x0 = D^dagger b
-    //                    (src/conjugate_gradient.cpp:4-84 with Methods::CG src/methods.cpp:43-59)
+    //   solve_cg:        conjugate_gradient on D^dagger D x = phi from x0 = phi
+    //                    (src/conjugate_gradient.cpp:4-84); relres of the normal equations
     //   solve_bicgstab:  bi_cgstab on D x = b from x0 (src/conjugate_gradient.cpp:87-176)
     //   solve_gmres:     FGMRES with the identity (FGMRES_fine_level, include/fgmres.h:153-234)
-    //                    or, with sap_prec, one level-0 SAP iteration of a zero initial guess
-    //                    as the flexible preconditioner (FGMRES_SAP, include/sap.h:236-333)
-    int solve_cg(int nrhs, const cplx<double>* b, cplx<double>* x, double m0, int max_iter, double tol, int* iters,
+    //                    or, with sap_bx > 0, one level-0 SAP iteration over sap_bx x sap_bt
+    //                    site blocks from a zero guess as the flexible preconditioner
+    //                    (FGMRES_SAP, include/sap.h:236-333)
+    int solve_cg(int nrhs, const cplx<double>* phi, cplx<double>* x, double m0, int max_iter, double tol, int* iters,
                  dd_solve_info* info);
     int solve_bicgstab(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0,
                        int max_iter, double tol, int* iters, dd_solve_info* info);
     int solve_gmres(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0, int m,
-                    int restarts, double tol, bool sap_prec, int* iters, dd_solve_info* info);
+                    int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info);
     int Nx, Nt, device;
     hipStream_t s = nullptr;
     DevMem U64, U32, theta;

@@ -91,7 +91,7 @@ _SIGS = {
     'dd_solve_bicgstab': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, _ip,
                                     C.POINTER(SolveInfo)]),
     'dd_solve_gmres': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
-                                 _ip, C.POINTER(SolveInfo)]),
+                                 C.c_int, _ip, C.POINTER(SolveInfo)]),
     'dd_solve_many_dev': (C.c_int, [C.c_int, C.POINTER(_ctx), _ip, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                     C.POINTER(SolveInfo)]),
     'dd_last_error': (C.c_char_p, []),
@@ -241,12 +241,13 @@ class Context:
 
     def solve_method(self, method: str, b: np.ndarray, m0: float, x0: np.ndarray | None = None,
                      tol: float = 1e-10, max_iter: int = 10000, m: int = 20, restarts: int = 50,
-                     check: bool = True):
+                     sap_block: tuple[int, int] = (4, 4), check: bool = True):
         """The reference's comparison solvers (src/methods.cpp) on the FP64 Wilson-Dirac
-        operator: method 'cg' (normal equations, from D^dagger b; x0 ignored), 'bicgstab',
-        'gmres' (no preconditioner) or 'fgmres_sap' (one level-0 SAP iteration per step;
-        needs setup() with the same m0).  b: [nrhs] + lattice shape or lattice shape.
-        Returns (x, iterations per system, info)."""
+        operator, for D x = b: method 'cg' (CG on D^dagger D x = D^dagger b from
+        D^dagger b, as Methods::CG; x0 ignored), 'bicgstab', 'gmres' (no preconditioner) or
+        'fgmres_sap' (one SAP iteration over sap_block-site blocks per step).
+        b: [nrhs] + lattice shape or lattice shape.  Returns (x, iterations per system,
+        info); info.relres is the residual of the system the method solves."""
         b = _c128(b)
         batch = b.ndim == 4
         bb = b if batch else b[None]
@@ -260,13 +261,15 @@ class Context:
         its = (C.c_int * nr)()
         info = SolveInfo()
         if method == 'cg':
-            code = lib.dd_solve_cg(self._h, nr, _cptr(bb), _cptr(x), m0, max_iter, tol, its, C.byref(info))
+            phi = self.apply_D(bb, m0, dagger=True)
+            code = lib.dd_solve_cg(self._h, nr, _cptr(phi), _cptr(x), m0, max_iter, tol, its, C.byref(info))
         elif method == 'bicgstab':
             code = lib.dd_solve_bicgstab(self._h, nr, _cptr(bb), x0p, _cptr(x), m0, max_iter, tol, its,
                                          C.byref(info))
         elif method in ('gmres', 'fgmres_sap'):
-            code = lib.dd_solve_gmres(self._h, nr, _cptr(bb), x0p, _cptr(x), m0, m, restarts, tol,
-                                      int(method == 'fgmres_sap'), its, C.byref(info))
+            sb = sap_block if method == 'fgmres_sap' else (0, 0)
+            code = lib.dd_solve_gmres(self._h, nr, _cptr(bb), x0p, _cptr(x), m0, m, restarts, tol, sb[0], sb[1],
+                                      its, C.byref(info))
         else:
             raise ValueError(f'unknown method {method!r}')
         _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))

@@ -1,14 +1,18 @@
-// Example driver with the reference's flow (src/main.cpp:10-105, src/methods.cpp:133-191):
+// Example driver with the reference's flow (src/main.cpp:10-105, src/methods.cpp:3-191):
 // reads the run parameters on stdin, the gauge field / right-hand side in the reference's
-// binary record format and the per-level parameters file, solves with FGMRES + V- and
-// K-cycle on the GPU and checks the solution with D_phi.
+// binary record format and the per-level parameters file; runs the comparison solvers
+// (CG on the normal equations as the reference's main does, then BiCGStab and
+// FGMRES + SAP), then FGMRES + V- and K-cycle, all on the GPU, and checks each solution
+// with D_phi.
 #include <chrono>
 #include <cstdio>
 #include <string>
 
 #include "amg.h"
+#include "conjugate_gradient.h"
 #include "io.h"
 #include "params.h"
+#include "sap.h"
 
 static void check_solution(const spinor& U, const spinor& rhs, const spinor& x_sol, double m0) {
     spinor x(mpi::maxSizeH);
@@ -48,6 +52,32 @@ int main() {
     read_binary(confFile, U);
     read_binary(rhsFile, rhs);
     const double tol = 1e-10;
+    {   // Methods::CG, Methods::BiCG, Methods::FGMRES_sap (src/methods.cpp:3-104)
+        std::cout << "--------Inverting the normal equations with CG----------" << std::endl;
+        spinor Ddag(mpi::maxSizeH), xcg(mpi::maxSizeH);
+        D_dagger_phi(U, rhs, Ddag, m0);
+        auto t0 = std::chrono::steady_clock::now();
+        const int itc = conjugate_gradient(U, Ddag, xcg, m0, true);
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("time elapsed CG: %.4fs (%d iterations).\n", el, itc);
+        check_solution(U, rhs, xcg, m0);
+        std::cout << "--------------Bi-CGstab inversion--------------" << std::endl;
+        spinor xb(mpi::maxSizeH);
+        t0 = std::chrono::steady_clock::now();
+        const int itb = bi_cgstab(U, rhs, x0, xb, m0, true);
+        el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("time elapsed Bi-CGstab: %.4fs (%d iterations).\n", el, itb);
+        check_solution(U, rhs, xb, m0);
+        std::cout << "--------------Flexible GMRES with SAP preconditioning--------------" << std::endl;
+        spinor xs(mpi::maxSizeH);
+        FGMRES_SAP fs(LV::Ntot, LV::dof, mpi::maxSizeH, 1, 1, mpi::width_x, mpi::width_t,
+                      FGMRESV::fgmres_restart_length, FGMRESV::fgmres_restarts, tol, U, m0);
+        t0 = std::chrono::steady_clock::now();
+        const int its = fs.fgmres(rhs, x0, xs, true);
+        el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("time elapsed FGMRES_SAP: %.4fs (%d iterations).\n", el, its);
+        check_solution(U, rhs, xs, m0);
+    }
     for (int cycle : {0, 1}) {
         spinor x(mpi::maxSizeH);
         auto t0 = std::chrono::steady_clock::now();

@@ -59,6 +59,16 @@ extern int fgmres_restart_length;
 extern int fgmres_restarts;
 }  // namespace FGMRESV
 
+// CG on the normal equations and BiCGStab (include/variables.h:116-126)
+namespace CG {
+extern int max_iter;
+extern double tol;
+}  // namespace CG
+namespace BiCG {
+extern int max_iter;
+extern double tol;
+}  // namespace BiCG
+
 namespace LevelV {
 extern int levels;
 extern int maxLevel;

@@ -7,15 +7,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <functional>
 #include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "../../../include/ddamg.h"
 #include "amg.h"
+#include "conjugate_gradient.h"
 #include "io.h"
 #include "params.h"
+#include "sap.h"
 
 double pi = 3.14159265359;
 c_double I_number(0, 1);
@@ -48,6 +52,14 @@ double fgmres_tolerance = 1e-10;
 int fgmres_restart_length = 20;
 int fgmres_restarts = 50;
 }  // namespace FGMRESV
+namespace CG {
+int max_iter = 10000;
+double tol = 1e-10;
+}  // namespace CG
+namespace BiCG {
+int max_iter = 10000;
+double tol = 1e-10;
+}  // namespace BiCG
 namespace LevelV {
 int levels = 2, maxLevel = 1;
 int* BlocksX = nullptr;
@@ -466,4 +478,97 @@ int FGMRES_AMG::fgmres(const spinor& phi, const spinor& x0, spinor& x, const boo
         }
     }
     return its;
+}
+
+// ---- comparison solvers (src/conjugate_gradient.cpp, include/fgmres.h:153-234,
+// include/sap.h:236-333) -------------------------------------------------------------------
+namespace {
+// ||v|| over the interior sites (the reference's sqrt(real(dot(v, v))))
+double interior_norm(const spinor& v) {
+    double s2 = 0;
+    for (int x = 1; x <= mpi::width_x; x++)
+        for (int t = 1; t <= mpi::width_t; t++)
+            for (int mu = 0; mu < LV::dof; mu++) s2 += std::norm(v.val[idx(x, t, mu)]);
+    return std::sqrt(s2);
+}
+
+// solver on the process-wide context with U installed; returns (status, iterations)
+std::pair<int, int> run_method(const spinor& U, const char* name, const std::function<int(dd_ctx*, int*, dd_solve_info*)>& f,
+                               dd_solve_info& info) {
+    dd_ctx* c = operator_ctx();
+    std::vector<double> u = pack(U, LV::Nx, LV::Nt, 2);
+    check(dd_set_gauge(c, u.data()), "dd_set_gauge");
+    int its = 0;
+    const int st = f(c, &its, &info);
+    if (st != DD_OK && st != DD_ERR_NOT_CONVERGED) check(st, name);
+    return {st, its};
+}
+}  // namespace
+
+int conjugate_gradient(const spinor& U, const spinor& phi, spinor& x, const double& m0, const bool print) {
+    std::vector<double> p = pack(phi, LV::Nx, LV::Nt, 2), xv(p.size());
+    dd_solve_info info{};
+    auto r = run_method(U, "dd_solve_cg", [&](dd_ctx* c, int* its, dd_solve_info* in) {
+        return dd_solve_cg(c, 1, p.data(), xv.data(), m0, CG::max_iter, CG::tol, its, in);
+    }, info);
+    unpack(xv, x, LV::Nx, LV::Nt, 2);
+    if (mpi::rank2d == 0 && (print || r.first != DD_OK)) {
+        if (r.first == DD_OK)
+            std::cout << "CG for D^+D converged in " << r.second << " iterations Error "
+                      << info.relres * interior_norm(phi) << std::endl;
+        else
+            std::cout << "CG for D^+D did not converge in " << CG::max_iter << " iterations" << std::endl;
+    }
+    return r.first == DD_OK ? r.second : 0;
+}
+
+int bi_cgstab(const spinor& U, const spinor& phi, const spinor& x0, spinor& x, const double& m0,
+              const bool& print_message) {
+    std::vector<double> p = pack(phi, LV::Nx, LV::Nt, 2), x0v = pack(x0, LV::Nx, LV::Nt, 2), xv(p.size());
+    dd_solve_info info{};
+    auto r = run_method(U, "dd_solve_bicgstab", [&](dd_ctx* c, int* its, dd_solve_info* in) {
+        return dd_solve_bicgstab(c, 1, p.data(), x0v.data(), xv.data(), m0, BiCG::max_iter, BiCG::tol, its, in);
+    }, info);
+    x = x0;
+    unpack(xv, x, LV::Nx, LV::Nt, 2);
+    if (print_message && mpi::rank2d == 0) {
+        if (r.first == DD_OK)
+            std::cout << "Bi-CG-stab for D converged in " << r.second << " iterations Error "
+                      << info.relres * interior_norm(phi) << std::endl;
+        else
+            std::cout << "Bi-CG-stab for D did not converge in " << BiCG::max_iter << " iterations" << std::endl;
+    }
+    return r.second;
+}
+
+namespace {
+int gmres_common(const spinor& U, double m0, const spinor& phi, const spinor& x0, spinor& x, int m, int restarts,
+                 double tol, int bx, int bt, bool print, const char* label) {
+    std::vector<double> p = pack(phi, LV::Nx, LV::Nt, 2), x0v = pack(x0, LV::Nx, LV::Nt, 2), xv(p.size());
+    dd_solve_info info{};
+    auto r = run_method(U, "dd_solve_gmres", [&](dd_ctx* c, int* its, dd_solve_info* in) {
+        return dd_solve_gmres(c, 1, p.data(), x0v.data(), xv.data(), m0, m, restarts, tol, bx, bt, its, in);
+    }, info);
+    x = x0;
+    unpack(xv, x, LV::Nx, LV::Nt, 2);
+    if (print && mpi::rank2d == 0) {
+        if (r.first == DD_OK)
+            std::cout << label << " converged in " << (r.second + m - 1) / m << " cycles Error " << info.relres
+                      << std::endl << "With " << r.second << " iterations" << std::endl;
+        else
+            std::cout << label << " did not converge in " << restarts << " restarts of lenght " << m << " Error "
+                      << info.relres << std::endl;
+    }
+    return r.second;
+}
+}  // namespace
+
+int FGMRES_fine_level::fgmres(const spinor& phi, const spinor& x0, spinor& x, const bool& print_message) {
+    return gmres_common(U, m0, phi, x0, x, m, restarts, tol, 0, 0, print_message, "FGMRES");
+}
+
+int FGMRES_SAP::fgmres(const spinor& phi, const spinor& x0, spinor& x, const bool& print_message) {
+    // SAPV::sap_block_x x sap_block_t blocks per (virtual) rank, as SAP_fine_level is built
+    const int bx = (LV::Nx / mpi::ranks_x) / SAPV::sap_block_x, bt = (LV::Nt / mpi::ranks_t) / SAPV::sap_block_t;
+    return gmres_common(U, m0, phi, x0, x, m, restarts, tol, bx, bt, print_message, "FGMRES");
 }

@@ -136,30 +136,33 @@ int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* 
 int dd_solve_x0(dd_ctx* ctx, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info);
 
 /* Comparison solvers of the reference on the FP64 Wilson-Dirac operator D(m0) of the
- * context's gauge field, batched over nrhs <= 64 independent systems (each follows the
- * reference's recurrence and stopping test).  info->relres is the final
- * ||b - D x|| / ||b||.  Returns DD_ERR_NOT_CONVERGED if any system missed its tolerance.
+ * context's gauge field (no dd_setup needed), batched over nrhs <= 64 independent
+ * systems (each follows the reference's recurrence and stopping test).  info->relres is
+ * the final relative residual of the system solved.  Returns DD_ERR_NOT_CONVERGED if any
+ * system missed its tolerance.
  *
- * Conjugate gradient on the normal equations D^dagger D x = D^dagger b, started from
- * D^dagger b, stopping when the normal-equation residual is below tol times
- * ||D^dagger b||.  Replaces Methods::CG (src/methods.cpp:43-59) with
- * conjugate_gradient (src/conjugate_gradient.cpp:4-84). */
-int dd_solve_cg(dd_ctx* ctx, int nrhs, const double* b, double* x, double m0, int max_iter, double tol, int* iters,
-                dd_solve_info* info);
+ * Conjugate gradient on the normal equations D^dagger D x = phi from x0 = phi, stopping
+ * when ||phi - D^dagger D x|| < tol ||phi||.  Replaces conjugate_gradient
+ * (src/conjugate_gradient.cpp:4-84); Methods::CG (src/methods.cpp:43-59) passes
+ * phi = D^dagger b (dd_apply_D with dagger = 1).  A system that does not converge
+ * reports max_iter (the reference returns 0). */
+int dd_solve_cg(dd_ctx* ctx, int nrhs, const double* phi, double* x, double m0, int max_iter, double tol,
+                int* iters, dd_solve_info* info);
 /* BiCGStab on D x = b from x0 (nullable: zero).  Replaces bi_cgstab
  * (src/conjugate_gradient.cpp:87-176) as Methods::BiCG calls it (src/methods.cpp:3-19);
  * a system that does not converge reports max_iter. */
 int dd_solve_bicgstab(dd_ctx* ctx, int nrhs, const double* b, const double* x0, double* x, double m0, int max_iter,
                       double tol, int* iters, dd_solve_info* info);
 /* GMRES(m) with `restarts` cycles on D x = b from x0 (nullable: zero), count k*m + maxIt.
- * sap_preconditioner = 0: no preconditioner, replaces FGMRES_fine_level
- * (include/fgmres.h:153-234, Methods::GMRES src/methods.cpp:21-41).
- * sap_preconditioner = 1: flexible GMRES preconditioned by one level-0 SAP iteration
- * from a zero guess with the setup's level-0 SAP blocks, replaces FGMRES_SAP
- * (include/sap.h:236-333, Methods::FGMRES_sap src/methods.cpp:83-104); needs dd_setup,
- * with m0 equal to the setup's. */
+ * sap_bx = 0: no preconditioner, replaces FGMRES_fine_level (include/fgmres.h:153-234,
+ * Methods::GMRES src/methods.cpp:21-41).
+ * sap_bx > 0: flexible GMRES preconditioned by one red-black SAP iteration from a zero
+ * guess over blocks of sap_bx x sap_bt sites (a power of two, at most 64; blocks solved by
+ * GMRES(5) x 5 to 1e-3), replaces FGMRES_SAP (include/sap.h:236-333, Methods::FGMRES_sap
+ * src/methods.cpp:83-104).  The reference's FGMRES_SAP uses 4 x 4 blocks per MPI rank
+ * (SAPV::sap_block_x/t): sap_bx = Nx / (4 ranks_x), sap_bt = Nt / (4 ranks_t). */
 int dd_solve_gmres(dd_ctx* ctx, int nrhs, const double* b, const double* x0, double* x, double m0, int m,
-                   int restarts, double tol, int sap_preconditioner, int* iters, dd_solve_info* info);
+                   int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info);
 
 /* Adjoint solve D^dagger y = eta on the same hierarchy through sigma3-hermiticity:
  * y = sigma3 D^-1 sigma3 eta.  The reference has D_dagger_phi only

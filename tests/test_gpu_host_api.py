@@ -28,8 +28,18 @@ def test_reference_style_driver(golden, tmp_path):
     stdin = f"2\n2\n2\n{meta['m0']}\n{tmp_path / 'conf.bin'}\n{tmp_path / 'rhs.bin'}\n{tmp_path / 'parameters'}\n"
     p = subprocess.run([MAIN], input=stdin, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0, p.stderr
-    assert p.stdout.count('Solution verified') == 2
+    assert p.stdout.count('Solution verified') == 5
     its = [int(line.split('(')[1].split()[0]) for line in p.stdout.splitlines() if 'time elapsed FGMRES AMG' in line]
+    count = lambda key: [int(line.split('(')[1].split()[0]) for line in p.stdout.splitlines()
+                         if line.startswith(f'time elapsed {key}:')][0]
+    # the comparison solvers through the host API: same counts as through the binding,
+    # FGMRES+SAP on 4x4 blocks per rank of the 2x2 grid (8x8 sites at 64^2)
+    with ddamg.Context(64, 64) as ctx:
+        ctx.set_gauge(U)
+        kw = dict(tol=1e-10, max_iter=10000, m=20, restarts=50, sap_block=(8, 8))
+        assert count('CG') == int(ctx.solve_method('cg', a['rhs'], meta['m0'], **kw)[1][0])
+        assert count('Bi-CGstab') == int(ctx.solve_method('bicgstab', a['rhs'], meta['m0'], **kw)[1][0])
+        assert count('FGMRES_SAP') == int(ctx.solve_method('fgmres_sap', a['rhs'], meta['m0'], **kw)[1][0])
     # same hierarchy and knobs through the Python binding (reference default nu2 = 2)
     prm = ddamg.params(2, agg=[(4, 4)], sap_block=[(4, 4)], nv=[8], m0=meta['m0'], nu2=2, fgmres_m=20,
                        rng_ranks=(2, 2))

@@ -23,24 +23,32 @@ def rel(a, b):
 def _ctx(meta, a):
     ctx = ddamg.Context(meta['N'], meta['N'])
     ctx.set_gauge(np.exp(1j * a['theta']))
-    # the level-0 SAP blocks of FGMRES+SAP: 4x4 blocks per rank of the 2x2 grid = 4x4 sites
-    ctx.setup(ddamg.params(2, agg=[(4, 4)], sap_block=[tuple(meta['sap_block'])], nv=[4], m0=meta['m0'],
-                           rng_ranks=tuple(meta['ranks']), max_rhs=3))
     return ctx
+
+
+# FGMRES+SAP: the reference's 4x4 blocks per rank of the 2x2 grid are 4x4-site blocks
+def _kw(meta):
+    return dict(tol=meta['tol'], max_iter=meta['max_iter'], m=meta['gmres_m'], restarts=meta['gmres_restarts'],
+                sap_block=tuple(meta['sap_block']))
 
 
 @pytest.mark.parametrize('method,key,slack', METHODS)
 def test_methods_golden(golden, method, key, slack):
     meta, a = golden('g32_methods')
     with _ctx(meta, a) as ctx:
-        x, its, info = ctx.solve_method(method, a['rhs'], meta['m0'], tol=meta['tol'], max_iter=meta['max_iter'],
-                                        m=meta['gmres_m'], restarts=meta['gmres_restarts'])
+        x, its, info = ctx.solve_method(method, a['rhs'], meta['m0'], **_kw(meta))
     U = np.exp(1j * a['theta'])
-    rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, meta['m0'])) / np.linalg.norm(a['rhs'])
+    m0 = meta['m0']
+    rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, m0)) / np.linalg.norm(a['rhs'])
     print(f'{method}: {its[0]} iterations (reference {meta[key + "_iterations"]}), relres {rr:.3e} '
           f'(reference {meta[key + "_relres"]:.3e}), error vs reference x {rel(x, a["x_" + key]):.2e}')
     assert abs(int(its[0]) - meta[key + '_iterations']) <= slack
-    assert abs(info.relres - rr) <= 1e-3 * rr + 1e-14
+    if method == 'cg':   # info.relres: the normal equations' residual
+        phi = orc.D_dagger_phi(U, a['rhs'], m0)
+        rn = np.linalg.norm(phi - orc.D_dagger_phi(U, orc.D_phi(U, x, m0), m0)) / np.linalg.norm(phi)
+        assert abs(info.relres - rn) <= 1e-3 * rn + 1e-14 and info.relres < meta['tol']
+    else:
+        assert abs(info.relres - rr) <= 1e-3 * rr + 1e-14
     assert rr < 2 * meta[key + '_relres'] + 1e-12
     assert rel(x, a['x_' + key]) < 1e-7
 
@@ -53,7 +61,7 @@ def test_methods_batched_equal_single(golden, method):
     rng = np.random.default_rng(7)
     B = np.stack([a['rhs']] + [rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape)
                                for _ in range(2)])
-    kw = dict(tol=meta['tol'], max_iter=meta['max_iter'], m=meta['gmres_m'], restarts=meta['gmres_restarts'])
+    kw = _kw(meta)
     with _ctx(meta, a) as ctx:
         xb, itb, _ = ctx.solve_method(method, B, meta['m0'], **kw)
         for r in range(3):
