@@ -77,6 +77,9 @@ struct GridBar {
     unsigned gen;
     unsigned fail;
     unsigned pad1[14];
+    // per system: (Arnoldi step << 1) | stop, stored by the system's owner with release
+    // once the step's update coefficients are written (see wait_coefs)
+    unsigned cflag[64];
 };
 
 // returns false when the kernel must end (a barrier timed out somewhere)
@@ -432,7 +435,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         return r;
     };
     // ---- phase B for the owned system at step j (whole work-group)
-    auto phaseB = [&](int j) {
+    auto phaseB = [&](int j, unsigned gstep) {
         const int q = sys;
         const bool hw = j < m;
         const int nv = hw ? 2 * j + 2 : j + 1;
@@ -515,6 +518,45 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (tid == 0 && !stop_now) cf[2 * m + 1] = mk<R>((R)ib, R(0));
         if (tid == 0) si[1] = stop_now;
         __syncthreads();
+        // publish: every thread's coefficient stores precede thread 0's release
+        if (tid == 0)
+            __hip_atomic_store(&a.bar->cflag[q], (gstep << 1) | (stop_now ? 1u : 0u), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // Waits (wave 0, a lane per system) until the owner of every system in `run` has
+    // published step gstep's coefficients, then acquires them for the work-group.  Returns
+    // the systems that continue (the owners' stop bits cleared), identical in every
+    // work-group; si[2] = 0 when the kernel must end (a wait timed out somewhere).  Replaces
+    // a grid barrier: a work-group waits for the owners, not for all work-groups.
+    auto wait_coefs = [&](uint64_t run, unsigned gstep) -> uint64_t {
+        __syncthreads();
+        if (wv == 0) {
+            const int q = lane;
+            const bool need = (run >> q) & 1ull;
+            unsigned v = 0, spins = 0;
+            bool ok = true;
+            if (need) {
+                while (((v = __hip_atomic_load(&a.bar->cflag[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 1) !=
+                       gstep) {
+                    if (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                        ++spins > kBarSpins) {
+                        __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint64_t stopm = __ballot(need && ok && (v & 1u));
+            const uint64_t bad = __ballot(!ok);
+            if (lane == 0) {
+                *smask = run & ~stopm;
+                si[2] = bad ? 0 : 1;
+            }
+        }
+        __syncthreads();
+        return *smask;
     };
     // ---- Givens rotation of column c = j-1 of the owned system (src/fgmres.cpp:99-115),
     // its rotated entries to Hrot and the break test |g_j| < tol ||eta||, by one thread
@@ -707,6 +749,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     }
     if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
 
+    unsigned gstep = 0;   // Arnoldi steps of this launch (the same in every work-group)
     for (int k = 0; k < a.restarts; k++) {
         const uint64_t live = sysmask(F_LIVE);
         if (!live) break;
@@ -726,11 +769,12 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             clk(0);
             if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
             clk(1);
-            if (owner && ((run >> sys) & 1ull)) phaseB(j);
+            ++gstep;
+            if (owner && ((run >> sys) & 1ull)) phaseB(j, gstep);
             clk(2);
-            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
+            const uint64_t cont = wait_coefs(run, gstep);
+            if (!si[2]) return;
             clk(3);
-            const uint64_t cont = sysmask(F_RUN) & run;
             if (tid == 0 && j >= 1 && owner && ((run >> sys) & 1ull)) givens(j);
             update(j, cont);
             clk(4);
