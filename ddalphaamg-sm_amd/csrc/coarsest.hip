@@ -52,9 +52,7 @@ constexpr int kGmax = 256;   // work-groups at most (one per compute unit)
 constexpr int kMaxM = 432;   // restart length at most (LDS state below 64 KB)
 constexpr int kBB = 32;      // block of the triangular solve
 
-// F_CONV: the least-squares residual met the tolerance (set by the owner after the
-// step's coefficients are published; the system stops at the next step)
-enum : int { F_LIVE = 1, F_RUN = 2, F_CONV = 4 };
+enum : int { F_LIVE = 1, F_RUN = 2 };
 
 struct Ctrl {
     int flags, maxit, iters, pad;
@@ -80,6 +78,9 @@ struct GridBar {
     // per system: (Arnoldi step << 1) | stop, stored by the system's owner with release
     // once the step's update coefficients are written (see wait_coefs)
     unsigned cflag[64];
+    // per work-group: the last Arnoldi step whose update pass it has finished (release),
+    // read by the work-groups whose stencil needs its rows (see wait_neighbours)
+    unsigned udone[kGmax];
 };
 
 // returns false when the kernel must end (a barrier timed out somewhere)
@@ -247,6 +248,19 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     const bool lrow = li * RPL < nrow;
     const int lb = lrow ? li * RPL : nrow - RPL;
     const int sys = wg;   // the system whose scalars this work-group owns (G >= nsys)
+    // lane (u, d) = (lane >> 2, lane & 3) of wave 0: the work-group owning the neighbour of
+    // own site s0 + u in direction d (t+1, x+1, t-1, x-1), or -1 (none, or this work-group)
+    int nbw = -1;
+    {
+        const int u = lane >> 2, d = lane & 3;
+        if (wv == 0 && s0 + u < s1 && u < kSpwMax) {
+            const int site = s0 + u, x = site / a.Nt, t = site - x * a.Nt;
+            const int nx = d == 1 ? (x + 1) % a.Nx : d == 3 ? (x + a.Nx - 1) % a.Nx : x;
+            const int nt = d == 0 ? (t + 1) % a.Nt : d == 2 ? (t + a.Nt - 1) % a.Nt : t;
+            const int w2 = (nx * a.Nt + nt) / a.spw;
+            nbw = w2 == wg ? -1 : w2;
+        }
+    }
     const bool owner = sys < a.nsys && ((a.mask >> sys) & 1ull);
     // partial sums: work-group-major [G][nsys][VALS], each work-group writes its own
     // contiguous block
@@ -275,9 +289,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     // systems whose control word has `flag`, read by the 64 lanes of wave 0 at once and
     // broadcast through LDS.  Control words are written between grid barriers and read
     // after them (the barrier orders and publishes them), so plain loads and stores
-    // suffice and the result is the same in every work-group.  The one write that
-    // overlaps a read, F_CONV set by an owner in the update phase, changes no bit that
-    // the update phase's own read (F_RUN) looks at.
+    // suffice and the result is the same in every work-group.  Owners write them in phase
+    // B, before the release of their coefficient flags, and every work-group reads them
+    // after it has acquired those flags.
     auto sysmask = [&](int flag, int off = 0) -> uint64_t {
         __syncthreads();
         if (wv == 0) {
@@ -490,14 +504,16 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         }
         __syncthreads();
         const double beta = sc[0];
-        // the last step of the cycle and a breakdown stop now (no update follows); the
-        // Givens rotation of column j-1 and the convergence test are left to givens(),
-        // after the coefficients are published
-        const bool stop_now = j == m || beta == 0.0;
+        // the last step of the cycle, a breakdown and a break found by the previous step's
+        // givens() (si[0]; it set maxIt) stop now: no update follows.  The Givens rotation
+        // of column j-1 and its break test are left to givens(), after the coefficients
+        // are published.
+        const bool conv = si[0] != 0;
+        const bool stop_now = j == m || beta == 0.0 || conv;
         if (tid == 0) {
             if (j == 0) gls[0] = mk<double>(beta, 0.0);
             if (stop_now) {
-                a.ctrl[q].maxit = j;
+                if (!conv) a.ctrl[q].maxit = j;
                 set_flags(q, a.ctrl[q].flags & ~F_RUN);
             }
         }
@@ -522,6 +538,35 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (tid == 0)
             __hip_atomic_store(&a.bar->cflag[q], (gstep << 1) | (stop_now ? 1u : 0u), __ATOMIC_RELEASE,
                                __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // Waits (wave 0, a lane per neighbour) until every work-group whose rows this slice's
+    // stencil reads has finished the update pass of Arnoldi step gstep, then acquires
+    // its stores; si[2] = 0 when the kernel must end.  Replaces a grid barrier after the
+    // update pass: a slice waits for its (at most 4 spw) neighbours.  No work-group can
+    // overwrite rows that a neighbour's stencil still reads: its next update waits for
+    // coefficients that need every work-group's next partial sums, which follow that
+    // work-group's stencil.
+    auto wait_neighbours = [&](unsigned gstep) {
+        __syncthreads();
+        if (wv == 0) {
+            bool ok = true;
+            if (nbw >= 0) {
+                unsigned spins = 0;
+                while (__hip_atomic_load(&a.bar->udone[nbw], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
+                    if (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                        ++spins > kBarSpins) {
+                        __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint64_t bad = __ballot(!ok);
+            if (lane == 0) si[2] = bad ? 0 : 1;
+        }
+        __syncthreads();
     };
     // Waits (wave 0, a lane per system) until the owner of every system in `run` has
     // published step gstep's coefficients, then acquires them for the work-group.  Returns
@@ -562,6 +607,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     // its rotated entries to Hrot and the break test |g_j| < tol ||eta||, by one thread
     // while the work-group's other waves already run the update pass
     auto givens = [&](int j) {
+        if (si[0]) return;   // stopped by the previous column's break (maxIt = j - 1)
         const int q = sys, c = j - 1;
         cplx<double> carry = hcol[0];
         for (int i = 0; i < c; i++) {
@@ -582,7 +628,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         for (int i = 0; i < j; i++) Ho[(size_t)i * m + c] = hcol[i];
         if (!si[1] && sqrt(norm2(gls[c + 1])) < a.tol * a.nrm[2 * q]) {
             a.ctrl[q].maxit = j;
-            set_flags(q, a.ctrl[q].flags | F_CONV);
+            si[0] = 1;   // the next step's phase B stops the system
         }
     };
     // ---- phase C: q_j and u_{j+1} on the own rows (wave per system); the update
@@ -754,11 +800,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         const uint64_t live = sysmask(F_LIVE);
         if (!live) break;
         const int nl = __builtin_popcountll(live);
+        if (tid == 0) si[0] = 0;   // no break pending (owner)
         for (int j = 0; j <= m; j++) {
-            const uint64_t run = sysmask(F_RUN, F_CONV);
+            const uint64_t run = sysmask(F_RUN);
             if (!run) break;
             const int nr = __builtin_popcountll(run);
             const bool hw = j < m;
+            // the rows of u_j the stencil reads were written by the neighbours' update of
+            // the previous step (j = 0: before the last grid barrier)
+            if (j > 0) {
+                wait_neighbours(gstep);
+                if (!si[2]) return;
+            }
             if (hw) stencil(a.u, a.w, nullptr, run, false);
             __syncthreads();
             clk(7);
@@ -782,7 +835,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * (sQ * j + sR * (hw ? 2 : 1)) +
                       nc * n * (sQ * (j + 1) + sR * 3),
                   1);
-            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
+            // publish this slice's update (every thread's stores precede thread 0's release)
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(&a.bar->udone[wg], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             clk(5);
         }
         if (owner && ((live >> sys) & 1ull)) backsub();
