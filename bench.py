@@ -33,6 +33,7 @@ def log(*a):
 
 
 COARSE_BASIS = None   # None: the library default; 0 FP32, 1 binary16 (--coarse-basis)
+SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-sap-links)
 
 
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
@@ -40,6 +41,8 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
                      fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
     if COARSE_BASIS is not None:
         p.coarse_basis = COARSE_BASIS
+    if SAP_LINKS is not None:
+        p.coarse_sap_links = SAP_LINKS
     return p
 
 
@@ -62,9 +65,12 @@ def main():
                     help='only run three device copies of 256 MiB (scale of the HBM byte counters)')
     ap.add_argument('--coarse-basis', type=int, choices=[0, 1], default=None,
                     help='storage of the coarsest GMRES basis in the FP32 hierarchy (0 FP32, 1 binary16)')
+    ap.add_argument('--coarse-sap-links', type=int, choices=[0, 1], default=None,
+                    help='links read by the matrix-core coarse SAP of the FP32 hierarchy (0 FP32, 1 binary16 copy)')
     args = ap.parse_args()
-    global COARSE_BASIS
+    global COARSE_BASIS, SAP_LINKS
     COARSE_BASIS = args.coarse_basis
+    SAP_LINKS = args.coarse_sap_links
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU only (the multi-GPU domain decomposition is '
               'design-only); run with --gpus 1', file=sys.stderr)

@@ -134,6 +134,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->autotune = 1;
     p->sap_variant = -1;
     p->coarse_basis = 1;
+    p->coarse_sap_links = 0;
     return DD_OK;
 }
 
@@ -571,7 +572,14 @@ int dd_get_links(dd_ctx* c, int level, double* out, int precision) {
             hipStream_t s = c->s->s;
             const size_t tot = (size_t)G.V() * 5 * G.dof * G.dof;
             DevMem o(sizeof(cplx<double>) * tot);
-            dd::convert<double, R>(s, o.as<cplx<double>>(), m->links(level), tot, 0, 0, 1, 1);
+            if (precision == 2) {
+                if constexpr (sizeof(R) == 4) {
+                    if (!m->hlinks(level)) throw dd::BadShape("no binary16 link copy on this level");
+                    dd::from_half(s, m->hlinks(level), o.as<double>(), 2 * tot);
+                }
+            } else {
+                dd::convert<double, R>(s, o.as<cplx<double>>(), m->links(level), tot, 0, 0, 1, 1);
+            }
             std::vector<double> h(2 * tot);
             download(h.data(), o.as<cplx<double>>(), tot, s);
             const int d = G.dof;

@@ -83,9 +83,16 @@ struct GridBar {
     unsigned udone[kGmax];
 };
 
+// Executed by every wave that stored data before the work-group barrier that precedes a
+// flag release by one thread: waits until this wave's global stores are acknowledged by
+// the L2.  The releasing thread's agent-scope release then writes back the L2 (all waves'
+// lines) and waits only for its own wave's stores.
+__device__ __forceinline__ void release_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // returns false when the kernel must end (a barrier timed out somewhere)
 __device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, int G, int* sflag) {
     (void)wg;
+    release_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
@@ -417,10 +424,17 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         };
         int i0 = 0;
         for (; i0 + kCH <= j; i0 += kCH) chunk(std::integral_constant<int, kCH>{}, i0);
-        if (kCH > kGV)
-            for (; i0 < j; i0 += kGV) chunk(std::integral_constant<int, kGV>{}, i0);
-        else if (i0 < j)
-            chunk(std::integral_constant<int, kCH>{}, i0);
+        // the remainder as one chunk of the next multiple of kGV vectors, so that all its
+        // loads are in flight together (at most kGV - 1 clamped duplicates, masked)
+        const int ng = (j - i0 + kGV - 1) / kGV;
+        if (ng == 1) chunk(std::integral_constant<int, kGV>{}, i0);
+        if constexpr (2 * kGV <= kCH)
+            if (ng == 2) chunk(std::integral_constant<int, 2 * kGV>{}, i0);
+        if constexpr (3 * kGV <= kCH)
+            if (ng == 3) chunk(std::integral_constant<int, 3 * kGV>{}, i0);
+        if constexpr (4 * kGV <= kCH)
+            if (ng == 4) chunk(std::integral_constant<int, 4 * kGV>{}, i0);
+        static_assert(kCH <= 4 * kGV, "remainder cases");
     };
     auto reduce_norm = [&](uint64_t mk2, int v, const cplx<R>* vec) {
         for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
@@ -533,6 +547,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         }
         if (tid == 0 && !stop_now) cf[2 * m + 1] = mk<R>((R)ib, R(0));
         if (tid == 0) si[1] = stop_now;
+        if (wv * 64 <= j) release_stores();   // the waves that stored coefficients
         __syncthreads();
         // publish: every thread's coefficient stores precede thread 0's release
         if (tid == 0)
@@ -677,7 +692,17 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             };
             int i1 = j;
             for (; i1 >= kCU; i1 -= kCU) chunk(std::integral_constant<int, kCU>{}, i1 - kCU);
-            for (; i1 > 0; i1 -= 4) chunk(std::integral_constant<int, 4>{}, i1 - 4);
+            // the oldest remainder as one chunk of the next multiple of 4 vectors (indices
+            // below 0 masked)
+            const int ng = (i1 + 3) / 4;
+            if (ng == 1) chunk(std::integral_constant<int, 4>{}, i1 - 4);
+            if constexpr (8 <= kCU)
+                if (ng == 2) chunk(std::integral_constant<int, 8>{}, i1 - 8);
+            if constexpr (12 <= kCU)
+                if (ng == 3) chunk(std::integral_constant<int, 12>{}, i1 - 12);
+            if constexpr (16 <= kCU)
+                if (ng == 4) chunk(std::integral_constant<int, 16>{}, i1 - 16);
+            static_assert(kCU <= 16, "remainder cases");
             // u_{j+1} is orthogonalised against the stored q_j
             const cplx<R> cj = cf[m + j];
             S* qo = a.Q + (size_t)j * vs + (size_t)q * n + row0 + lb;
@@ -836,6 +861,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                       nc * n * (sQ * (j + 1) + sR * 3),
                   1);
             // publish this slice's update (every thread's stores precede thread 0's release)
+            release_stores();
             __syncthreads();
             if (tid == 0) __hip_atomic_store(&a.bar->udone[wg], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             clk(5);

@@ -52,11 +52,13 @@ void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, 
                  int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
                  const int* gate);
 // variant: kernel variant (systems per work-group, load batching), -1 = default;
-// sapc_variants<R>() lists the compiled ones (autotuned in solver.hip)
+// sapc_variants<R>() lists the compiled ones (autotuned in solver.hip).  Lh: binary16
+// copy of L (to_half), read by variant 33 (the matrix-core kernel on binary16 links);
+// nullptr where there is none.
 template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
                  int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask,
-                 size_t ld, const int* gate, int variant = -1);
+                 size_t ld, const int* gate, int variant = -1, const _Float16* Lh = nullptr);
 template <typename R>
 std::vector<int> sapc_variants();
 template <typename R>
@@ -124,6 +126,10 @@ template <typename To, typename From>
 void convert(hipStream_t s, cplx<To>* out, const cplx<From>* in, size_t n, size_t ld_out, size_t ld_in, int nrhs,
              uint64_t mask);
 // multiply the spin-1 components by -1 (sigma_3 on dof = colour*2 + spin)
+// out[i] = in[i] rounded to binary16 (n reals)
+void to_half(hipStream_t s, const float* in, _Float16* out, size_t n);
+// out[i] = in[i] (exact)
+void from_half(hipStream_t s, const _Float16* in, double* out, size_t n);
 template <typename R>
 void sigma3(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, int nrhs, uint64_t mask);
 // gate[r] = 0 where sqrt(err2[r*stride].x) < tol * sqrt(v2[r*stride].x)

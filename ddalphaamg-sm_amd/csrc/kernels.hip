@@ -766,6 +766,26 @@ void convert(hipStream_t s, cplx<To>* out, const cplx<From>* in, size_t n, size_
     DD_LAUNCH_CHECK();
 }
 
+__global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ in, _Float16* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (_Float16)in[i];
+}
+
+void to_half(hipStream_t s, const float* in, _Float16* out, size_t n) {
+    k_to_half<<<ew_blocks(n), 256, 0, s>>>(in, out, n);
+    DD_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(256) void k_from_half(const _Float16* __restrict__ in, double* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+void from_half(hipStream_t s, const _Float16* in, double* out, size_t n) {
+    k_from_half<<<ew_blocks(n), 256, 0, s>>>(in, out, n);
+    DD_LAUNCH_CHECK();
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_zero(cplx<R>* out, size_t n, size_t ld, uint64_t mask) {
     const int r = blockIdx.y;

@@ -63,11 +63,12 @@ __device__ __forceinline__ constexpr int link_step(int s, int t) {
     return (2 * s * DOF + 8 * t) * 2;
 }
 
-// acc[t] += hop contribution of one complex link matrix Lh (reals) on a B operand
-// given per K-step in bv[s] (already multiplied by the hop's boundary sign).  The link
-// operands are loaded CH K-steps at a time (all MT tiles) ahead of their products.
-template <typename R, int DOF, int CH = DOF / 2>
-__device__ __forceinline__ void mfma_hop(typename Mfma<R>::acc_t (&acc)[DOF / 8], const R* __restrict__ Lh,
+// acc[t] += hop contribution of one complex link matrix Lh (reals of type T: R, or
+// binary16 widened to R on load) on a B operand given per K-step in bv[s] (already
+// multiplied by the hop's boundary sign).  The link operands are loaded CH K-steps at a
+// time (all MT tiles) ahead of their products.
+template <typename R, int DOF, int CH = DOF / 2, typename T = R>
+__device__ __forceinline__ void mfma_hop(typename Mfma<R>::acc_t (&acc)[DOF / 8], const T* __restrict__ Lh,
                                          const LinkLane& ll, const R (&bv)[DOF / 2]) {
     constexpr int MT = DOF / 8, KS = DOF / 2;
     static_assert(KS % CH == 0, "chunk must divide the K-steps");
@@ -77,7 +78,7 @@ __device__ __forceinline__ void mfma_hop(typename Mfma<R>::acc_t (&acc)[DOF / 8]
 #pragma unroll
         for (int s = 0; s < CH; s++)
 #pragma unroll
-            for (int t = 0; t < MT; t++) a[s][t] = Lh[ll.aoff + link_step<DOF>(s0 + s, t)];
+            for (int t = 0; t < MT; t++) a[s][t] = (R)Lh[ll.aoff + link_step<DOF>(s0 + s, t)];
 #pragma unroll
         for (int s = 0; s < CH; s++)
 #pragma unroll

@@ -5,7 +5,8 @@ namespace dd {
 template void sap_blocks0<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
                                double, int, int, double, int, uint64_t, size_t, const int*);
 template void sap_blocksc<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
-                               int, double, int, int, double, int, uint64_t, size_t, const int*, int);
+                               int, double, int, int, double, int, uint64_t, size_t, const int*, int,
+                                 const _Float16*);
 template std::vector<int> sapc_variants<float>();
 template bool sapc_supported<float>(int, int, int);
 }  // namespace dd

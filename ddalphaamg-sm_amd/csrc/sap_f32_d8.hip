@@ -3,6 +3,6 @@
 #include "sap_impl.h"
 
 namespace dd {
-template void dispatch_sapc<float, 8>(int, hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int,
+template void dispatch_sapc<float, 8>(int, hipStream_t, const _Float16*, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int,
                                       int, int, int, double, int, double, int, uint64_t, size_t, const int*);
 }  // namespace dd

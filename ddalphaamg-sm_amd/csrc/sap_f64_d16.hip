@@ -3,6 +3,6 @@
 #include "sap_impl.h"
 
 namespace dd {
-template void dispatch_sapc<double, 16>(int, hipStream_t, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int,
+template void dispatch_sapc<double, 16>(int, hipStream_t, const _Float16*, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int,
                                       int, int, int, double, int, double, int, uint64_t, size_t, const int*);
 }  // namespace dd

@@ -506,29 +506,31 @@ static void launch_sapc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<
 // loads of a hop are issued together.  variant = 2*NB + FULL; sapc_variants lists the
 // ones compiled for precision R (the autotuner in solver.hip times them).
 // Variant 32 (FP32 only) is the matrix-core kernel k_sapm (16 systems per work-group,
-// sapm.h), available for dof <= 24 and blocks of at most 16 sites.
+// sapm.h), available for dof <= 24 and blocks of at most 16 sites; variant 33 is the same
+// kernel reading a binary16 copy of the links.
 template <typename R>
 std::vector<int> sapc_variants() {
-    if (sizeof(R) == 4) return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1, 2 * 8 + 0, 2 * 8 + 1, 32};
+    if (sizeof(R) == 4) return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1, 2 * 8 + 0, 2 * 8 + 1, 32, 33};
     return {2 * 1 + 0, 2 * 1 + 1, 2 * 2 + 1, 2 * 4 + 0, 2 * 4 + 1};
 }
 
 template <typename R>
 bool sapc_supported(int variant, int dof, int S) {
-    if (variant == 32) return sizeof(R) == 4 && dof <= 24 && S <= 16;
+    if (variant == 32 || variant == 33) return sizeof(R) == 4 && dof <= 24 && S <= 16;
     return S * dof <= 512;
 }
 
 template <typename R>
-int sapc_default_variant(int nrhs, int dof, int S) {
-    if (sapc_supported<R>(32, dof, S)) return 32;
+int sapc_default_variant(int nrhs, int dof, int S, bool half_links) {
+    if (sapc_supported<R>(32, dof, S)) return half_links ? 33 : 32;
     if (nrhs <= 1) return 2 * 1 + 0;
     if (sizeof(R) == 8 || nrhs <= 4) return 2 * 4 + 0;
     return 2 * 8 + 0;
 }
 
 template <typename R, int D>
-void dispatch_sapc(int variant, hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt,
+void dispatch_sapc(int variant, hipStream_t s, const _Float16* Lh, const cplx<R>* L, const cplx<R>* r, cplx<R>* x,
+                   int Nx, int Nt,
                           int bx, int bt, int colour, double m0, int restarts, double tol, int nrhs, uint64_t mask,
                           size_t ld, const int* gate) {
 #define DD_V(NB, F) launch_sapc<R, D, NB, F>(s, L, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs, mask, ld, gate)
@@ -547,9 +549,12 @@ void dispatch_sapc(int variant, hipStream_t s, const cplx<R>* L, const cplx<R>* 
             else throw BadShape("sap_blocksc: variant not compiled for FP64");
             break;
         case 32:
-            if constexpr (sizeof(R) == 4 && D <= 24)
-                launch_sapm<D>(s, L, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs, mask, ld, gate);
-            else throw BadShape("sap_blocksc: matrix-core variant needs FP32 and dof <= 24");
+        case 33:
+            if constexpr (sizeof(R) == 4 && D <= 24) {
+                if (variant == 33 && !Lh) throw BadShape("sap_blocksc: variant 33 needs the binary16 links");
+                launch_sapm<D>(s, L, variant == 33 ? Lh : nullptr, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs,
+                               mask, ld, gate);
+            } else throw BadShape("sap_blocksc: matrix-core variant needs FP32 and dof <= 24");
             break;
         default:
             throw BadShape("sap_blocksc: unknown variant");
@@ -560,7 +565,8 @@ void dispatch_sapc(int variant, hipStream_t s, const cplx<R>* L, const cplx<R>* 
 // Each (precision, dof) pair of the coarse SAP dispatch is compiled in its own
 // translation unit (sap_<prec>_d<dof>.hip) to keep builds parallel.
 #define DD_SAPC_DECL(R, D)                                                                                         \
-    extern template void dispatch_sapc<R, D>(int, hipStream_t, const cplx<R>*, const cplx<R>*, cplx<R>*, int, int, \
+    extern template void dispatch_sapc<R, D>(int, hipStream_t, const _Float16*, const cplx<R>*, const cplx<R>*,     \
+                                             cplx<R>*, int, int,                                                 \
                                              int, int, int, double, int, double, int, uint64_t, size_t, const int*);
 #ifndef DD_SAPC_UNIT
 DD_SAPC_DECL(float, 8)
@@ -576,23 +582,25 @@ DD_SAPC_DECL(double, 32)
 template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
                  int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
-                 const int* gate, int variant) {
+                 const int* gate, int variant, const _Float16* Lh) {
     timer_begin(s, K_SAPC);
     const int S = bx * bt;
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
-    if (variant < 0) variant = sapc_default_variant<R>(nrhs, dof, S);
+    if (variant < 0) variant = sapc_default_variant<R>(nrhs, dof, S, Lh != nullptr);
     if (!sapc_supported<R>(variant, dof, S)) throw BadShape("coarse SAP: variant unsupported for this block/dof");
     switch (dof) {
 #define DD_CASE(D)                                                                                             \
     case D:                                                                                                    \
-        dispatch_sapc<R, D>(variant, s, L, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs, mask, ld, gate); \
+        dispatch_sapc<R, D>(variant, s, Lh, L, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, nrhs, mask, ld,   \
+                            gate);                                                                                \
         break;
         DD_DOF_LIST(DD_CASE)
 #undef DD_CASE
         default:
             throw BadShape("sap_blocksc: unsupported dof");
     }
-    timer_end(s, K_SAPC, 0.5 * Nx * Nt * dof * sizeof(cplx<R>) * (5.0 * dof + 3.0 * nact(mask, nrhs)));
+    const double lb = variant == 33 ? 2.0 * sizeof(_Float16) : (double)sizeof(cplx<R>);   // bytes per link element
+    timer_end(s, K_SAPC, 0.5 * Nx * Nt * dof * (5.0 * dof * lb + 3.0 * nact(mask, nrhs) * sizeof(cplx<R>)));
     DD_LAUNCH_CHECK();
 }
 

@@ -19,7 +19,12 @@
 //     lane j of wave 0; per-system flags are uniform 16-bit masks (ballots);
 //   * the work-groups of one block's system tiles are numbered 8 apart (1-D grid), so
 //     the round-robin dispatch puts them on the same XCD at the same time and the second
-//     tile reads the block's links from that XCD's L2.
+//     tile reads the block's links from that XCD's L2;
+//   * the links are read as FP32 or as a binary16 copy (LT = _Float16, widened on load):
+//     the kernel streams every link of its block once per operator application, so the
+//     copy halves its dominant memory traffic.  The copy perturbs each link element by at
+//     most 2^-11 relative; it serves only the smoother's block solves (the level operator
+//     and the Galerkin products keep the FP32 links).
 #pragma once
 #include "mfma.h"
 
@@ -37,8 +42,8 @@ constexpr size_t sapm_lds_bytes(int S) {
            sizeof(float) * 16 + sizeof(int) * 20;
 }
 
-template <int DOF, int SPW>
-__global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L, const cplx<float>* __restrict__ rr,
+template <int DOF, int SPW, typename LT>
+__global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cplx<float>* __restrict__ rr,
                                               cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
                                               int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
                                               const int* __restrict__ gate, int nrhs, int ntile) {
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L,
             acc_t acc[MT];
 #pragma unroll
             for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
-            const R* Ln = reinterpret_cast<const R*>(L + (size_t)gn[u] * 5 * DOF * DOF);
+            const LT* Ln = L + (size_t)gn[u] * 5 * DOF * DOF * 2;
 #pragma unroll 1
             for (int hop = 0; hop < 5; hop++) {
                 const int nk = nbl[u][hop];
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L,
                 R bv[KS];
 #pragma unroll
                 for (int s2 = 0; s2 < KS; s2++) bv[s2] = stgf[((nk * DOF + 2 * s2 + (g >> 1)) * 16 + j) * 2 + (g & 1)];
-                mfma_hop<R, DOF, (DOF / 2) % 4 == 0 ? 4 : DOF / 2>(acc, Ln + (size_t)hop * DOF * DOF * 2, ll, bv);
+                mfma_hop<R, DOF, (DOF / 2) % 4 == 0 ? 4 : DOF / 2, LT>(acc, Ln + (size_t)hop * DOF * DOF * 2, ll, bv);
             }
 #pragma unroll
             for (int t = 0; t < MT; t++) {
@@ -333,35 +338,41 @@ __global__ __launch_bounds__(512) void k_sapm(const cplx<float>* __restrict__ L,
     }
 }
 
+template <int DOF, int SPW, typename LT>
+static void launch_sapm_k(hipStream_t s, dim3 grid, int threads, size_t sh, const LT* L, const cplx<float>* r,
+                          cplx<float>* x, int Nx, int Nt, int bx, int bt, int colour, double m0, int restarts,
+                          double tol, int nrhs, uint64_t mask, size_t ld, const int* gate, int ntile) {
+    static bool attr = false;
+    if (!attr) {
+        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sapm<DOF, SPW, LT>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    k_sapm<DOF, SPW, LT><<<grid, threads, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
+                                                    mask, ld, gate, nrhs, ntile);
+}
+
+// L: FP32 links [V][5][DOF][DOF] complex, or (Lh non-null) their binary16 copy, which is
+// then the one read
 template <int DOF>
-static void launch_sapm(hipStream_t s, const cplx<float>* L, const cplx<float>* r, cplx<float>* x, int Nx, int Nt,
-                        int bx, int bt, int colour, double m0, int restarts, double tol, int nrhs, uint64_t mask,
-                        size_t ld, const int* gate) {
+static void launch_sapm(hipStream_t s, const cplx<float>* L, const _Float16* Lh, const cplx<float>* r,
+                        cplx<float>* x, int Nx, int Nt, int bx, int bt, int colour, double m0, int restarts,
+                        double tol, int nrhs, uint64_t mask, size_t ld, const int* gate) {
     const int S = bx * bt;
     if (S > 16) throw BadShape("matrix-core coarse SAP: at most 16 sites per block");
     const size_t sh = sapm_lds_bytes<DOF>(S);
     if (sh > 160 * 1024) throw BadShape("matrix-core coarse SAP: block does not fit the LDS");
     const int nblk = (Nx / bx) * (Nt / bt) / 2, ntile = (nrhs + 15) / 16;
     const dim3 grid(((nblk + 7) / 8) * 8 * ntile);
-    if (S > 8) {   // two sites per wave
-        static bool attr = false;
-        if (!attr) {
-            DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sapm<DOF, 2>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr = true;
-        }
-        k_sapm<DOF, 2><<<grid, 64 * ((S + 1) / 2), sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts,
-                                                           (float)tol, mask, ld, gate, nrhs, ntile);
-    } else {
-        static bool attr = false;
-        if (!attr) {
-            DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sapm<DOF, 1>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr = true;
-        }
-        k_sapm<DOF, 1><<<grid, 64 * S, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
-                                                mask, ld, gate, nrhs, ntile);
-    }
+    const float* Lf = reinterpret_cast<const float*>(L);
+#define DD_SAPM(SPW, THREADS)                                                                                      \
+    (Lh ? launch_sapm_k<DOF, SPW, _Float16>(s, grid, THREADS, sh, Lh, r, x, Nx, Nt, bx, bt, colour, m0, restarts, \
+                                            tol, nrhs, mask, ld, gate, ntile)                                      \
+        : launch_sapm_k<DOF, SPW, float>(s, grid, THREADS, sh, Lf, r, x, Nx, Nt, bx, bt, colour, m0, restarts, tol, \
+                                         nrhs, mask, ld, gate, ntile))
+    if (S > 8) DD_SAPM(2, 64 * ((S + 1) / 2));   // two sites per wave
+    else DD_SAPM(1, 64 * S);
+#undef DD_SAPM
 }
 
 }  // namespace dd

@@ -99,6 +99,8 @@ class Multigrid {
     int levels() const { return (int)g_.size(); }
     const cplx<R>* links(int l) const { return links_[l].template as<cplx<R>>(); }
     const cplx<R>* W(int l) const { return W_[l].template as<cplx<R>>(); }
+    // binary16 copy of level l's links for the coarse SAP, or nullptr
+    const _Float16* hlinks(int l) const { return hlinks_[l].template as<_Float16>(); }
     int cap(int l) const { return cap_[l]; }
     // times every compiled coarse-SAP variant on the built hierarchy (a batch of
     // max_rhs pseudo-random systems, one colour) and keeps the fastest per level
@@ -124,7 +126,7 @@ class Multigrid {
     std::vector<LevelGeom> g_;
     std::vector<int> cap_;
     const cplx<R>* U_;
-    std::vector<DevMem> links_, W_;
+    std::vector<DevMem> links_, W_, hlinks_;
     // per-level work: residual, restricted rhs / coarse solution of the next level,
     // prolongated correction, SAP residual and gates
     std::vector<DevMem> res_, eta_, psi_, sapr_, gate_, nrm_;

@@ -297,6 +297,7 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
     if (cap > 64) throw BadShape("max_rhs must be <= 64");
     cap_.resize(L);
     links_.resize(L);
+    hlinks_.resize(L);
     W_.resize(L);
     res_.resize(L);
     eta_.resize(L);
@@ -321,6 +322,9 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
             eta_[l].alloc(vb * cap);
             psi_[l].alloc(vb * cap);
             links_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * 5 * G.dof * G.dof);
+            // binary16 link copy of an FP32 level smoothed by the matrix-core coarse SAP
+            if (sizeof(R) == 4 && p.coarse_sap_links == 1 && l < L - 1 && sapc_supported<R>(32, G.dof, G.bx * G.bt))
+                hlinks_[l].alloc(sizeof(_Float16) * 2 * (size_t)G.V() * 5 * G.dof * G.dof);
             kry_[l].reset(new Krylov<R>(s, G.n(), p.kcycle_m, p.kcycle_restarts, p.kcycle_tol, cap, true, p.ortho_inner));
         }
         if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
@@ -383,7 +387,7 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
             else
                 sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, colour, p_.m0, p_.sap_m,
                                p_.sap_restarts, p_.sap_tol, nr, mask, n, gate,
-                               l < (int)sapc_var_.size() ? sapc_var_[l] : -1);
+                               l < (int)sapc_var_.size() ? sapc_var_[l] : -1, hlinks(l));
             apply(l, x, r, v, mask);
         }
         dot_partial<R>(s_, r, r, n, n, nr, mask, rb_[l]);
@@ -505,6 +509,9 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
         orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
         const cplx<R>* Lf = (l == 0) ? L0.as<cplx<R>>() : links(l);
         galerkin<R>(s_, Lf, W(l), links_[l + 1].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, l > 0);
+        if (hlinks_[l + 1].bytes())
+            to_half(s_, reinterpret_cast<const float*>(links(l + 1)), hlinks_[l + 1].as<_Float16>(),
+                    hlinks_[l + 1].bytes() / sizeof(_Float16));
         DD_HIP(hipStreamSynchronize(s_));
     }
 }
@@ -529,13 +536,14 @@ void Multigrid<R>::autotune() {
         int bestv = -1;
         for (int v : sapc_variants<R>()) {
             if (!sapc_supported<R>(v, G.dof, G.bx * G.bt)) continue;
-            if (v != 32 && v / 2 > 1 && nr == 1) continue;   // multi-system vector variants only for batches
+            if (v != 32 && v != 33 && v / 2 > 1 && nr == 1) continue;   // multi-system vector variants only for batches
+            if (v == 33 && !hlinks(l)) continue;                          // no binary16 link copy
             double t = 0;
             for (int rep = 0; rep < 2; rep++) {
                 zero<R>(s_, x, n * nr, 0, 1, 1);
                 DD_HIP(hipEventRecord(e0, s_));
                 sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, 0, p_.m0, p_.sap_m,
-                               p_.sap_restarts, p_.sap_tol, nr, full_mask(nr), n, nullptr, v);
+                               p_.sap_restarts, p_.sap_tol, nr, full_mask(nr), n, nullptr, v, hlinks(l));
                 DD_HIP(hipEventRecord(e1, s_));
                 DD_HIP(hipEventSynchronize(e1));
                 float ms = 0;
@@ -548,7 +556,10 @@ void Multigrid<R>::autotune() {
             }
             if (std::getenv("DDAMG_VERBOSE"))
                 fprintf(stderr, "autotune level %d: coarse SAP variant %d (%d systems/work-group%s): %.3f ms\n", l, v,
-                        v == 32 ? 16 : v / 2, v == 32 ? ", matrix cores" : ((v & 1) ? ", batched link loads" : ""), t);
+                        v >= 32 ? 16 : v / 2,
+                        v == 32 ? ", matrix cores" : v == 33 ? ", matrix cores, binary16 links"
+                                                   : ((v & 1) ? ", batched link loads" : ""),
+                        t);
         }
         sapc_var_[l] = bestv;
         sapc_ms_[l] = best;
@@ -602,8 +613,8 @@ void Multigrid<R>::set_sap_variant(int v) {
     sapc_var_.assign(L, -1);
     sapc_ms_.assign(L, 0.0);
     for (int l = 1; l < L - 1; l++) {
-        if (!sapc_supported<R>(v, g_[l].dof, g_[l].bx * g_[l].bt))
-            throw BadShape("sap_variant not available for this level's precision, dof or block");
+        if (!sapc_supported<R>(v, g_[l].dof, g_[l].bx * g_[l].bt) || (v == 33 && !hlinks(l)))
+            throw BadShape("sap_variant not available for this level's precision, dof, block or link copy");
         sapc_var_[l] = v;
     }
 }

@@ -48,7 +48,7 @@ class Params(C.Structure):
                 ('rng_ranks_x', C.c_int), ('rng_ranks_t', C.c_int),
                 ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
                 ('max_rhs', C.c_int), ('autotune', C.c_int),
-                ('sap_variant', C.c_int), ('coarse_basis', C.c_int)]
+                ('sap_variant', C.c_int), ('coarse_basis', C.c_int), ('coarse_sap_links', C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -365,7 +365,8 @@ class Context:
         return out
 
     def links(self, level, precision=0):
-        """(A, B[2], C[2]) of level >= 1, each (Nx, Nt, dof, dof)."""
+        """(A, B[2], C[2]) of level >= 1, each (Nx, Nt, dof, dof); precision 0 FP64, 1 FP32,
+        2 the binary16 copy of the FP32 links that the matrix-core coarse SAP reads."""
         Nx, Nt, d = self.level_shape(level)
         out = np.zeros((Nx, Nt, 5, d, d), complex)
         _chk(lib.dd_get_links(self._h, level, _cptr(out), precision))

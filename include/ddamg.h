@@ -69,11 +69,17 @@ typedef struct dd_params {
                                       built hierarchy at setup and keep the fastest  */
     int sap_variant;               /* coarse-SAP kernel variant for every coarse level
                                       (2*systems_per_workgroup + batched_link_loads;
-                                      32 = matrix cores, FP32); -1: autotuned/default */
+                                      32 = matrix cores, FP32; 33 = matrix cores on
+                                      the binary16 link copy); -1: autotuned/default */
     int coarse_basis;              /* storage of the persistent coarsest GMRES's Krylov
                                       basis in the FP32 hierarchy: 0 = FP32, 1 = IEEE
                                       binary16 (compressed basis, products in FP32;
                                       default).  Ignored by the FP64 hierarchy.     */
+    int coarse_sap_links;          /* links read by the matrix-core coarse SAP of the
+                                      FP32 hierarchy: 0 = the FP32 links (default),
+                                      1 = a binary16 copy made at setup (each element
+                                      within 2^-11 relative; the level operator keeps
+                                      FP32).  Ignored by the FP64 hierarchy.         */
 } dd_params;
 
 typedef struct dd_solve_info {
@@ -189,7 +195,9 @@ int dd_prolong(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, 
 int dd_sap(dd_ctx* ctx, int level, const double* v, double* x, int nu, double tol, int nrhs, int precision);
 /* Test vectors of level l (after setup): out[nv][V][dof]. */
 int dd_get_test_vectors(dd_ctx* ctx, int level, double* out, int precision);
-/* Links of level l >= 1 as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1). */
+/* Links of level l >= 1 as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1);
+   precision 0 FP64, 1 FP32, 2 the FP32 hierarchy's binary16 copy read by the
+   matrix-core coarse SAP (coarse_sap_links = 1). */
 int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
 /* Autotuner result for level l >= 1: chosen coarse-SAP kernel variant
  * (2*systems_per_workgroup + batched_link_loads) and its measured half-sweep time. */

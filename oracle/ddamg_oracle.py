@@ -578,12 +578,15 @@ class SapParams:
 
 
 def sap(op: Stencil, blocks: SapBlocks, v: np.ndarray, x: np.ndarray, nu: int, tol: float,
-        prm: SapParams = SapParams(), stats: dict | None = None) -> tuple[np.ndarray, int]:
+        prm: SapParams = SapParams(), stats: dict | None = None,
+        block_op: Stencil | None = None) -> tuple[np.ndarray, int]:
     """Red-black multiplicative Schwarz, SAP_C::SAP src/sap.cpp:81-165 with the block solves
     of SAP_C::I_D_B_1_It :48-77 (GMRES(m) x restarts, tol, initial guess = block rhs).
 
     All red blocks see the same residual; the global residual is recomputed between
     colours and once more at the end of each iteration.  Returns (x, iterations).
+    block_op: the operator whose blocks the block solves invert, when it is not op (the
+    device's binary16 link copy in the FP32 hierarchy's smoother; the residuals keep op).
     """
     x = x.copy()
     v_norm = math.sqrt(np.real(np.vdot(v, v)))
@@ -593,7 +596,7 @@ def sap(op: Stencil, blocks: SapBlocks, v: np.ndarray, x: np.ndarray, nu: int, t
         for colour in (0, 1):
             ids = blocks.colour_ids(colour)
             rb = _gather_blocks(r, blocks, ids)
-            res = fgmres(block_operator(op, blocks, ids), None, rb, rb, prm.m, prm.restarts, prm.tol)
+            res = fgmres(block_operator(block_op or op, blocks, ids), None, rb, rb, prm.m, prm.restarts, prm.tol)
             if stats is not None:
                 stats.setdefault('block_iters', []).append(res.iterations.copy())
             x = _scatter_add_blocks(x, res.x, blocks, ids)

@@ -186,24 +186,35 @@ def test_coarse_sap_batched(precision):
                 assert rel(ctx.sap(1, v[r], x0[r], 2), ref) < 1e-11
 
 
+def _half(a):
+    """Each real and imaginary part rounded to IEEE binary16 (nearest, ties to even)."""
+    return a.real.astype(np.float16).astype(np.float64) + 1j * a.imag.astype(np.float16).astype(np.float64)
+
+
 @pytest.mark.parametrize('blk', [(4, 4), (2, 2)])
 def test_coarse_sap_matrix_cores(blk):
-    """FP32 coarse SAP on the matrix cores (variant 32, 16 systems per work-group).
+    """FP32 coarse SAP on the matrix cores (variant 32, 16 systems per work-group; variant
+    33, the same kernel on the binary16 link copy).
     (a) With a tolerance FP32 cannot reach, block GMRES runs all 5x5 iterations, so the
     control flow is fixed; against the oracle's FP64 SAP on the same (FP32) level-1
     operator the matrix-core result must be as accurate as the FP32 vector kernel
     (variant 2): error at most twice the vector kernel's (or 1e-5) and below 1e-3.
+    Variant 33 reads a copy of the links rounded to binary16 (checked bit for bit against
+    numpy's rounding of the FP32 links); it meets the same bound against the oracle's SAP
+    whose block solves use the rounded links (the residuals keep the FP32 ones), and stays
+    within 1e-2 of the FP32-link result.
     (b) Batches of 1, 3 and 17 systems (two 16-system tiles) give each system the same
     result as alone: the product columns and reductions of a system do not depend on
     the others."""
     rng = np.random.default_rng(33)
     N = 64
     U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.4)
-    out, ref = {}, None
+    out, ref, ref16 = {}, None, None
     v = x0 = None
-    for variant in (32, 2):
+    for variant in (32, 33, 2):
         prm = ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), blk], nv=[12, 8], m0=-0.2,
-                           precision=1, max_rhs=17, rng_ranks=(2, 2), sap_variant=variant)
+                           precision=1, max_rhs=17, rng_ranks=(2, 2), sap_variant=variant,
+                           coarse_sap_links=1)
         with ddamg.Context(N, N) as ctx:
             ctx.set_gauge(U)
             ctx.setup(prm)
@@ -214,17 +225,28 @@ def test_coarse_sap_matrix_cores(blk):
                 v = rng.standard_normal((17,) + shape) + 1j * rng.standard_normal((17,) + shape)
                 x0 = np.zeros_like(v)
                 A, B, Cm = ctx.links(1, precision=1)
-                op = orc.Stencil(A, B, Cm, -0.2)
                 blocks = orc.SapBlocks(shape[0], shape[1], *blk)
+                op = orc.Stencil(A, B, Cm, -0.2)
                 ref = np.stack([orc.sap(op, blocks, v[r], x0[r], 2, 1e-12)[0] for r in range(3)])
+                op16 = orc.Stencil(_half(A), _half(B), _half(Cm), -0.2)
+                ref16 = np.stack([orc.sap(op, blocks, v[r], x0[r], 2, 1e-12, block_op=op16)[0] for r in range(3)])
             out[variant] = ctx.sap(1, v[:3], x0[:3], 2, tol=1e-12, precision=1)
-            if variant == 32:
+            if variant == 33:   # the device's binary16 copy: the FP32 links rounded to nearest
+                A16, B16, C16 = ctx.links(1, precision=2)
+                A32, B32, C32 = ctx.links(1, precision=1)
+                assert np.array_equal(A16, _half(A32)) and np.array_equal(B16, _half(B32))
+                assert np.array_equal(C16, _half(C32))
+            if variant in (32, 33):
                 xb = ctx.sap(1, v, x0, 2, precision=1)
                 for r in (0, 5, 16):
                     assert rel(xb[r], ctx.sap(1, v[r], x0[r], 2, precision=1)) < 1e-6
     e32, e2 = rel(out[32], ref), rel(out[2], ref)
-    print(f'blocks {blk}: FP32 error vs FP64 oracle: matrix cores {e32:.2e}, vector kernel {e2:.2e}')
+    e33, d33 = rel(out[33], ref16), rel(out[33], out[32])
+    print(f'blocks {blk}: FP32 error vs FP64 oracle: matrix cores {e32:.2e}, vector kernel {e2:.2e}; '
+          f'binary16 links vs the oracle on rounded links {e33:.2e}, vs FP32 links {d33:.2e}')
     assert e32 < max(2 * e2, 1e-5) and e32 < 1e-3
+    assert e33 < max(2 * e2, 1e-5) and e33 < 1e-3
+    assert d33 < 1e-2
 
 
 @pytest.mark.parametrize('precision,basis', [(0, 0), (1, 0), (1, 1)])
