@@ -34,6 +34,7 @@ def log(*a):
 
 COARSE_BASIS = None   # None: the library default; 0 FP32, 1 binary16 (--coarse-basis)
 SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-sap-links)
+BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
 
 
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
@@ -43,6 +44,7 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
         p.coarse_basis = COARSE_BASIS
     if SAP_LINKS is not None:
         p.coarse_sap_links = SAP_LINKS
+    p.bootstrap = BOOTSTRAP
     return p
 
 
@@ -67,8 +69,11 @@ def main():
                     help='storage of the coarsest GMRES basis in the FP32 hierarchy (0 FP32, 1 binary16)')
     ap.add_argument('--coarse-sap-links', type=int, choices=[0, 1], default=None,
                     help='links read by the matrix-core coarse SAP of the FP32 hierarchy (0 FP32, 1 binary16 copy)')
+    ap.add_argument('--bootstrap', type=int, default=0,
+                    help='bootstrap passes after the setup (0: the reference setup only)')
     args = ap.parse_args()
-    global COARSE_BASIS, SAP_LINKS
+    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP
+    BOOTSTRAP = args.bootstrap
     COARSE_BASIS = args.coarse_basis
     SAP_LINKS = args.coarse_sap_links
     if args.gpus != 1:

@@ -135,6 +135,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->sap_variant = -1;
     p->coarse_basis = 1;
     p->coarse_sap_links = 0;
+    p->bootstrap = 0;
     return DD_OK;
 }
 

@@ -88,6 +88,13 @@ class Multigrid {
     Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const cplx<R>* U);
     // setup phase from raw level-0 test vectors [nv0][V0][2] (device, precision R)
     void setup(const cplx<R>* tv0, RankStreams& rng);
+    // bootstrap passes after setup: per level l (finest first) every test vector w (as
+    // it was before its orthonormalisation) is replaced by the current method's
+    // approximate solution of D x = w - D w (one cycle from level l, zero initial guess),
+    // the prolongator is rebuilt from the new vectors and the coarse operators of levels
+    // l+1 .. from the prolongators.  Needs a setup with p.bootstrap > 0 (which keeps
+    // those vectors).
+    void bootstrap(int passes);
     void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
@@ -127,6 +134,7 @@ class Multigrid {
     std::vector<int> cap_;
     const cplx<R>* U_;
     std::vector<DevMem> links_, W_, hlinks_;
+    std::vector<DevMem> tvraw_;   // smoothed test vectors [nv][n] before orthonormalisation (bootstrap)
     // per-level work: residual, restricted rhs / coarse solution of the next level,
     // prolongated correction, SAP residual and gates
     std::vector<DevMem> res_, eta_, psi_, sapr_, gate_, nrm_;

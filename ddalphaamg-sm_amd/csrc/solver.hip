@@ -298,6 +298,7 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
     cap_.resize(L);
     links_.resize(L);
     hlinks_.resize(L);
+    tvraw_.resize(L);
     W_.resize(L);
     res_.resize(L);
     eta_.resize(L);
@@ -505,6 +506,11 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
         zero<R>(s_, zero_rhs.as<cplx<R>>(), G.n() * G.nv, 0, 1, 1);
         // SAP(0, w, setup iterations) on all test vectors of the level at once
         sap(l, zero_rhs.as<cplx<R>>(), raw[l].as<cplx<R>>(), p_.setup_sap_iters, 1e-10, full_mask(G.nv));
+        if (p_.bootstrap > 0) {   // the bootstrap passes start from these vectors
+            tvraw_[l].alloc(sizeof(cplx<R>) * G.nv * G.n());
+            DD_HIP(hipMemcpyAsync(tvraw_[l].as<void>(), raw[l].as<void>(), sizeof(cplx<R>) * G.nv * G.n(),
+                                  hipMemcpyDeviceToDevice, s_));
+        }
         transpose_tv<R>(s_, raw[l].as<cplx<R>>(), W_[l].as<cplx<R>>(), G.V(), G.nv, G.dof, true);
         orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
         const cplx<R>* Lf = (l == 0) ? L0.as<cplx<R>>() : links(l);
@@ -514,6 +520,56 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
                     hlinks_[l + 1].bytes() / sizeof(_Float16));
         DD_HIP(hipStreamSynchronize(s_));
     }
+}
+
+// Bootstrap adaptivity as the reference's documentation describes it (AMG_Documentation,
+// section 3: "use the current multilevel method defined by P to approximately solve
+// D x = (w_i - D w_i) and update w_i <- x; the resultant test vectors are orthonormalized
+// again"; the reference declares the knob, AMGV::Nit include/variables.h:152, but its
+// AlgebraicMG::setUpPhase src/amg.cpp:4-62 does not use it).  The approximate solve is
+// one cycle of the configured kind from level l; test vectors go through it in batches
+// of the solver's capacity.  w_i are the test vectors as they were before their
+// orthonormalisation: the aggregate-wise orthonormalised ones are not smooth across
+// aggregates, and passes started from them degrade the prolongator.
+template <typename R>
+void Multigrid<R>::bootstrap(int passes) {
+    const int L = levels();
+    if (passes > 0 && !tvraw_[0].bytes()) throw BadShape("bootstrap needs a setup with bootstrap > 0");
+    const int cb = std::max(1, p_.max_rhs);
+    DevMem L0(sizeof(cplx<R>) * (size_t)g_[0].V() * 5 * 4);
+    dirac_links<R>(s_, U_, L0.as<cplx<R>>(), g_[0].Nx, g_[0].Nt);
+    auto rebuild = [&](int l0) {   // coarse operators of levels l0+1 .. L-1 from the current W
+        for (int l = l0; l < L - 1; l++) {
+            const LevelGeom& G = g_[l];
+            const cplx<R>* Lf = (l == 0) ? L0.as<cplx<R>>() : links(l);
+            galerkin<R>(s_, Lf, W(l), links_[l + 1].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, l > 0);
+            if (hlinks_[l + 1].bytes())
+                to_half(s_, reinterpret_cast<const float*>(links(l + 1)), hlinks_[l + 1].as<_Float16>(),
+                        hlinks_[l + 1].bytes() / sizeof(_Float16));
+        }
+    };
+    for (int it = 0; it < passes; it++)
+        for (int l = 0; l < L - 1; l++) {
+            const LevelGeom& G = g_[l];
+            const size_t n = G.n();
+            DevMem bb(sizeof(cplx<R>) * G.nv * n), xx(sizeof(cplx<R>) * G.nv * n);
+            cplx<R>* w = tvraw_[l].as<cplx<R>>();
+            cplx<R>* b = bb.as<cplx<R>>();
+            cplx<R>* x = xx.as<cplx<R>>();
+            for (int i0 = 0; i0 < G.nv; i0 += cb) {
+                const int c = std::min(cb, G.nv - i0);
+                const uint64_t mk = full_mask(c);
+                apply(l, w + i0 * n, b + i0 * n, w + i0 * n, mk);   // b = w - D w
+                zero<R>(s_, x + i0 * n, n, n, c, mk);
+                if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk);
+                else vcycle(l, b + i0 * n, x + i0 * n, mk);
+            }
+            DD_HIP(hipMemcpyAsync(w, x, sizeof(cplx<R>) * G.nv * n, hipMemcpyDeviceToDevice, s_));
+            transpose_tv<R>(s_, x, W_[l].as<cplx<R>>(), G.V(), G.nv, G.dof, true);
+            orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
+            rebuild(l);
+            DD_HIP(hipStreamSynchronize(s_));
+        }
 }
 
 template <typename R>
@@ -682,6 +738,7 @@ void Solver::setup(const dd_params& p) {
     if (p.precision == 0) {
         mg64.reset(new Multigrid<double>(s, Nx, Nt, p, U64.as<cplx<double>>()));
         mg64->setup(tv.as<cplx<double>>(), rng);
+        if (p.bootstrap > 0) mg64->bootstrap(p.bootstrap);
         if (p.sap_variant >= 0) mg64->set_sap_variant(p.sap_variant);
         else if (p.autotune) mg64->autotune();
     } else {
@@ -689,6 +746,7 @@ void Solver::setup(const dd_params& p) {
         convert<float, double>(s, tv32.as<cplx<float>>(), tv.as<cplx<double>>(), h.size(), 0, 0, 1, 1);
         mg32.reset(new Multigrid<float>(s, Nx, Nt, p, U32.as<cplx<float>>()));
         mg32->setup(tv32.as<cplx<float>>(), rng);
+        if (p.bootstrap > 0) mg32->bootstrap(p.bootstrap);
         if (p.sap_variant >= 0) mg32->set_sap_variant(p.sap_variant);
         else if (p.autotune) mg32->autotune();
     }

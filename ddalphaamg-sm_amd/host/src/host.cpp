@@ -352,8 +352,12 @@ AlgebraicMG::~AlgebraicMG() {
     dd_destroy(ctx);
 }
 
-void AlgebraicMG::setUpPhase(const int& /*Nit: unused by the reference, src/amg.cpp:4*/) {
+// Nit counts the setup passes (AMGV::Nit, default 1): the first is the reference's setup,
+// each further one a bootstrap pass (dd_params::bootstrap).  The reference declares the
+// knob but does not use it (src/amg.cpp:4).
+void AlgebraicMG::setUpPhase(const int& Nit) {
     dd_params p = make_params(m0, nu1, nu2, fgmres_m, fgmres_restarts, fgmres_tol, cycle);
+    p.bootstrap = Nit > 1 ? Nit - 1 : 0;
     check(dd_setup(ctx, &p), "dd_setup");
     for (Level* l : levels) delete l;
     levels.clear();

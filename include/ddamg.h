@@ -80,6 +80,12 @@ typedef struct dd_params {
                                       1 = a binary16 copy made at setup (each element
                                       within 2^-11 relative; the level operator keeps
                                       FP32).  Ignored by the FP64 hierarchy.         */
+    int bootstrap;                 /* bootstrap passes after the setup (0, the
+                                      reference's behaviour): each replaces every
+                                      level's test vectors w by one cycle's solution of
+                                      D x = w - D w, re-orthonormalises them and
+                                      rebuilds the coarse operators.  Needs max_rhs
+                                      systems of room per cycle (batches of max_rhs). */
 } dd_params;
 
 typedef struct dd_solve_info {

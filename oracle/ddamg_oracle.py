@@ -28,6 +28,13 @@ Pinning (what checks each function, see tests/test_oracle.py):
   cg_normal, bicgstab    golden iteration counts and solutions of the compiled
                          reference (g32_methods), dense solve, and CG's finite
                          termination in at most n steps on a 4x4 lattice.
+  AMG.bootstrap          parity unpinned: the reference does not implement it.
+                         Checked for what the mathematics fixes: zero passes change
+                         nothing, the new prolongators are orthonormal and the
+                         coarse operators their Galerkin products (dense brute
+                         force), and the test vectors move toward the low modes of
+                         D (smaller residual of the two-grid error propagator on
+                         them).
 """
 from __future__ import annotations
 
@@ -695,6 +702,7 @@ class Level:
     op: Stencil
     spec: LevelSpec | None       # None on the coarsest level
     W: np.ndarray | None = None  # orthonormal test vectors (prolongator)
+    V: np.ndarray | None = None  # the smoothed test vectors before orthonormalisation
 
 
 class AMG:
@@ -743,10 +751,40 @@ class AMG:
             W = tv[l]
             for cc in range(spec.nv):
                 W[cc], _ = sap(op, blocks, zero, W[cc], prm.setup_sap_iters, 1e-10, prm.sap)
+            V = W.copy()
             W = orthonormalize(W, spec.agg)
-            self.levels.append(Level(op, spec, W))
+            self.levels.append(Level(op, spec, W, V))
             op = coarse_links(op, W, spec.agg, prm.m0)
         self.levels.append(Level(op, None))
+
+    def bootstrap(self, passes: int, cycle: str = 'K') -> None:
+        """Bootstrap adaptivity as AMG_Documentation.pdf section 3 describes it ("use the
+        current multilevel method defined by P to approximately solve D x = (w_i - D w_i)
+        and update w_i <- x; the resultant test vectors are orthonormalized again"); the
+        reference declares AMGV::Nit (include/variables.h:152) but AlgebraicMG::setUpPhase
+        (src/amg.cpp:4-62) never uses it, so no reference output pins this function.
+
+        Per pass and per level l (finest first): w_i are the test vectors as they were
+        before their orthonormalisation (the aggregate-wise orthonormalised ones are not
+        smooth across aggregates; started from them the pass degrades the prolongator);
+        the approximate solve is one cycle from level l with zero initial guess; the new
+        vectors replace w_i, are orthonormalised per aggregate into the prolongator, and
+        the coarse operators of levels l+1 .. are rebuilt from the current prolongators.
+        """
+        cyc = self.k_cycle if cycle == 'K' else self.v_cycle
+        L = len(self.levels)
+        for _ in range(passes):
+            for l in range(L - 1):
+                lv = self.levels[l]
+                X = np.empty_like(lv.V)
+                for i in range(lv.V.shape[0]):
+                    w = lv.V[i]
+                    X[i] = cyc(l, w - lv.op.apply(w), np.zeros_like(w))
+                lv.V = X
+                lv.W = orthonormalize(X, lv.spec.agg)
+                for k in range(l, L - 1):
+                    lk = self.levels[k]
+                    self.levels[k + 1].op = coarse_links(lk.op, lk.W, lk.spec.agg, self.prm.m0)
 
     # -- cycles -----------------------------------------------------------------------
     def _count(self, key):

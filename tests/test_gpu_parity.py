@@ -249,6 +249,42 @@ def test_coarse_sap_matrix_cores(blk):
     assert d33 < 1e-2
 
 
+def test_bootstrap_matches_oracle(golden):
+    """Bootstrap adaptivity (dd_params.bootstrap = 2 passes; documented by the reference
+    but not implemented there) against the oracle's AMG.bootstrap on the same field and
+    raw test vectors: test vectors of both levels and the coarsest operator agree (FP64,
+    launch-per-step coarsest GMRES with modified Gram-Schmidt like the oracle; batches of
+    4 test vectors per cycle).  Then the FP64 solve on the bootstrapped hierarchy meets
+    1e-10, and the FP32 hierarchy bootstraps and solves too."""
+    meta, a = golden('g16_setup3')
+    U = np.exp(1j * a['theta'])
+    spec = [((4, 4), (4, 4), 4), ((2, 2), (2, 2), 4)]
+    amg = orc.AMG(U, orc.AmgParams(levels=[orc.LevelSpec(*l) for l in spec], m0=meta['m0'],
+                                   rng_ranks=tuple(meta['ranks'])))
+    amg.setup()
+    amg.bootstrap(2)
+    kw = dict(agg=[l[0] for l in spec], sap_block=[l[1] for l in spec], nv=[l[2] for l in spec], m0=meta['m0'],
+              rng_ranks=tuple(meta['ranks']), ortho_coarse=0, max_rhs=4, bootstrap=2)
+    with ddamg.Context(16, 16) as ctx:
+        ctx.set_gauge(U)
+        ctx.setup(ddamg.params(3, **kw))
+        for l in (0, 1):
+            e = rel(ctx.test_vectors(l), amg.levels[l].W)
+            print(f'level {l} test vectors vs oracle: {e:.2e}')
+            assert e < 1e-8
+        A, B, Cm = ctx.links(2)
+        op = amg.levels[2].op
+        assert rel(A, op.A) < 1e-8 and rel(B, op.B) < 1e-8 and rel(Cm, op.C) < 1e-8
+        x, its, info = ctx.solve(a['rhs'])
+        rr = np.linalg.norm(a['rhs'] - orc.D_phi(U, x, meta['m0'])) / np.linalg.norm(a['rhs'])
+        assert rr < 1e-10
+        ctx.setup(ddamg.params(3, precision=1, **kw))
+        x32, its32, _ = ctx.solve(a['rhs'])
+        rr32 = np.linalg.norm(a['rhs'] - orc.D_phi(U, x32, meta['m0'])) / np.linalg.norm(a['rhs'])
+        print(f'bootstrapped solves: FP64 {its[0]} iterations, mixed {its32[0]} iterations')
+        assert rr32 < 1e-10 and its32[0] <= its[0] + 3
+
+
 @pytest.mark.parametrize('precision,basis', [(0, 0), (1, 0), (1, 1)])
 def test_coarsest_persistent(precision, basis):
     """Coarsest-level GMRES in one persistent kernel (CGS with delayed

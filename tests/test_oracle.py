@@ -269,3 +269,53 @@ def test_cg_bicgstab_dense():
     assert rel(x, xd) < 1e-9 and (its > 0).all() and (its <= 48).all()
     x, its = orc.bicgstab(U, b, np.zeros_like(b), m0, 500, 1e-12)
     assert rel(x, xd) < 1e-9 and (its < 500).all()
+
+
+def _low_mode_defect(U, m0, W, agg, k=6):
+    """Mean of ||(1 - P P^dagger) v|| over the k right singular vectors of D_W with the
+    smallest singular values (unit norm): how well the prolongator's range holds the
+    near-kernel."""
+    D = orc.dense_D(U, m0)
+    _, _, vh = np.linalg.svd(D)
+    Nx, Nt = U.shape[:2]
+    out = []
+    for v in vh[-k:].conj():
+        vv = v.reshape(Nx, Nt, 2)
+        r = vv - orc.prolong(W, orc.restrict(W, vv, agg), agg)
+        out.append(np.linalg.norm(r))
+    return float(np.mean(out))
+
+
+def test_bootstrap_properties(golden):
+    """Bootstrap adaptivity (AMG.bootstrap, documented but not implemented by the
+    reference): zero passes change nothing; after two passes every prolongator is
+    orthonormal and every coarse operator is the Galerkin product of the new ones (dense
+    brute force), and the prolongator holds the low modes of D_W better (inverse-iteration
+    effect: the part of the six lowest right singular vectors outside its range shrinks by
+    more than 20 %)."""
+    meta, a = golden('g16_setup3')
+    U = np.exp(1j * a['theta'])
+    # aggregates of 4x4 sites with 4 test vectors: P has rank 4 of 16 per aggregate and spin
+    levels = [orc.LevelSpec((4, 4), (4, 4), 4), orc.LevelSpec((2, 2), (2, 2), 4)]
+    amg = orc.AMG(U, orc.AmgParams(levels=levels, m0=meta['m0'], rng_ranks=tuple(meta['ranks'])))
+    amg.setup()
+    W0 = [lv.W.copy() for lv in amg.levels[:-1]]
+    A0 = amg.levels[-1].op.A.copy()
+    amg.bootstrap(0)
+    assert all(np.array_equal(lv.W, w) for lv, w in zip(amg.levels[:-1], W0))
+    assert np.array_equal(amg.levels[-1].op.A, A0)
+    lv0 = amg.levels[0]
+    before = _low_mode_defect(U, meta['m0'], lv0.W, lv0.spec.agg)
+    amg.bootstrap(2)
+    after = _low_mode_defect(U, meta['m0'], lv0.W, lv0.spec.agg)
+    print(f'low-mode defect of P: {before:.3e} -> {after:.3e}')
+    assert after < 0.8 * before
+    assert rel(amg.levels[0].W, W0[0]) > 1e-3
+    rng = np.random.default_rng(4)
+    for l in range(len(amg.levels) - 1):
+        lv, nxt = amg.levels[l], amg.levels[l + 1]
+        shape = nxt.op.shape + (nxt.op.dof,)
+        vc = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        Pv = orc.prolong(lv.W, vc, lv.spec.agg)
+        assert rel(orc.restrict(lv.W, Pv, lv.spec.agg), vc) < 1e-13
+        assert rel(orc.restrict(lv.W, lv.op.apply(Pv), lv.spec.agg), nxt.op.apply(vc)) < 1e-12
