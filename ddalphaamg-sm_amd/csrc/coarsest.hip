@@ -69,12 +69,16 @@ struct Ctrl {
 // kernel ends; the host reports it.  All fields are zero when a launch starts.
 constexpr unsigned kBarSpins = 1u << 25;
 
+// Each word that work-groups update or poll sits in its own 256-byte region (one memory
+// channel's interleave unit), so that the barrier's atomics, the polling of its generation
+// and the flag polls do not queue behind each other at one channel.
 struct GridBar {
     unsigned cnt;
-    unsigned pad0[15];
+    unsigned pad0[63];
     unsigned gen;
+    unsigned pad1[63];
     unsigned fail;
-    unsigned pad1[14];
+    unsigned pad2[63];
     // per system: (Arnoldi step << 1) | stop, stored by the system's owner with release
     // once the step's update coefficients are written (see wait_coefs)
     unsigned cflag[64];
@@ -103,7 +107,8 @@ __device__ __forceinline__ bool grid_barrier(GridBar* b, unsigned& gen, int wg, 
         }
         unsigned spins = 0;
         while (__hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-            if (__hip_atomic_load(&b->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > kBarSpins) {
+            if ((++spins & 63u) == 0 &&
+                (__hip_atomic_load(&b->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kBarSpins)) {
                 __hip_atomic_store(&b->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -568,8 +573,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (nbw >= 0) {
                 unsigned spins = 0;
                 while (__hip_atomic_load(&a.bar->udone[nbw], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
-                    if (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                        ++spins > kBarSpins) {
+                    if ((++spins & 63u) == 0 &&
+                        (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                         spins > kBarSpins)) {
                         __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ok = false;
                         break;
@@ -598,8 +604,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (need) {
                 while (((v = __hip_atomic_load(&a.bar->cflag[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 1) !=
                        gstep) {
-                    if (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                        ++spins > kBarSpins) {
+                    if ((++spins & 63u) == 0 &&
+                        (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                         spins > kBarSpins)) {
                         __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ok = false;
                         break;
