@@ -68,6 +68,7 @@ struct Ctrl {
 // polls (seconds) raises `fail`, which every waiting work-group also watches, and the
 // kernel ends; the host reports it.  All fields are zero when a launch starts.
 constexpr unsigned kBarSpins = 1u << 25;
+constexpr int kFlagStride = 16;   // words between two flags (each flag in its own 64-byte line)
 
 // Each word that work-groups update or poll sits in its own 256-byte region (one memory
 // channel's interleave unit), so that the barrier's atomics, the polling of its generation
@@ -81,10 +82,10 @@ struct GridBar {
     unsigned pad2[63];
     // per system: (Arnoldi step << 1) | stop, stored by the system's owner with release
     // once the step's update coefficients are written (see wait_coefs)
-    unsigned cflag[64];
+    unsigned cflag[64 * kFlagStride];
     // per work-group: the last Arnoldi step whose update pass it has finished (release),
     // read by the work-groups whose stencil needs its rows (see wait_neighbours)
-    unsigned udone[kGmax];
+    unsigned udone[kGmax * kFlagStride];
 };
 
 // Executed by every wave that stored data before the work-group barrier that precedes a
@@ -164,9 +165,78 @@ struct CArgs {
     double* wgclk;          // [G][9] per-work-group phase clocks (diagnostics), or null
     uint64_t mask;
     int Nx, Nt, nsys, m, restarts, spw;
+    int lds_links;          // 1: the slice's links are cached in LDS (after the scalar state)
     R m0;
     double tol;
 };
+
+// LDS of the kernel: the scalar state of the owned system and scratch, then (lds_links)
+// the links of the slice's sites, 16-byte aligned
+__host__ __device__ constexpr size_t coarsest_lds(int m) {
+    return ((sizeof(cplx<double>) * ((size_t)(7 * m + 5) + kBB * kBB) + 64) + 15) & ~(size_t)15;
+}
+
+// One site of the level stencil (as coarse_site in mfma.h, one 16-system tile) with the
+// site's links in LDS (Ls: [5][DOF][DOF] column-major complex, as reals): the five hops'
+// vector loads and the own rows are issued together, then the products take their A
+// operands from LDS, so the site costs one global-load latency instead of five.
+template <typename R, int DOF, bool RES>
+__device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
+                                                const cplx<R>* __restrict__ b, int Nx, int Nt, R m0, uint64_t mask,
+                                                size_t ld, int nrhs, int n, int tile, int lane) {
+    using acc_t = typename Mfma<R>::acc_t;
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+    const int g = lane >> 4;
+    const int jj = tile * 16 + (lane & 15);
+    const bool on = jj < nrhs && ((mask >> jj) & 1ull);
+    const size_t sysoff = (size_t)(on ? jj : 0) * ld;
+    const R* vin = reinterpret_cast<const R*>(in + sysoff);
+    const int x = n / Nt, t = n - x * Nt;
+    const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
+                       (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
+    const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+    R bv[5][1][KS];
+#pragma unroll
+    for (int h = 0; h < 5; h++) {
+        const R* vb = vin + (size_t)nb[h] * DOF * 2 + g;
+#pragma unroll
+        for (int s2 = 0; s2 < KS; s2++) bv[h][0][s2] = vb[4 * s2];
+    }
+    const size_t base = sysoff + (size_t)n * DOF;
+    cplx<R> own[MT][2], rhs[MT][2];
+#pragma unroll
+    for (int tt = 0; tt < MT; tt++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            own[tt][e] = in[base + 8 * tt + 2 * g + e];
+            if constexpr (RES) rhs[tt][e] = b[base + 8 * tt + 2 * g + e];
+            else rhs[tt][e] = czero<R>();
+        }
+#pragma unroll
+    for (int h = 0; h < 5; h++)
+#pragma unroll
+        for (int s2 = 0; s2 < KS; s2++) bv[h][0][s2] = on ? sg[h] * bv[h][0][s2] : R(0);
+    const LinkLane ll = LinkLane::make<R, DOF>(lane);
+    acc_t acc[1][MT];
+#pragma unroll
+    for (int tt = 0; tt < MT; tt++) acc[0][tt] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int h = 0; h < 5; h++) mfma_hop_nt<R, DOF, 1>(acc, Ls + (size_t)h * DOF * DOF * 2, ll, bv[h]);
+    if (!on) return;
+    const R mass = m0 + R(2);
+#pragma unroll
+    for (int tt = 0; tt < MT; tt++) {
+        const int r0 = 8 * tt + 2 * g;
+        cplx<R> o0 = mk<R>(mass * own[tt][0].x - acc[0][tt][0], mass * own[tt][0].y - acc[0][tt][1]);
+        cplx<R> o1 = mk<R>(mass * own[tt][1].x - acc[0][tt][2], mass * own[tt][1].y - acc[0][tt][3]);
+        if (RES) {
+            o0 = rhs[tt][0] - o0;
+            o1 = rhs[tt][1] - o1;
+        }
+        out[base + r0] = o0;
+        out[base + r0 + 1] = o1;
+    }
+}
 
 __device__ __forceinline__ cplx<double> cdivr(cplx<double> a, double d) { return mk<double>(a.x / d, a.y / d); }
 
@@ -253,6 +323,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     const int nrow = max(0, s1 - s0) * DOF;   // at most LPS * RPL
     const size_t row0 = (size_t)s0 * DOF;
     const int ntile = (a.nsys + 15) / 16;
+    const R* lnk = reinterpret_cast<const R*>(smem_raw + coarsest_lds(m));   // lds_links: [spw][5][DOF][DOF]
     const double sR = sizeof(cplx<R>), sQ = sizeof(S);
     // the lane's system within a wave's pair (h) and its rows lb..lb+RPL-1 (lanes past the
     // slice read its last rows, masked)
@@ -328,6 +399,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             for (int u = wv; u < ns * ntile; u += kNW) {
                 const int site = s0 + u % ns, tl = u / ns;
                 if (!((mk2 >> (16 * tl)) & 0xffffull)) continue;
+                if constexpr (sizeof(R) == 4 && DOF <= 24) {   // the cases whose links fit the LDS
+                    if (a.lds_links) {
+                        const R* Ls = lnk + (size_t)(site - s0) * 5 * DOF * DOF * 2;
+                        if (res)
+                            coarse_site_lds<R, DOF, true>(Ls, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                                          lane);
+                        else
+                            coarse_site_lds<R, DOF, false>(Ls, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site,
+                                                           tl, lane);
+                        continue;
+                    }
+                }
                 if (res)
                     coarse_site<R, DOF, true, 1>(a.L, in, out, b, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane);
                 else
@@ -556,7 +639,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         __syncthreads();
         // publish: every thread's coefficient stores precede thread 0's release
         if (tid == 0)
-            __hip_atomic_store(&a.bar->cflag[q], (gstep << 1) | (stop_now ? 1u : 0u), __ATOMIC_RELEASE,
+            __hip_atomic_store(&a.bar->cflag[q * kFlagStride], (gstep << 1) | (stop_now ? 1u : 0u), __ATOMIC_RELEASE,
                                __HIP_MEMORY_SCOPE_AGENT);
     };
     // Waits (wave 0, a lane per neighbour) until every work-group whose rows this slice's
@@ -572,7 +655,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             bool ok = true;
             if (nbw >= 0) {
                 unsigned spins = 0;
-                while (__hip_atomic_load(&a.bar->udone[nbw], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
+                while (__hip_atomic_load(&a.bar->udone[nbw * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
                     if ((++spins & 63u) == 0 &&
                         (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                          spins > kBarSpins)) {
@@ -602,7 +685,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             unsigned v = 0, spins = 0;
             bool ok = true;
             if (need) {
-                while (((v = __hip_atomic_load(&a.bar->cflag[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 1) !=
+                while (((v = __hip_atomic_load(&a.bar->cflag[q * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 1) !=
                        gstep) {
                     if ((++spins & 63u) == 0 &&
                         (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
@@ -807,6 +890,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (!((a.mask >> q) & 1ull)) continue;
         for (int row = lane; row < nrow; row += 64) a.x[(size_t)q * n + row0 + row] = a.eta[(size_t)q * n + row0 + row];
     }
+    // the slice's links into LDS, read by every stencil of the launch (the barrier below
+    // orders these stores before the first read)
+    if (a.lds_links) {
+        const float4* src = reinterpret_cast<const float4*>(a.L + (size_t)s0 * 5 * DOF * DOF);
+        float4* dst = reinterpret_cast<float4*>(smem_raw + coarsest_lds(m));
+        const int nq = (int)((size_t)(s1 - s0) * 5 * DOF * DOF * sizeof(cplx<R>) / 16);
+        for (int i = tid; i < nq; i += kCT) dst[i] = src[i];
+    }
     if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
     stencil(a.x, a.u, a.eta, a.mask, true);
     __syncthreads();
@@ -870,7 +961,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             // publish this slice's update (every thread's stores precede thread 0's release)
             release_stores();
             __syncthreads();
-            if (tid == 0) __hip_atomic_store(&a.bar->udone[wg], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) __hip_atomic_store(&a.bar->udone[wg * kFlagStride], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             clk(5);
         }
         if (owner && ((live >> sys) & 1ull)) backsub();
@@ -901,6 +992,12 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
 
 template <typename R, typename S, int DOF, int LPS>
 void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
+    static bool attr = false;   // dynamic LDS above 64 KB (links cached in LDS)
+    if (!attr) {
+        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
     void* args[] = {const_cast<CArgs<R, S>*>(&a)};
     DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS>), dim3(G), dim3(kCT), args,
                                       (unsigned)sh, s));
@@ -918,7 +1015,6 @@ int coresident_workgroups() {
     return coop ? ncu : 0;
 }
 
-size_t coarsest_lds(int m) { return sizeof(cplx<double>) * ((size_t)(7 * m + 5) + kBB * kBB) + 64; }
 
 }  // namespace
 
@@ -1008,7 +1104,10 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     a.spw = spw_;
     a.m0 = (R)m0;
     a.tol = tol_;
-    const size_t sh = coarsest_lds(m_);
+    // FP32 slices of up to 4 sites keep their links in LDS when they fit beside the state
+    const size_t lbytes = (size_t)spw_ * 5 * dof_ * dof_ * sizeof(cplx<R>);
+    a.lds_links = (sizeof(R) == 4 && dof_ <= 24 && spw_ <= 4 && coarsest_lds(m_) + lbytes <= 160 * 1024) ? 1 : 0;
+    const size_t sh = coarsest_lds(m_) + (a.lds_links ? lbytes : 0);
     // one cooperative grid in flight per process: two grids that each hold part of the
     // device could wait on each other's barriers (contexts solving from several threads)
     static std::mutex coop_mutex;
