@@ -86,6 +86,9 @@ struct GridBar {
     // per work-group: the last Arnoldi step whose update pass it has finished (release),
     // read by the work-groups whose stencil needs its rows (see wait_neighbours)
     unsigned udone[kGmax * kFlagStride];
+    // per work-group: the last Arnoldi step whose partial sums it has written (release),
+    // read by the owners of the running systems (see wait_partials)
+    unsigned pdone[kGmax * kFlagStride];
 };
 
 // Executed by every wave that stored data before the work-group barrier that precedes a
@@ -672,6 +675,37 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         }
         __syncthreads();
     };
+    // Waits (wave 0, four work-groups per lane) until every work-group has written its
+    // partial sums of Arnoldi step gstep, then acquires them for this (owner) work-group;
+    // si[2] = 0 when the kernel must end.  Every work-group publishes its partials after
+    // its stencil and reduction of the step, which follow its update of the previous
+    // step: once an owner has seen all of them, no work-group still reads the
+    // coefficients or the control flags the owner is about to rewrite, nor the rows of
+    // u that their next updates overwrite.
+    auto wait_partials = [&](unsigned gstep) {
+        __syncthreads();
+        if (wv == 0) {
+            bool ok = true;
+            for (int k = lane; k < G && ok; k += 64) {
+                unsigned spins = 0;
+                while (__hip_atomic_load(&a.bar->pdone[k * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       gstep) {
+                    if ((++spins & 63u) == 0 &&
+                        (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                         spins > kBarSpins)) {
+                        __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const uint64_t bad = __ballot(!ok);
+            if (lane == 0) si[2] = bad ? 0 : 1;
+        }
+        __syncthreads();
+    };
     // Waits (wave 0, a lane per system) until the owner of every system in `run` has
     // published step gstep's coefficients, then acquires them for the work-group.  Returns
     // the systems that continue (the owners' stop bits cleared), identical in every
@@ -943,9 +977,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 if (pm) reduce_step(qb, j, hw, pm);
             }
             clk(0);
-            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
-            clk(1);
+            // publish this slice's partial sums of the step; only the owners of running
+            // systems wait for every work-group's (a grid barrier's worth of ordering for
+            // them, none for the rest)
             ++gstep;
+            release_stores();
+            __syncthreads();
+            if (tid == 0) __hip_atomic_store(&a.bar->pdone[wg * kFlagStride], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (owner && ((run >> sys) & 1ull)) {
+                wait_partials(gstep);
+                if (!si[2]) return;
+            }
+            clk(1);
             if (owner && ((run >> sys) & 1ull)) phaseB(j, gstep);
             clk(2);
             const uint64_t cont = wait_coefs(run, gstep);
