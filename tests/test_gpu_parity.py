@@ -330,6 +330,35 @@ def test_coarsest_persistent(precision, basis):
                 assert np.linalg.norm(res) < 0.1 * np.linalg.norm(v[r]) * (1 + 1e-5)
 
 
+def test_coarsest_persistent_64_systems():
+    """The persistent coarsest GMRES at its largest batch (64 systems: every lane of the
+    coefficient-flag wait and both halves of the 64-bit system masks in use), FP32 with the
+    binary16 basis, one site per work-group (64 work-groups, one per system): every
+    system equals the dense solve to 1e-4 at a tight tolerance, and systems 0, 33 and 63
+    equal their single-system solves."""
+    rng = np.random.default_rng(64)
+    N = 64
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.5)
+    with ddamg.Context(N, N) as ctx:
+        ctx.set_gauge(U)
+        ctx.setup(ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[8, 8], m0=-0.2,
+                               precision=1, max_rhs=64, rng_ranks=(2, 2), coarse_tol=1e-6, ortho_coarse=2))
+        assert ctx.coarsest_stats()['persistent'] == 1
+        shape = ctx.level_shape(2)
+        v = rng.standard_normal((64,) + shape) + 1j * rng.standard_normal((64,) + shape)
+        x = ctx.cycle(2, v, np.zeros_like(v), kcycle=True, precision=1)
+        A, B, Cm = ctx.links(2, precision=1)
+        op = orc.Stencil(A, B, Cm, -0.2)
+        nn = int(np.prod(shape))
+        D = np.stack([op.apply(e.reshape(shape)).ravel() for e in np.eye(nn)], axis=1)
+        xd = np.linalg.solve(D, v.reshape(64, nn).T).T.reshape(v.shape)
+        errs = [rel(x[r], xd[r]) for r in range(64)]
+        print(f'64 systems: max error vs dense solve {max(errs):.2e}')
+        assert max(errs) < 1e-4
+        for r in (0, 33, 63):
+            assert rel(ctx.cycle(2, v[r], np.zeros_like(v[r]), kcycle=True, precision=1), x[r]) < 1e-12
+
+
 def test_ragged_batch_equals_separate_solves():
     """dd_solve_many_dev: several lattices of different sizes (and masses), each with its
     own hierarchy and batch of right-hand sides, solved at the same time give every system
