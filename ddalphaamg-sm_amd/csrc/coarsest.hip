@@ -789,15 +789,19 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             // its most recent vectors are still in the last-level cache
             // chunks [i0, i0 + NC) from the newest vectors down: full chunks of kCU, the
             // oldest remainder in groups of 4 (indices below 0 masked)
+            // coefficients of a chunk: lane li of each lane group loads those of vector i0 + li
+            // (NC <= LPS), the group's lanes take them from there (cross-lane reads instead of
+            // two loads per vector and lane)
+            const int gbase = lane & ~(LPS - 1);
             auto chunk = [&](auto nc_, int i0) {
                 constexpr int NC = decltype(nc_)::value;
+                static_assert(NC <= LPS, "one coefficient per lane of the group");
                 S qv[NC][RPL];
-                cplx<R> ai[NC], ci[NC];
+                const int icl = max(0, min(i0 + min(li, NC - 1), j - 1));
+                const cplx<R> al = cf[icl], cl = cf[m + icl];
 #pragma unroll
                 for (int ii = 0; ii < NC; ii++) {
                     const int ic = max(0, min(i0 + ii, j - 1));
-                    ai[ii] = cf[ic];
-                    ci[ii] = cf[m + ic];
                     const S* qi = a.Q + (size_t)ic * vs + (size_t)q * n + row0 + lb;
 #pragma unroll
                     for (int k = 0; k < RPL; k++) qv[ii][k] = qi[k];
@@ -805,7 +809,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
 #pragma unroll
                 for (int ii = 0; ii < NC; ii++) {
                     const bool in = i0 + ii >= 0 && i0 + ii < j;
-                    const cplx<R> aa = in ? ai[ii] : czero<R>(), cc = in ? ci[ii] : czero<R>();
+                    const cplx<R> ab = mk<R>(__shfl(al.x, gbase + ii, 64), __shfl(al.y, gbase + ii, 64));
+                    const cplx<R> cb = mk<R>(__shfl(cl.x, gbase + ii, 64), __shfl(cl.y, gbase + ii, 64));
+                    const cplx<R> aa = in ? ab : czero<R>(), cc = in ? cb : czero<R>();
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
                         const cplx<R> qq = widen(qv[ii][k]);
