@@ -183,6 +183,12 @@ __host__ __device__ constexpr size_t coarsest_lds(int m) {
 // site's links in LDS (Ls: [5][DOF][DOF] column-major complex, as reals): the five hops'
 // vector loads and the own rows are issued together, then the products take their A
 // operands from LDS, so the site costs one global-load latency instead of five.
+// The K index of the 16x16x4 products is ordered k = KS*g + s (lane group g, K-step s;
+// mfma.h uses k = 4s + g; any bijection works when A and B agree), so a lane's B operand
+// is KS consecutive reals of the neighbour's vector: KS/4 16-byte loads per hop.  The A
+// element (row m = 2r + p, k = 2c + q) is Lr for p == q, Li for p != q, negated for
+// (p, q) = (0, 1); with c = (KS/2) g + s/2 and q = s & 1 its LDS offset is a lane part
+// plus a constant per (s, t).  FP32 only (the accumulator rows m - 16t = i of mfma.h).
 template <typename R, int DOF, bool RES>
 __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
                                                 const cplx<R>* __restrict__ b, int Nx, int Nt, R m0, uint64_t mask,
@@ -198,12 +204,19 @@ __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __re
     const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
                        (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
     const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
-    R bv[5][1][KS];
+    static_assert(sizeof(R) == 4 && KS % 4 == 0, "FP32, K-steps in groups of four reals");
+    R bv[5][KS];
 #pragma unroll
     for (int h = 0; h < 5; h++) {
-        const R* vb = vin + (size_t)nb[h] * DOF * 2 + g;
+        const float4* vb = reinterpret_cast<const float4*>(vin + (size_t)nb[h] * DOF * 2 + KS * g);
 #pragma unroll
-        for (int s2 = 0; s2 < KS; s2++) bv[h][0][s2] = vb[4 * s2];
+        for (int s4 = 0; s4 < KS / 4; s4++) {
+            const float4 v4 = vb[s4];
+            bv[h][4 * s4] = v4.x;
+            bv[h][4 * s4 + 1] = v4.y;
+            bv[h][4 * s4 + 2] = v4.z;
+            bv[h][4 * s4 + 3] = v4.w;
+        }
     }
     const size_t base = sysoff + (size_t)n * DOF;
     cplx<R> own[MT][2], rhs[MT][2];
@@ -218,13 +231,29 @@ __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __re
 #pragma unroll
     for (int h = 0; h < 5; h++)
 #pragma unroll
-        for (int s2 = 0; s2 < KS; s2++) bv[h][0][s2] = on ? sg[h] * bv[h][0][s2] : R(0);
-    const LinkLane ll = LinkLane::make<R, DOF>(lane);
+        for (int s2 = 0; s2 < KS; s2++) bv[h][s2] = on ? sg[h] * bv[h][s2] : R(0);
+    // lane part of the A offsets: row r = 8t + i/2, p = i & 1; columns (KS/2) g + s/2
+    const int i = lane & 15, p = i & 1;
+    const int lbase = 2 * ((KS / 2) * g * DOF + (i >> 1));
+    const int oe = lbase + p, oo = lbase + (1 - p);   // even / odd K-steps (q = 0 / 1)
+    const bool negodd = p == 0;
     acc_t acc[1][MT];
 #pragma unroll
     for (int tt = 0; tt < MT; tt++) acc[0][tt] = acc_t{0, 0, 0, 0};
 #pragma unroll
-    for (int h = 0; h < 5; h++) mfma_hop_nt<R, DOF, 1>(acc, Ls + (size_t)h * DOF * DOF * 2, ll, bv[h]);
+    for (int h = 0; h < 5; h++) {
+        const R* Lh = Ls + (size_t)h * DOF * DOF * 2;
+#pragma unroll
+        for (int s2 = 0; s2 < KS; s2++) {
+#pragma unroll
+            for (int tt = 0; tt < MT; tt++) {
+                const int c = 2 * ((s2 >> 1) * DOF + 8 * tt);
+                R av = (s2 & 1) ? Lh[oo + c] : Lh[oe + c];
+                if ((s2 & 1) && negodd) av = -av;
+                acc[0][tt] = Mfma<R>::mac(av, bv[h][s2], acc[0][tt]);
+            }
+        }
+    }
     if (!on) return;
     const R mass = m0 + R(2);
 #pragma unroll
