@@ -1082,9 +1082,10 @@ void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
 }
 
 // Work-groups that are certainly co-resident: the kernel is bounded to 512 threads
-// (at most 256 VGPRs per lane, two waves per SIMD) and its LDS request is checked to be
-// at most 64 KB, so every CU holds one work-group.  0 when the device cannot launch
-// cooperatively.
+// (at most 256 VGPRs per lane, two waves per SIMD) and its LDS request (the scalar state,
+// checked to be at most 64 KB, plus the slice's links when they are cached, at most
+// 160 KB in all) fits one compute unit, so every CU holds one work-group.  0 when the
+// device cannot launch cooperatively.
 int coresident_workgroups() {
     int dev = 0, ncu = 0, coop = 0;
     DD_HIP(hipGetDevice(&dev));
