@@ -136,6 +136,7 @@ int dd_default_params(dd_params* p, int levels) {
     p->coarse_basis = 1;
     p->coarse_sap_links = 0;
     p->bootstrap = 0;
+    p->graph = 0;
     return DD_OK;
 }
 
@@ -429,6 +430,15 @@ int dd_coarsest_stats(dd_ctx* c, double* out) {
         };
         if (c->s->mg64) fill(c->s->mg64.get());
         else fill(c->s->mg32.get());
+        return DD_OK;
+    });
+}
+
+int dd_graph_count(dd_ctx* c, int* count) {
+    if (check(c) || !count) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
+        *count = c->s->mg64 ? c->s->mg64->graph_count() : c->s->mg32->graph_count();
         return DD_OK;
     });
 }

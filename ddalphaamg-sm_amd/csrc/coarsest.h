@@ -27,6 +27,12 @@ class CoarsestGmres {
     bool supported() const { return ok_; }
     // psi = D_c^-1 eta for the systems in mask (x0 = eta); iters[r] per system
     void solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int* iters);
+    // deferred: solve() only enqueues device work (no host read-back, no stream
+    // synchronisation; iters and the statistics are not updated), so that it can be
+    // captured in a graph.  A timed-out wait is then reported by check_failure().
+    bool deferred = false;
+    // throws HipFailure when any launch since the last call timed out (synchronises)
+    void check_failure();
     long long steps = 0;   // Arnoldi steps taken (cumulative)
     // restart length (at most the constructed one), cycles and tolerance of later solves
     void set_limits(int m, int restarts, double tol) {
@@ -47,6 +53,7 @@ class CoarsestGmres {
     double tol_;
     bool half_ = false, ok_ = false;
     DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_, bar_, wgclk_;
+    DevMem sticky_;   // 1 once a launch's fail flag was set (kept across launches)
     long long solves_ = 0;
     PinMem ctrlh_;
 };

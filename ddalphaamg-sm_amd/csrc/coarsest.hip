@@ -1068,6 +1068,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     }
 }
 
+// Clears the grid-barrier state before a launch (a kernel, not a memset, so that graph
+// captures hold no memset node), after folding the previous launch's fail flag into the
+// sticky word.
+__global__ __launch_bounds__(256) void k_bar_reset(GridBar* __restrict__ b, unsigned* __restrict__ sticky) {
+    __shared__ unsigned failed;
+    if (threadIdx.x == 0) failed = b->fail;
+    __syncthreads();
+    if (threadIdx.x == 0 && failed) *sticky = 1u;
+    unsigned* w = reinterpret_cast<unsigned*>(b);
+    for (size_t i = threadIdx.x; i < sizeof(GridBar) / sizeof(unsigned); i += blockDim.x) w[i] = 0u;
+}
+
 template <typename R, typename S, int DOF, int LPS>
 void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
     static bool attr = false;   // dynamic LDS above 64 KB (links cached in LDS)
@@ -1129,8 +1141,30 @@ CoarsestGmres<R>::CoarsestGmres(hipStream_t s, int Nx, int Nt, int dof, int m, i
     nrm_.alloc(sizeof(double) * (2 * cap + 12));
     ctrl_.alloc(sizeof(Ctrl) * cap);
     bar_.alloc(sizeof(GridBar));
-    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 12 + sizeof(GridBar));
+    sticky_.alloc(sizeof(unsigned));
+    ctrlh_.alloc(sizeof(Ctrl) * cap + sizeof(double) * 12 + sizeof(GridBar) + sizeof(unsigned));
+    DD_HIP(hipMemsetAsync(bar_.as<void>(), 0, sizeof(GridBar), s_));
+    DD_HIP(hipMemsetAsync(sticky_.as<void>(), 0, sizeof(unsigned), s_));
+    DD_HIP(hipStreamSynchronize(s_));
     ok_ = true;
+}
+
+template <typename R>
+void CoarsestGmres<R>::check_failure() {
+    if (!ok_) return;
+    // fold the last launch's flag in, then read the sticky word
+    k_bar_reset<<<1, 256, 0, s_>>>(bar_.as<GridBar>(), sticky_.as<unsigned>());
+    DD_HIP(hipGetLastError());
+    unsigned* h = reinterpret_cast<unsigned*>(reinterpret_cast<unsigned char*>(ctrlh_.as<void>()) + sizeof(Ctrl) * cap_ +
+                                              sizeof(double) * 12 + sizeof(GridBar));
+    DD_HIP(hipMemcpyAsync(h, sticky_.as<void>(), sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipStreamSynchronize(s_));
+    if (*h) {
+        DD_HIP(hipMemsetAsync(sticky_.as<void>(), 0, sizeof(unsigned), s_));
+        DD_HIP(hipStreamSynchronize(s_));
+        fprintf(stderr, "coarsest GMRES: grid barrier timed out\n");
+        throw HipFailure(hipErrorLaunchTimeOut);
+    }
 }
 
 template <typename R>
@@ -1169,11 +1203,11 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     a.stats = nrm_.as<double>() + 2 * cap_;
     a.bar = bar_.as<GridBar>();
     static const bool verbose = std::getenv("DDAMG_VERBOSE") != nullptr;
-    if (verbose && !wgclk_.bytes()) {
+    if (verbose && !deferred && !wgclk_.bytes()) {
         wgclk_.alloc(sizeof(double) * 9 * G_);
         DD_HIP(hipMemsetAsync(wgclk_.as<void>(), 0, sizeof(double) * 9 * G_, s_));
     }
-    a.wgclk = verbose ? wgclk_.as<double>() : nullptr;
+    a.wgclk = (verbose && !deferred) ? wgclk_.as<double>() : nullptr;
     a.mask = mask;
     a.Nx = Nx_;
     a.Nt = Nt_;
@@ -1191,7 +1225,8 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     // device could wait on each other's barriers (contexts solving from several threads)
     static std::mutex coop_mutex;
     std::lock_guard<std::mutex> lock(coop_mutex);
-    DD_HIP(hipMemsetAsync(a.bar, 0, sizeof(GridBar), s_));
+    k_bar_reset<<<1, 256, 0, s_>>>(a.bar, sticky_.as<unsigned>());
+    DD_HIP(hipGetLastError());
     timer_begin(s_, K_COARSEST);
     switch (dof_) {
 #define DD_CASE(D)                                                   \
@@ -1205,6 +1240,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
             throw BadShape("coarsest GMRES: unsupported dof");
     }
     timer_end(s_, K_COARSEST, 0.0);
+    if (deferred) return;   // device work only (graph capture); failures via check_failure()
     Ctrl* ch = ctrlh_.as<Ctrl>();
     double* st = reinterpret_cast<double*>(ch + cap_);
     DD_HIP(hipMemcpyAsync(ch, a.ctrl, sizeof(Ctrl) * nsys, hipMemcpyDeviceToHost, s_));
@@ -1216,7 +1252,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
         fprintf(stderr, "coarsest GMRES: grid barrier timed out\n");
         throw HipFailure(hipErrorLaunchTimeOut);
     }
-    if (verbose && ++solves_ % 19 == 0) {   // spread of the phase times over the work-groups
+    if (verbose && wgclk_.bytes() && ++solves_ % 19 == 0) {   // spread of the phase times over the work-groups
         std::vector<double> h((size_t)9 * G_);
         DD_HIP(hipMemcpy(h.data(), a.wgclk, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
         const char* nm[9] = {"reduce", "barrier1", "scalar", "barrier2", "update", "barrier3", "cycle", "stencil",

@@ -19,6 +19,8 @@ void timer_begin(hipStream_t s, int id);
 void timer_end(hipStream_t s, int id, double bytes);
 // adds algorithmic bytes to kernel `id` after the fact (kernels that count their own work)
 void timer_add_bytes(int id, double bytes);
+// true when any kernel timer is on (their event records stay out of graph captures)
+bool timer_any();
 // total elapsed ms, number of timed launches and algorithmic bytes of kernel `id`
 // since the last reset (synchronises the events)
 double timer_read(int id, long long* count, double* bytes);

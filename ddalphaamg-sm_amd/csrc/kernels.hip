@@ -62,6 +62,11 @@ void timer_end(hipStream_t s, int id, double bytes) {
     t.used += 2;
     t.bytes += bytes;
 }
+bool timer_any() {
+    for (const auto& t : g_timers)
+        if (t.on) return true;
+    return false;
+}
 void timer_add_bytes(int id, double bytes) {
     TimerSlot& t = g_timers[id];
     if (t.on) t.bytes += bytes;

@@ -2,6 +2,8 @@
 // Krylov engine, the multigrid hierarchy with its setup and cycles, and the top-level
 // solver object behind the C ABI.
 #pragma once
+#include <map>
+#include <tuple>
 #include <complex>
 #include <functional>
 #include <memory>
@@ -98,6 +100,15 @@ class Multigrid {
     void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    // psi = V-cycle(eta) from level 0 with zero initial guess, replayed from a HIP graph
+    // captured on the first call for these buffers and this mask.  false (nothing done)
+    // when the cycle cannot be captured (no persistent coarsest solver, kernel timers on,
+    // or the runtime refused the capture once); the caller then runs it eagerly.
+    bool vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    // reports (throws) a timed-out persistent coarsest solve among the graph replays
+    void check_graph_failures();
+    int graph_count() const { return (int)graphs_.size(); }
+    ~Multigrid();
     void apply(int l, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mask);
     void restrict_(int l, const cplx<R>* in, cplx<R>* out, uint64_t mask);
     void prolong(int l, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, uint64_t mask);
@@ -148,6 +159,8 @@ class Multigrid {
     std::vector<int> iters_scratch_;
     std::vector<int> sapc_var_;
     std::vector<double> sapc_ms_;
+    bool capturing_ = false, graphs_ok_ = true;
+    std::map<std::tuple<const void*, const void*, uint64_t>, hipGraphExec_t> graphs_;
 };
 
 // ---- top-level solver --------------------------------------------------------------------
@@ -190,7 +203,7 @@ class Solver {
 
    private:
     std::unique_ptr<Krylov<double>> outer_;
-    DevMem x32_, y32_;
+    DevMem x32_, y32_, x64_, y64_;
     int outer_cap_ = 0;
 };
 

@@ -401,9 +401,11 @@ template <typename R>
 void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
     const int L = levels();
     if (use_dev_) {
+        coarse_dev_->deferred = capturing_;
         try {
             coarse_dev_->solve(links(L - 1), p_.m0, eta, psi, mask, iters_scratch_.data());
         } catch (const HipFailure& e) {
+            if (capturing_) throw;   // the capture is abandoned; the caller runs eagerly
             // the runtime refused the cooperative launch: continue with the
             // launch-per-step engine for the rest of this hierarchy's life
             (void)hipGetLastError();
@@ -416,6 +418,7 @@ void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
                                             std::max(1, p_.max_rhs), false, 1));
         }
     }
+    if (capturing_) return;   // no host read-back inside a capture
     if (!use_dev_) {
         auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
         typename Krylov<R>::Func fA = A;
@@ -425,6 +428,51 @@ void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
     for (int q = 0; q < mask_count(mask); q++)
         if ((mask >> q) & 1) mx = std::max(mx, iters_scratch_[q]);
     coarse_iters.push_back(mx);
+}
+
+template <typename R>
+Multigrid<R>::~Multigrid() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+}
+
+template <typename R>
+bool Multigrid<R>::vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    if (!graphs_ok_ || !use_dev_ || timer_any()) return false;
+    const auto key = std::make_tuple(static_cast<const void*>(eta), static_cast<const void*>(psi), mask);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+        const size_t n = g_[0].n();
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        DD_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+        bool ok = true;
+        try {
+            zero<R>(s_, psi, n, n, mask_count(mask), mask);
+            vcycle(0, eta, psi, mask);
+        } catch (...) {
+            ok = false;
+        }
+        capturing_ = false;
+        const hipError_t e = hipStreamEndCapture(s_, &g);
+        ok = ok && e == hipSuccess && g != nullptr && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            (void)hipGetLastError();
+            if (ex) (void)hipGraphExecDestroy(ex);
+            graphs_ok_ = false;
+            if (std::getenv("DDAMG_VERBOSE")) fprintf(stderr, "V-cycle graph capture refused; eager cycles\n");
+            return false;
+        }
+        it = graphs_.emplace(key, ex).first;
+    }
+    DD_HIP(hipGraphLaunch(it->second, s_));
+    return true;
+}
+
+template <typename R>
+void Multigrid<R>::check_graph_failures() {
+    if (coarse_dev_) coarse_dev_->check_failure();
 }
 
 template <typename R>
@@ -767,15 +815,32 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
         if (prm.precision == 1) {
             x32_.alloc(sizeof(cplx<float>) * n * cap);
             y32_.alloc(sizeof(cplx<float>) * n * cap);
+        } else if (prm.graph && prm.cycle == 0) {   // fixed buffers for the captured cycle
+            x64_.alloc(sizeof(cplx<double>) * n * cap);
+            y64_.alloc(sizeof(cplx<double>) * n * cap);
         }
     }
+    bool graphs_used = false;
     const uint64_t mask = full_mask(nrhs);
     Krylov<double>::Func A = [this, n](const cplx<double>* in, cplx<double>* out, uint64_t m) {
         wilson<double>(s, U64.as<cplx<double>>(), in, out, nullptr, Nx, Nt, prm.m0, false, mask_count(m), m, n);
     };
-    Krylov<double>::Prec M = [this, n](const cplx<double>* in, cplx<double>* io, uint64_t m) {
+    // V-cycles (graph = 1): the cycle on fixed buffers is captured once per mask and
+    // replayed; the copies in and out stay outside the graph
+    Krylov<double>::Prec M = [this, n, &graphs_used](const cplx<double>* in, cplx<double>* io, uint64_t m) {
         const int nr = mask_count(m);
+        const bool graph = prm.graph && prm.cycle == 0;
         if (prm.precision == 0) {
+            if (graph) {
+                cplx<double>* a = x64_.as<cplx<double>>();
+                cplx<double>* y = y64_.as<cplx<double>>();
+                copy<double>(s, a, in, n, n, n, nr, m);
+                if (mg64->vcycle_graph(a, y, m)) {
+                    graphs_used = true;
+                    copy<double>(s, io, y, n, n, n, nr, m);
+                    return;
+                }
+            }
             zero<double>(s, io, n, n, nr, m);
             if (prm.cycle == 1) mg64->kcycle(0, in, io, m);
             else mg64->vcycle(0, in, io, m);
@@ -783,9 +848,13 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             cplx<float>* a = x32_.as<cplx<float>>();
             cplx<float>* y = y32_.as<cplx<float>>();
             convert<float, double>(s, a, in, n, n, n, nr, m);
-            zero<float>(s, y, n, n, nr, m);
-            if (prm.cycle == 1) mg32->kcycle(0, a, y, m);
-            else mg32->vcycle(0, a, y, m);
+            if (graph && mg32->vcycle_graph(a, y, m)) {
+                graphs_used = true;
+            } else {
+                zero<float>(s, y, n, n, nr, m);
+                if (prm.cycle == 1) mg32->kcycle(0, a, y, m);
+                else mg32->vcycle(0, a, y, m);
+            }
             convert<double, float>(s, io, y, n, n, n, nr, m);
         }
     };
@@ -793,6 +862,10 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
     std::vector<int> its(64, 0);
     std::vector<double> rr(64, 0.0);
     const uint64_t conv = outer_->solve(b, x, x, mask, A, &M, its.data(), rr.data());
+    if (graphs_used) {
+        if (mg64) mg64->check_graph_failures();
+        if (mg32) mg32->check_graph_failures();
+    }
     DD_HIP(hipStreamSynchronize(s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int mx = 0;

@@ -49,7 +49,7 @@ class Params(C.Structure):
                 ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
                 ('max_rhs', C.c_int), ('autotune', C.c_int),
                 ('sap_variant', C.c_int), ('coarse_basis', C.c_int), ('coarse_sap_links', C.c_int),
-                ('bootstrap', C.c_int)]
+                ('bootstrap', C.c_int), ('graph', C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -88,6 +88,7 @@ _SIGS = {
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
     'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
     'dd_coarsest_stats': (C.c_int, [_ctx, _dp]),
+    'dd_graph_count': (C.c_int, [_ctx, C.POINTER(C.c_int)]),
     'dd_solve_cg': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, _ip, C.POINTER(SolveInfo)]),
     'dd_solve_bicgstab': (C.c_int, [_ctx, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_double, _ip,
                                     C.POINTER(SolveInfo)]),
@@ -312,6 +313,12 @@ class Context:
         keys = ['solves', 'steps', 'ms_reduce', 'ms_barrier1', 'ms_scalar', 'ms_barrier2', 'ms_update',
                 'ms_barrier3', 'ms_cycle_end', 'ms_stencil', 'ms_gather', 'persistent']
         return dict(zip(keys, list(out)))
+
+    def graph_count(self) -> int:
+        """V-cycle graphs captured by the hierarchy (params graph = 1)."""
+        k = C.c_int()
+        _chk(lib.dd_graph_count(self._h, C.byref(k)))
+        return k.value
 
     def level_shape(self, level: int):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -86,6 +86,10 @@ typedef struct dd_params {
                                       D x = w - D w, re-orthonormalises them and
                                       rebuilds the coarse operators.  Needs max_rhs
                                       systems of room per cycle (batches of max_rhs). */
+    int graph;                     /* 1: with cycle = 0, solve() captures the V-cycle
+                                      (SAP, transfers, persistent coarsest GMRES) as a
+                                      HIP graph per batch mask and replays it; 0 (the
+                                      default) runs it as separate launches.        */
 } dd_params;
 
 typedef struct dd_solve_info {
@@ -214,6 +218,9 @@ int dd_get_tuning(dd_ctx* ctx, int level, int* variant, double* ms);
  * barrier, per-system scalar phase, barrier, basis update, barrier, cycle end, stencil,
  * gathering of the partial sums; out[11] = 1 when the persistent kernel is in use. */
 int dd_coarsest_stats(dd_ctx* ctx, double* out);
+/* Number of V-cycle graphs the hierarchy has captured (dd_params.graph = 1); no
+ * reference counterpart (diagnostics). */
+int dd_graph_count(dd_ctx* ctx, int* count);
 /* Shape of level l: Nx, Nt, dof. */
 int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);
 /* ---- live kernel timing -----------------------------------------------------------------

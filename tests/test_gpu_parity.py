@@ -359,6 +359,37 @@ def test_coarsest_persistent_64_systems():
             assert rel(ctx.cycle(2, v[r], np.zeros_like(v[r]), kcycle=True, precision=1), x[r]) < 1e-12
 
 
+@pytest.mark.parametrize('precision', [0, 1])
+def test_vcycle_graph_equals_eager(golden, precision):
+    """V-cycle preconditioner captured as a HIP graph (graph = 1) and replayed once per outer
+    FGMRES iteration, with the solver's own eager kernels and host synchronisations between
+    the replays: a 4-system batch gives the iteration counts and solutions of the eager
+    cycles exactly (every replay feeds the next iteration), a second solve replays the
+    cached graphs with the same result, and at least one graph was captured."""
+    meta, a = golden('g64_solve')
+    rng = np.random.default_rng(9)
+    B = np.stack([a['rhs']] + [rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape)
+                               for _ in range(3)])
+    U = np.exp(1j * a['theta'])
+    out = {}
+    for graph in (0, 1):
+        with _ctx_from_golden(meta, a) as ctx:
+            ctx.setup(_prm(meta, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'], cycle=0, precision=precision,
+                           max_rhs=4, graph=graph))
+            x, its, info = ctx.solve(B)
+            if graph:
+                assert ctx.graph_count() >= 1
+                x2, its2, _ = ctx.solve(B)
+                assert list(its2) == list(its) and np.array_equal(x2, x)
+            out[graph] = (x, list(its))
+    print(f'precision {precision}: eager iterations {out[0][1]}, graph {out[1][1]}')
+    assert out[1][1] == out[0][1]
+    assert rel(out[1][0], out[0][0]) < 1e-13
+    for r in range(4):
+        rr = np.linalg.norm(B[r] - orc.D_phi(U, out[1][0][r], meta['m0'])) / np.linalg.norm(B[r])
+        assert rr < 1e-10
+
+
 def test_ragged_batch_equals_separate_solves():
     """dd_solve_many_dev: several lattices of different sizes (and masses), each with its
     own hierarchy and batch of right-hand sides, solved at the same time give every system
