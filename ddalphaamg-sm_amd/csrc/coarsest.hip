@@ -87,7 +87,7 @@ struct GridBar {
     // read by the work-groups whose stencil needs its rows (see wait_neighbours)
     unsigned udone[kGmax * kFlagStride];
     // per work-group: the last Arnoldi step whose partial sums it has written (release),
-    // read by the owners of the running systems (see wait_partials)
+    // read by the owners of the running systems (see phaseB)
     unsigned pdone[kGmax * kFlagStride];
 };
 
@@ -591,6 +591,34 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             // their partials loaded before the first addition); the waves' sums are added
             // in LDS in a fixed order
             constexpr int KW = kGmax / kNW, KB = sizeof(R) == 4 ? KW / 2 : KW / 4;
+            // each wave first waits for the partial-sum flags of its own work-groups (lane u
+            // polls w + 8u) and acquires them, so that a wave whose work-groups are done
+            // gathers while the others still wait; a timed-out wait clears si[2].  Every
+            // work-group publishes its partials after its stencil and reduction of the
+            // step, which follow its update of the previous step: once an owner has seen
+            // all of them, no work-group still reads the coefficients or the control flags
+            // the owner is about to rewrite, nor the rows of u that their next updates
+            // overwrite.
+            {
+                bool ok = true;
+                const int k = wv + kNW * lane;
+                if (lane < KW && k < G) {
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(&a.bar->pdone[k * kFlagStride], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) < gstep) {
+                        if ((++spins & 63u) == 0 &&
+                            (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                             spins > kBarSpins)) {
+                            __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            ok = false;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (__ballot(!ok) && lane == 0) si[2] = 0;
+            }
             for (int base = 0; base < nv; base += 64) {
                 const int idx = base + lane;
                 int v = 0;
@@ -669,8 +697,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (tid == 0) si[1] = stop_now;
         if (wv * 64 <= j) release_stores();   // the waves that stored coefficients
         __syncthreads();
-        // publish: every thread's coefficient stores precede thread 0's release
-        if (tid == 0)
+        // publish: every thread's coefficient stores precede thread 0's release (not after
+        // a timed-out wait: the kernel ends)
+        if (tid == 0 && si[2])
             __hip_atomic_store(&a.bar->cflag[q * kFlagStride], (gstep << 1) | (stop_now ? 1u : 0u), __ATOMIC_RELEASE,
                                __HIP_MEMORY_SCOPE_AGENT);
     };
@@ -688,37 +717,6 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (nbw >= 0) {
                 unsigned spins = 0;
                 while (__hip_atomic_load(&a.bar->udone[nbw * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
-                    if ((++spins & 63u) == 0 &&
-                        (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                         spins > kBarSpins)) {
-                        __hip_atomic_store(&a.bar->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = false;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const uint64_t bad = __ballot(!ok);
-            if (lane == 0) si[2] = bad ? 0 : 1;
-        }
-        __syncthreads();
-    };
-    // Waits (wave 0, four work-groups per lane) until every work-group has written its
-    // partial sums of Arnoldi step gstep, then acquires them for this (owner) work-group;
-    // si[2] = 0 when the kernel must end.  Every work-group publishes its partials after
-    // its stencil and reduction of the step, which follow its update of the previous
-    // step: once an owner has seen all of them, no work-group still reads the
-    // coefficients or the control flags the owner is about to rewrite, nor the rows of
-    // u that their next updates overwrite.
-    auto wait_partials = [&](unsigned gstep) {
-        __syncthreads();
-        if (wv == 0) {
-            bool ok = true;
-            for (int k = lane; k < G && ok; k += 64) {
-                unsigned spins = 0;
-                while (__hip_atomic_load(&a.bar->pdone[k * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                       gstep) {
                     if ((++spins & 63u) == 0 &&
                         (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                          spins > kBarSpins)) {
@@ -1017,14 +1015,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             // them, none for the rest)
             ++gstep;
             release_stores();
+            if (tid == 0) si[2] = 1;   // no timed-out wait in phase B (yet)
             __syncthreads();
             if (tid == 0) __hip_atomic_store(&a.bar->pdone[wg * kFlagStride], gstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            clk(1);
             if (owner && ((run >> sys) & 1ull)) {
-                wait_partials(gstep);
+                phaseB(j, gstep);
                 if (!si[2]) return;
             }
-            clk(1);
-            if (owner && ((run >> sys) & 1ull)) phaseB(j, gstep);
             clk(2);
             const uint64_t cont = wait_coefs(run, gstep);
             if (!si[2]) return;
