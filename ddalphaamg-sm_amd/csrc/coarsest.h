@@ -40,9 +40,10 @@ class CoarsestGmres {
         restarts_ = restarts;
         tol_ = tol;
     }
-    // wall-clock ticks of work-group 0 per phase (cumulative): reduction, barrier,
-    // scalar phase (after the totals), barrier, update, barrier, cycle end, stencil,
-    // totals of the scalar phase
+    // wall-clock ticks of work-group 0 per phase (cumulative): reduction, publication of
+    // the partial sums, scalar phase (after the totals), wait for the coefficient flags,
+    // update, publication of the update, cycle end, neighbour wait and stencil, totals
+    // (gather with its waits) of the scalar phase
     double phase_clk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
    private:

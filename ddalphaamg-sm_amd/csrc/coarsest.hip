@@ -1253,7 +1253,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     if (verbose && wgclk_.bytes() && ++solves_ % 19 == 0) {   // spread of the phase times over the work-groups
         std::vector<double> h((size_t)9 * G_);
         DD_HIP(hipMemcpy(h.data(), a.wgclk, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-        const char* nm[9] = {"reduce", "barrier1", "scalar", "barrier2", "update", "barrier3", "cycle", "stencil",
+        const char* nm[9] = {"reduce", "pub-part", "scalar", "wait-cf", "update", "pub-upd", "cycle", "nbr+sten",
                              "gather"};
         for (int k = 0; k < 9; k++) {
             double mn = 1e300, mx = 0, sum = 0;

@@ -310,8 +310,8 @@ class Context:
         """Coarsest-solver counters since setup (solves, Arnoldi steps, phase ms)."""
         out = (C.c_double * 12)()
         _chk(lib.dd_coarsest_stats(self._h, out))
-        keys = ['solves', 'steps', 'ms_reduce', 'ms_barrier1', 'ms_scalar', 'ms_barrier2', 'ms_update',
-                'ms_barrier3', 'ms_cycle_end', 'ms_stencil', 'ms_gather', 'persistent']
+        keys = ['solves', 'steps', 'ms_reduce', 'ms_publish_partials', 'ms_scalar', 'ms_wait_coefs', 'ms_update',
+                'ms_publish_update', 'ms_cycle_end', 'ms_wait_neighbours_stencil', 'ms_gather', 'persistent']
         return dict(zip(keys, list(out)))
 
     def graph_count(self) -> int:

@@ -214,9 +214,11 @@ int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);
 int dd_get_tuning(dd_ctx* ctx, int level, int* variant, double* ms);
 /* Coarsest-solver statistics since setup (no reference counterpart; diagnostics):
  * out[0] coarsest solves, out[1] Arnoldi steps (max over the batch per step), out[2..10]
- * milliseconds spent by the persistent kernel's first work-group in: reduction pass,
- * barrier, per-system scalar phase, barrier, basis update, barrier, cycle end, stencil,
- * gathering of the partial sums; out[11] = 1 when the persistent kernel is in use. */
+ * milliseconds spent by the persistent kernel's first work-group (wave 0) in: reduction
+ * pass, publication of its partial sums (its other waves, its stores), per-system scalar
+ * phase, wait for the owners' coefficient flags, basis update, publication of the update,
+ * cycle end, wait for the neighbours' flags and stencil, gathering of the partial sums
+ * (with the waits for them); out[11] = 1 when the persistent kernel is in use. */
 int dd_coarsest_stats(dd_ctx* ctx, double* out);
 /* Number of V-cycle graphs the hierarchy has captured (dd_params.graph = 1); no
  * reference counterpart (diagnostics). */
