@@ -227,9 +227,10 @@ def main():
 
 def secondary_rows(ddamg, torch, sweeps, seed):
     """Further workloads: the 256^2 plain FP64 single-RHS solve, 32 RHS on one 256^2 field
-    (mixed, batched), and a ragged batch of 8 fields of sizes {64,96,128,128,192,256,256,256}^2
-    with m0 drawn from [-0.18, -0.10] and 4 RHS each, all solved at the same time
-    (dd_solve_many_dev).  Gauge fields: seeded quenched Metropolis, beta = 2."""
+    (mixed, batched; K-cycle, and V-cycle eager and from captured graphs), and a ragged
+    batch of 8 fields of sizes {64,96,128,128,192,256,256,256}^2 with m0 drawn from
+    [-0.18, -0.10] and 4 RHS each, all solved at the same time (dd_solve_many_dev).
+    Gauge fields: seeded quenched Metropolis, beta = 2."""
     out = {}
     rng = np.random.default_rng(seed + 7)
 
@@ -258,6 +259,22 @@ def secondary_rows(ddamg, torch, sweeps, seed):
         out[key] = {'ms_per_rhs': t * 1e3 / nr, 'iterations': [int(its.min()), int(its.max())],
                     'max_relres': info.relres}
         log(f'{key}: {t * 1e3 / nr:.2f} ms/RHS, its {its.min()}..{its.max()}')
+    # the same 32 RHS with V-cycles, eager and replayed from captured HIP graphs
+    for graph in (0, 1):
+        p = hierarchy_params(ddamg, 256, 1, 32, -0.18)
+        p.cycle = 0
+        p.graph = graph
+        ctx.setup(p)
+        ctx.solve_dev(b.data_ptr(), x.data_ptr(), 32)   # warm-up (captures the graphs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its, info = ctx.solve_dev(b.data_ptr(), x.data_ptr(), 32)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        key = 'mixed_256_32rhs_vcycle_graph' if graph else 'mixed_256_32rhs_vcycle'
+        out[key] = {'ms_per_rhs': t * 1e3 / 32, 'iterations': [int(its.min()), int(its.max())],
+                    'max_relres': info.relres, 'graphs': ctx.graph_count()}
+        log(f'{key}: {t * 1e3 / 32:.2f} ms/RHS, its {its.min()}..{its.max()}, graphs {ctx.graph_count()}')
     ctx.close()
 
     # ragged batch: 8 fields, 4 RHS each, mixed precision
