@@ -9,6 +9,11 @@
 
 struct dd_ctx {
     std::unique_ptr<dd::Solver> s;
+    void* caller = nullptr;        // stream whose queued work the _dev calls wait for
+    hipEvent_t ordered = nullptr;  // recorded on `caller`, waited for by the context's stream
+    ~dd_ctx() {
+        if (ordered) (void)hipEventDestroy(ordered);
+    }
 };
 
 namespace {
@@ -59,6 +64,27 @@ int check(dd_ctx* c) {
     return DD_OK;
 }
 
+// dd_params from a caller compiled against another layout of the struct
+int check_params(const dd_params* p) {
+    if (!p) {
+        set_err("null dd_params");
+        return DD_ERR_ARG;
+    }
+    if (p->size != (int)sizeof(dd_params)) {
+        set_err("dd_params of " + std::to_string(p->size) + " bytes; this library's is " +
+                std::to_string(sizeof(dd_params)) + " (fill it with dd_default_params of the same header)");
+        return DD_ERR_ARG;
+    }
+    return DD_OK;
+}
+
+// the context's stream waits for the work queued so far on the caller's stream
+void order_after_caller(dd_ctx* c) {
+    if (!c->ordered) DD_HIP(hipEventCreateWithFlags(&c->ordered, hipEventDisableTiming));
+    DD_HIP(hipEventRecord(c->ordered, (hipStream_t)c->caller));
+    DD_HIP(hipStreamWaitEvent(c->s->s, c->ordered, 0));
+}
+
 template <typename R>
 dd::Multigrid<R>* mg(dd_ctx* c);
 template <>
@@ -98,9 +124,20 @@ extern "C" {
 
 const char* dd_last_error(void) { return g_err.c_str(); }
 
-int dd_default_params(dd_params* p, int levels) {
-    if (!p || levels < 2 || levels > DD_MAX_LEVELS) return DD_ERR_ARG;
+int dd_params_size(void) { return (int)sizeof(dd_params); }
+
+int dd_default_params_sized(dd_params* p, int size, int levels) {
+    if (!p || levels < 2 || levels > DD_MAX_LEVELS) {
+        set_err("dd_default_params: null pointer or levels outside [2, 8]");
+        return DD_ERR_ARG;
+    }
+    if (size != (int)sizeof(dd_params)) {
+        set_err("dd_default_params: caller's dd_params has " + std::to_string(size) + " bytes, this library's " +
+                std::to_string(sizeof(dd_params)) + " (rebuild against the library's ddamg.h)");
+        return DD_ERR_ARG;
+    }
     std::memset(p, 0, sizeof(*p));
+    p->size = (int)sizeof(dd_params);
     p->levels = levels;
     for (int l = 0; l < DD_MAX_LEVELS; l++) {
         p->agg_x[l] = p->agg_t[l] = 4;
@@ -137,6 +174,19 @@ int dd_default_params(dd_params* p, int levels) {
     p->coarse_sap_links = 0;
     p->bootstrap = 0;
     p->graph = 0;
+    return DD_OK;
+}
+
+int dd_finalize(void) {
+    return guard([&] {
+        dd::timer_release();
+        return DD_OK;
+    });
+}
+
+int dd_set_stream(dd_ctx* c, void* stream) {
+    if (check(c)) return DD_ERR_ARG;
+    c->caller = stream;
     return DD_OK;
 }
 
@@ -215,6 +265,7 @@ int dd_apply_D_dev(dd_ctx* c, const double* in, double* out, int nrhs, double m0
     if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
     return guard([&] {
         if (!c->s->have_gauge) throw dd::BadShape("no gauge field");
+        order_after_caller(c);
         c->s->apply_D((const cplx<double>*)in, (cplx<double>*)out, nrhs, m0, dagger != 0, precision);
         DD_HIP(hipStreamSynchronize(c->s->s));
         return DD_OK;
@@ -235,7 +286,7 @@ int dd_apply_D(dd_ctx* c, const double* in, double* out, int nrhs, double m0, in
 }
 
 int dd_setup(dd_ctx* c, const dd_params* p) {
-    if (check(c) || !p) return DD_ERR_ARG;
+    if (check_params(p) || check(c)) return DD_ERR_ARG;
     return guard([&] {
         c->s->setup(*p);
         return DD_OK;
@@ -246,7 +297,10 @@ double dd_setup_seconds(dd_ctx* c) { return (c && c->s) ? c->s->setup_seconds : 
 
 int dd_solve_dev(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
     if (check(c) || !b || !x) return DD_ERR_ARG;
-    return guard([&] { return c->s->solve(nrhs, (const cplx<double>*)b, (cplx<double>*)x, iters, info); });
+    return guard([&] {
+        order_after_caller(c);
+        return c->s->solve(nrhs, (const cplx<double>*)b, (cplx<double>*)x, iters, info);
+    });
 }
 
 int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* const* b, double* const* x,
@@ -261,6 +315,7 @@ int dd_solve_many_dev(int n, dd_ctx* const* ctx, const int* nrhs, const double* 
         th.emplace_back([&, i] {
             try {
                 DD_HIP(hipSetDevice(ctx[i]->s->device));
+                order_after_caller(ctx[i]);
                 dd_solve_info inf{};
                 st[i] = ctx[i]->s->solve(nrhs[i], (const cplx<double>*)b[i], (cplx<double>*)x[i], nullptr, &inf);
                 if (info) info[i] = inf;
@@ -357,6 +412,7 @@ int dd_solve_adjoint_dev(dd_ctx* c, int nrhs, const double* eta, double* y, int*
     return guard([&] {
         hipStream_t s = c->s->s;
         const size_t n = vsize(c);
+        order_after_caller(c);
         DevMem t(sizeof(cplx<double>) * n * nrhs);
         dd::sigma3<double>(s, t.as<cplx<double>>(), (const cplx<double>*)eta, n, n, nrhs, dd::full_mask(nrhs));
         const int st = c->s->solve(nrhs, t.as<cplx<double>>(), (cplx<double>*)y, iters, info);
@@ -376,6 +432,46 @@ int dd_solve_adjoint(dd_ctx* c, int nrhs, const double* eta, double* y, int* ite
         if (st != DD_OK && st != DD_ERR_NOT_CONVERGED) return st;
         download(y, yy.p(), n, s);
         return st;
+    });
+}
+
+int dd_apply_sigma3(dd_ctx* c, const double* in, double* out, int nrhs) {
+    if (check(c) || !in || !out || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c);
+        Staged a(in, n * nrhs, s), b(nullptr, n * nrhs, s);
+        dd::sigma3<double>(s, b.p(), a.p(), n, n, nrhs, dd::full_mask(nrhs));
+        download(out, b.p(), n * nrhs, s);
+        return DD_OK;
+    });
+}
+
+int dd_dot_dev(dd_ctx* c, long long n, int nrhs, const double* a, const double* b, double* out) {
+    if (check(c) || !a || !b || !out || n < 1 || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        order_after_caller(c);
+        DevMem part(dd::dot_scratch_bytes(nrhs));
+        dd::RedBuf rb;
+        rb.partial = part.as<void>();
+        dd::dot_partial<double>(s, (const cplx<double>*)a, (const cplx<double>*)b, (size_t)n, (size_t)n, nrhs,
+                                dd::full_mask(nrhs), rb);
+        dd::dot_final<double>(s, rb, (cplx<double>*)out, 1, nrhs, dd::full_mask(nrhs));
+        DD_HIP(hipStreamSynchronize(s));
+        return DD_OK;
+    });
+}
+
+int dd_dot(dd_ctx* c, long long n, int nrhs, const double* a, const double* b, double* out) {
+    if (check(c) || !a || !b || !out || n < 1 || nrhs < 1 || nrhs > 64) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        Staged aa(a, (size_t)n * nrhs, s), bb(b, (size_t)n * nrhs, s), oo(nullptr, nrhs, s);
+        const int st = dd_dot_dev(c, n, nrhs, (const double*)aa.p(), (const double*)bb.p(), (double*)oo.p());
+        if (st != DD_OK) return st;
+        download(out, oo.p(), nrhs, s);
+        return DD_OK;
     });
 }
 

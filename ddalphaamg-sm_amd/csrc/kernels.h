@@ -19,6 +19,8 @@ void timer_begin(hipStream_t s, int id);
 void timer_end(hipStream_t s, int id, double bytes);
 // adds algorithmic bytes to kernel `id` after the fact (kernels that count their own work)
 void timer_add_bytes(int id, double bytes);
+// destroys the timers' event pools (re-created on demand)
+void timer_release();
 // true when any kernel timer is on (their event records stay out of graph captures)
 bool timer_any();
 // total elapsed ms, number of timed launches and algorithmic bytes of kernel `id`
@@ -32,6 +34,10 @@ struct RedBuf {            // scratch for two-pass reductions
     void* partial = nullptr;   // [nrhs][nblk] complex<double>
     int nblk = 0;
 };
+// most partial sums a reduction keeps per system (blocks of dot_partial)
+constexpr int kMaxRedBlocks = 256;
+// bytes of RedBuf::partial that dot_partial needs for nrhs systems
+inline size_t dot_scratch_bytes(int nrhs) { return sizeof(cplx<double>) * (size_t)kMaxRedBlocks * nrhs; }
 
 // ---- level-0 Wilson-Dirac ---------------------------------------------------------
 // out = D in  (b == nullptr)   or   out = b - D in.   dagger selects D^dagger.

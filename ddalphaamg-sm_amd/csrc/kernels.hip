@@ -62,6 +62,14 @@ void timer_end(hipStream_t s, int id, double bytes) {
     t.used += 2;
     t.bytes += bytes;
 }
+void timer_release() {
+    for (auto& t : g_timers) {
+        for (hipEvent_t e : t.pool) (void)hipEventDestroy(e);
+        t.pool.clear();
+        t.used = 0;
+        t.bytes = 0;
+    }
+}
 bool timer_any() {
     for (const auto& t : g_timers)
         if (t.on) return true;
@@ -480,7 +488,9 @@ __global__ __launch_bounds__(256) void k_dot_partial(const cplx<R>* __restrict__
     if (threadIdx.x == 0) partial[(size_t)r * gridDim.x + blockIdx.x] = mk<double>(s.x, s.y);
 }
 
-static inline int red_blocks(size_t n) { return (int)std::min<size_t>(256, std::max<size_t>(1, (n + 2047) / 2048)); }
+static inline int red_blocks(size_t n) {
+    return (int)std::min<size_t>(kMaxRedBlocks, std::max<size_t>(1, (n + 2047) / 2048));
+}
 
 template <typename R>
 void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, size_t ld, int nrhs, uint64_t mask,

@@ -8,8 +8,10 @@ arrays of shape (Nx, Nt, dof) or (nrhs, Nx, Nt, dof) with dof = colour*2 + spin.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -35,7 +37,7 @@ MAX_LEVELS = 8
 
 class Params(C.Structure):
     """Mirror of dd_params (include/ddamg.h)."""
-    _fields_ = [('levels', C.c_int),
+    _fields_ = [('size', C.c_int), ('levels', C.c_int),
                 ('agg_x', C.c_int * MAX_LEVELS), ('agg_t', C.c_int * MAX_LEVELS),
                 ('sap_x', C.c_int * MAX_LEVELS), ('sap_t', C.c_int * MAX_LEVELS),
                 ('nv', C.c_int * MAX_LEVELS),
@@ -61,7 +63,13 @@ _ip = C.POINTER(C.c_int)
 _ctx = C.c_void_p
 
 _SIGS = {
-    'dd_default_params': (C.c_int, [C.POINTER(Params), C.c_int]),
+    'dd_default_params_sized': (C.c_int, [C.POINTER(Params), C.c_int, C.c_int]),
+    'dd_params_size': (C.c_int, []),
+    'dd_finalize': (C.c_int, []),
+    'dd_set_stream': (C.c_int, [_ctx, C.c_void_p]),
+    'dd_apply_sigma3': (C.c_int, [_ctx, _dp, _dp, C.c_int]),
+    'dd_dot': (C.c_int, [_ctx, C.c_longlong, C.c_int, _dp, _dp, _dp]),
+    'dd_dot_dev': (C.c_int, [_ctx, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     'dd_create': (C.c_int, [C.POINTER(_ctx), C.c_int, C.c_int, C.c_int]),
     'dd_destroy': (C.c_int, [_ctx]),
     'dd_set_gauge': (C.c_int, [_ctx, _dp]),
@@ -123,6 +131,21 @@ for _name, (_res, _args) in _SIGS.items():
     _f.restype = _res
     _f.argtypes = _args
 
+if C.sizeof(Params) != lib.dd_params_size():
+    raise ImportError(f'Params mirror has {C.sizeof(Params)} bytes, the library\'s dd_params '
+                      f'{lib.dd_params_size()}: the binding and include/ddamg.h disagree')
+_LIVE = weakref.WeakSet()   # open contexts
+
+
+@atexit.register
+def _finalize():
+    """Destroys the open contexts and the library's process-wide HIP objects while the
+    HIP runtime is still up (interpreter exit runs this before the runtime's own
+    teardown)."""
+    for c in list(_LIVE):
+        c.close()
+    lib.dd_finalize()
+
 
 class DDError(RuntimeError):
     def __init__(self, code, msg):
@@ -148,7 +171,7 @@ def params(levels: int = 2, **kw) -> Params:
     """dd_params with the reference defaults; per-level lists (agg, sap_block, nv) and
     scalar knobs as keywords: agg=[(4,4),...], sap_block=[(4,4),...], nv=[8,...]."""
     p = Params()
-    _chk(lib.dd_default_params(C.byref(p), levels))
+    _chk(lib.dd_default_params_sized(C.byref(p), C.sizeof(Params), levels))
     for l, (ax, at) in enumerate(kw.pop('agg', [])):
         p.agg_x[l], p.agg_t[l] = ax, at
     for l, (bx, bt) in enumerate(kw.pop('sap_block', [])):
@@ -172,6 +195,7 @@ class Context:
         self._h = _ctx()
         _chk(lib.dd_create(C.byref(self._h), Nx, Nt, device))
         self.prm: Params | None = None
+        _LIVE.add(self)
 
     def close(self):
         if self._h:
@@ -277,9 +301,22 @@ class Context:
         _chk(code, allow=(DD_OK,) if check else (DD_OK, DD_ERR_NOT_CONVERGED))
         return (x if batch else x[0]), np.array(list(its)), info
 
+    def _order_after_torch(self):
+        """The next _dev call waits for the work torch has queued on its current stream."""
+        s = None
+        try:
+            import torch
+            if torch.cuda.is_initialized():
+                s = torch.cuda.current_stream().cuda_stream or None
+        except Exception:  # noqa: BLE001  (no torch: the null stream)
+            pass
+        _chk(lib.dd_set_stream(self._h, s))
+
     def solve_dev(self, b_ptr: int, x_ptr: int, nrhs: int, adjoint: bool = False, check: bool = True):
         """Solve with device pointers (complex128 interleaved, nrhs systems back to back),
-        e.g. torch tensors' data_ptr().  Returns (iterations per system, info)."""
+        e.g. torch tensors' data_ptr().  The solve first waits for the work queued on
+        torch's current stream.  Returns (iterations per system, info)."""
+        self._order_after_torch()
         its = (C.c_int * nrhs)()
         info = SolveInfo()
         f = lib.dd_solve_adjoint_dev if adjoint else lib.dd_solve_dev
@@ -289,8 +326,33 @@ class Context:
 
     def apply_D_dev(self, in_ptr: int, out_ptr: int, nrhs: int, m0: float, dagger: bool = False,
                     precision: int = 0):
+        self._order_after_torch()
         _chk(lib.dd_apply_D_dev(self._h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), nrhs, m0, int(dagger),
                                 precision))
+
+    def sigma3(self, v: np.ndarray) -> np.ndarray:
+        """sigma3 v (spin-1 components negated) of level-0 vectors."""
+        v = _c128(v)
+        vb = v if v.ndim == 4 else v[None]
+        out = np.zeros_like(vb)
+        _chk(lib.dd_apply_sigma3(self._h, _cptr(vb), _cptr(out), vb.shape[0]))
+        return out if v.ndim == 4 else out[0]
+
+    def dot(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        """sum a conj(b) per system: a, b of shape (nrhs, ...) (or one vector)."""
+        a, b = _c128(a), _c128(b)
+        single = a.ndim < 4
+        ab = a.reshape(1 if single else a.shape[0], -1)
+        bb = b.reshape(ab.shape)
+        out = np.zeros(ab.shape[0], complex)
+        _chk(lib.dd_dot(self._h, ab.shape[1], ab.shape[0], _cptr(np.ascontiguousarray(ab)),
+                        _cptr(np.ascontiguousarray(bb)), _cptr(out)))
+        return out[0] if single else out
+
+    def dot_dev(self, a_ptr: int, b_ptr: int, out_ptr: int, n: int, nrhs: int):
+        """Device dot products: out[r] = sum a_r conj(b_r), n complex entries per system."""
+        self._order_after_torch()
+        _chk(lib.dd_dot_dev(self._h, n, nrhs, C.c_void_p(a_ptr), C.c_void_p(b_ptr), C.c_void_p(out_ptr)))
 
     def dD_dtheta(self, y: np.ndarray, x: np.ndarray, scale: float = -1.0) -> np.ndarray:
         y, x = _c128(y), _c128(x)
@@ -386,6 +448,8 @@ def solve_many_dev(contexts, nrhs, b_ptrs, x_ptrs):
     pointers b_ptrs[i], x_ptrs[i]; all contexts at the same time.  Returns the SolveInfo
     list."""
     n = len(contexts)
+    for c in contexts:
+        c._order_after_torch()
     ctxs = (C.c_void_p * n)(*[c._h.value for c in contexts])
     nr = (C.c_int * n)(*nrhs)
     bp = (C.c_void_p * n)(*b_ptrs)

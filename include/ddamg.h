@@ -36,6 +36,12 @@ typedef struct dd_ctx dd_ctx;
  * knobs, src/variables.cpp:90-134, and its per-level parameters file,
  * include/params.h:7-57 (here in SITES per aggregate / SAP block, not block counts). */
 typedef struct dd_params {
+    int size;                      /* sizeof(dd_params) in the caller's build; set by
+                                      dd_default_params.  Every entry point that takes
+                                      a dd_params rejects (DD_ERR_ARG) a size other than
+                                      the library's, so a caller compiled against
+                                      another version of this header fails loudly
+                                      instead of reading or writing past its struct. */
     int levels;                    /* number of levels L (>= 2)                     */
     int agg_x[DD_MAX_LEVELS];      /* aggregate size in sites at level l < L-1      */
     int agg_t[DD_MAX_LEVELS];
@@ -99,13 +105,32 @@ typedef struct dd_solve_info {
 } dd_solve_info;
 
 /* Fills p with the reference defaults for a `levels`-level hierarchy with 4x4
- * aggregates, 4x4 SAP blocks and Nv = 8. */
-int dd_default_params(dd_params* p, int levels);
+ * aggregates, 4x4 SAP blocks and Nv = 8.  `size` is sizeof(dd_params) as the caller
+ * compiled it: on a mismatch nothing is written and DD_ERR_ARG is returned.
+ * dd_default_params() passes it. */
+int dd_default_params_sized(dd_params* p, int size, int levels);
+static inline int dd_default_params(dd_params* p, int levels) {
+    return dd_default_params_sized(p, (int)sizeof(dd_params), levels);
+}
+/* sizeof(dd_params) of the library (bindings check their mirror of the struct). */
+int dd_params_size(void);
+
+/* Releases the library's process-wide HIP objects (the kernel timers' events).  Call
+ * after the last dd_destroy and before the HIP runtime shuts down (the Python binding
+ * registers it with atexit); later calls re-create what they need. */
+int dd_finalize(void);
 
 /* Context for an Nx x Nt lattice on HIP device `device`.  Replaces the global set-up
  * of src/main.cpp:57-72 (allocate_lattice_arrays, initializeMPI, boundaries). */
 int dd_create(dd_ctx** ctx, int Nx, int Nt, int device);
 int dd_destroy(dd_ctx* ctx);
+
+/* Stream ordering of the _dev entry points.  Each _dev call first makes the context's
+ * own stream wait for all work queued so far on `stream` (a hipStream_t; NULL, the
+ * default, is the null stream, which is where torch queues work unless the caller
+ * switched streams), so device inputs written there are complete; it returns after its
+ * own work has finished.  No reference counterpart (the reference is host-only). */
+int dd_set_stream(dd_ctx* ctx, void* stream);
 
 /* Gauge field (complex128, layout above).  Replaces read_binary of the configuration
  * src/io.cpp:4-55 followed by the halo set-up of Level::makeDirac src/level.cpp:286-331. */
@@ -185,6 +210,17 @@ int dd_solve_gmres(dd_ctx* ctx, int nrhs, const double* b, const double* x0, dou
  * (src/dirac_operator.cpp:56-85); no adjoint solver. */
 int dd_solve_adjoint(dd_ctx* ctx, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info);
 int dd_solve_adjoint_dev(dd_ctx* ctx, int nrhs, const double* eta, double* y, int* iters, dd_solve_info* info);
+
+/* out = sigma3 in (the spin-1 components negated) for nrhs level-0 vectors: the
+ * gamma5 of the reference's D^dagger = sigma3 D sigma3 (src/dirac_operator.cpp:56-85). */
+int dd_apply_sigma3(dd_ctx* ctx, const double* in, double* out, int nrhs);
+
+/* out[r] = sum_i a_r[i] conj(b_r[i]) (complex128, out as re, im pairs) for nrhs pairs of
+ * vectors of n complex entries each, stored back to back; FP64, summed in a fixed order
+ * (deterministic).  Replaces dot(A, B) of include/utils.h:68-84 (and the FGMRES dot
+ * hooks, include/fgmres.h:97-98).  dd_dot_dev: device pointers, out on the device. */
+int dd_dot(dd_ctx* ctx, long long n, int nrhs, const double* a, const double* b, double* out);
+int dd_dot_dev(dd_ctx* ctx, long long n, int nrhs, const double* a, const double* b, double* out);
 
 /* grad[x][t][mu] += scale * sum_r Re[ y_r^H (dD/dtheta_mu(x)) x_r ] for U = e^{i theta};
  * with scale = -1 this is dL/dtheta of L = Re<eta, D^-1 b> for y = D^-dagger eta,

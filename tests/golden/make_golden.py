@@ -17,6 +17,14 @@ Fixtures:
               level-2 links, transfer ops, D_1, SAP at levels 0 and 1
   g64_solve   cfg1 shape (64^2, m0=-0.1, 2 levels, Nv=8, nu2=1, FGMRES(20)): K- and
               V-cycle solutions, iteration counts and residuals
+  g256_solve  cfg2 shape (256^2, beta=2, m0=-0.18, 3 levels, 4x4 aggregates, Nv=12,
+              4x4-site SAP blocks, nu2=2, FGMRES(30), 2x4 ranks) on the field that the
+              library's device Metropolis generator makes from seed 256: K- and V-cycle
+              iteration counts, residuals and the solution at 512 sampled sites
+  g512_solve  cfg3 shape (512^2, as g256_solve, seed 512): K-cycle iteration count,
+              residual and the solution at 512 sampled sites
+              (both need the field as .npy: python tests/golden/make_golden.py
+               g256_solve=theta_256.npy g512_solve=theta_512.npy)
   g32_methods the comparison solvers of src/methods.cpp on a 32^2 field (m0=-0.1, tol
               1e-10, 2x2 ranks so that FGMRES_SAP's 4x4 blocks per rank are 4x4 sites):
               CG on the normal equations, BiCGStab, GMRES(20) and FGMRES(20)+SAP --
@@ -75,10 +83,51 @@ def g32_methods():
     run_ref.cleanup(wd)
 
 
+def sample_sites(N, count=512, seed=7):
+    """Site indices (x*N + t) at which the large-lattice fixtures keep the solution."""
+    return np.sort(np.random.default_rng(seed).choice(N * N, size=count, replace=False))
+
+
+def big_solve(name, N, theta_path, cycles):
+    """Reference solves at the cfg2/cfg3 shape on a field from the library's device
+    generator (seed = N).  The fixture keeps the field's SHA-256, so a test can tell
+    that it regenerated the same field, the iteration counts and residuals, and the
+    solution at sampled sites."""
+    import hashlib
+    seed, m0, ranks = N, -0.18, (2, 4)
+    theta = np.load(theta_path)
+    assert theta.shape == (N, N, 2) and theta.dtype == np.float64
+    U = np.exp(1j * theta)
+    b = gaussian(N, seed + 1)
+    levels = [orc.LevelSpec((4, 4), (4, 4), 12), orc.LevelSpec((4, 4), (4, 4), 12)]
+    sites = sample_sites(N)
+    meta = dict(N=N, m0=m0, beta=2.0, sweeps=2000, seed=seed, step=0.5, rhs_seed=seed + 1,
+                theta_sha256=hashlib.sha256(np.ascontiguousarray(theta).tobytes()).hexdigest(),
+                levels=level_meta(levels), ranks=list(ranks), nu1=0, nu2=2, fgmres_m=30, fgmres_restarts=50,
+                tol=1e-10, plaquette=float(np.mean(np.cos(orc.plaquette_angle(theta)))))
+    arr = dict(sites=sites)
+    for cyc in cycles:
+        wd, _ = run_ref.run('solve', U, b, levels, m0, ranks=ranks, nu2=2, fgmres_m=30, cycle=cyc, timeout=3600)
+        res = run_ref.result(wd)
+        meta[f'{cyc}_iterations'] = res['iterations']
+        meta[f'{cyc}_relres'] = res['relres']
+        meta[f'{cyc}_setup_s'] = res['setup_s']
+        meta[f'{cyc}_solve_s'] = res['solve_s']
+        x = run_ref.load(wd, 'x', ranks)
+        arr[f'x_{cyc}'] = x.reshape(N * N, 2)[sites]
+        run_ref.cleanup(wd)
+    save(name, meta, **arr)
+
+
 def main(names=()):
     if names:
         for nm in names:
-            {'g32_methods': g32_methods}[nm]()
+            if nm.startswith('g256_solve='):
+                big_solve('g256_solve', 256, nm.split('=', 1)[1], ('K', 'V'))
+            elif nm.startswith('g512_solve='):
+                big_solve('g512_solve', 512, nm.split('=', 1)[1], ('K',))
+            else:
+                {'g32_methods': g32_methods}[nm]()
         return
     g32_methods()
     # ---- 16^2 operator fixtures -------------------------------------------------------

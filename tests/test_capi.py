@@ -33,8 +33,30 @@ def test_params_keywords():
 
 
 def test_bad_arguments_are_status_codes():
-    assert ddamg.lib.dd_default_params(None, 2) == ddamg.DD_ERR_ARG
+    sz = C.sizeof(ddamg.Params)
+    assert ddamg.lib.dd_default_params_sized(None, sz, 2) == ddamg.DD_ERR_ARG
     p = ddamg.Params()
-    assert ddamg.lib.dd_default_params(C.byref(p), 1) == ddamg.DD_ERR_ARG
+    assert ddamg.lib.dd_default_params_sized(C.byref(p), sz, 1) == ddamg.DD_ERR_ARG
     assert ddamg.lib.dd_create(None, 8, 8, 0) == ddamg.DD_ERR_ARG
     assert ddamg.lib.dd_solve(None, 1, None, None, None, None) == ddamg.DD_ERR_ARG
+
+
+def test_params_mirror_matches_library():
+    assert C.sizeof(ddamg.Params) == ddamg.lib.dd_params_size()
+    assert ddamg.params(2).size == C.sizeof(ddamg.Params)
+
+
+def test_params_of_another_layout_are_rejected():
+    """A caller compiled against a dd_params of another size gets DD_ERR_ARG and its
+    struct is left untouched (no write past its end); dd_setup refuses a struct whose
+    size field is not the library's."""
+    sz = C.sizeof(ddamg.Params)
+    for wrong in (sz - 8, sz - 4, sz + 8):
+        buf = (C.c_ubyte * (sz + 64))(*([0xA5] * (sz + 64)))
+        assert ddamg.lib.dd_default_params_sized(C.cast(buf, C.POINTER(ddamg.Params)), wrong, 3) == ddamg.DD_ERR_ARG
+        assert bytes(buf) == bytes([0xA5] * (sz + 64))
+        assert 'dd_params' in ddamg.lib.dd_last_error().decode()
+    p = ddamg.params(3)
+    p.size = sz - 4
+    assert ddamg.lib.dd_setup(None, C.byref(p)) == ddamg.DD_ERR_ARG
+    assert 'bytes' in ddamg.lib.dd_last_error().decode()
