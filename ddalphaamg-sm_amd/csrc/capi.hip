@@ -2,10 +2,18 @@
 // include/ddamg.h).  Converts between the boundary format (complex128, reference
 // interior order) and the device layouts, and maps exceptions to status codes.
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 
 #include "solver.h"
+
+// Device scratch of dd_dot_dev_noalloc: the partial sums of up to 64 systems x 256
+// blocks (256 KiB per GPU, held until the process ends).  Host code never uses this
+// name as an address: it takes the per-device address from hipGetSymbolAddress.
+static constexpr int kNoAllocSystems = 64;
+static constexpr size_t kNoAllocScratchBytes = sizeof(dd::cplx<double>) * dd::kMaxRedBlocks * kNoAllocSystems;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 struct dd_ctx {
     std::unique_ptr<dd::Solver> s;
@@ -83,6 +91,22 @@ void order_after_caller(dd_ctx* c) {
     if (!c->ordered) DD_HIP(hipEventCreateWithFlags(&c->ordered, hipEventDisableTiming));
     DD_HIP(hipEventRecord(c->ordered, (hipStream_t)c->caller));
     DD_HIP(hipStreamWaitEvent(c->s->s, c->ordered, 0));
+}
+
+// address of `scratch` on the current device, looked up once per device
+void* scratch_on_current_device() {
+    static std::mutex mu;
+    static void* ptr[256] = {};
+    int dev = 0;
+    DD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) throw dd::BadShape("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        DD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = p;
+    }
+    return ptr[dev];
 }
 
 template <typename R>
@@ -459,6 +483,28 @@ int dd_dot_dev(dd_ctx* c, long long n, int nrhs, const double* a, const double* 
                                 dd::full_mask(nrhs), rb);
         dd::dot_final<double>(s, rb, (cplx<double>*)out, 1, nrhs, dd::full_mask(nrhs));
         DD_HIP(hipStreamSynchronize(s));
+        return DD_OK;
+    });
+}
+
+int dd_dot_dev_noalloc(dd_ctx* c, long long n, int nrhs, const double* a, const double* b, double* out) {
+    if (check(c) || !a || !b || !out || n < 1) return DD_ERR_ARG;
+    if (nrhs < 1 || nrhs > kNoAllocSystems) {
+        set_err("dd_dot_dev_noalloc: nrhs " + std::to_string(nrhs) + " outside [1, " +
+                std::to_string(kNoAllocSystems) + "] (the fixed device scratch holds " +
+                std::to_string(kNoAllocSystems) + " systems)");
+        return DD_ERR_ARG;
+    }
+    return guard([&] {
+        DD_HIP(hipSetDevice(c->s->device));
+        dd::RedBuf rb;
+        rb.partial = scratch_on_current_device();
+        // every partial slot the final pass reads is written by the first pass: nothing
+        // in the scratch needs clearing between calls
+        hipStream_t s = (hipStream_t)c->caller;
+        dd::dot_partial<double>(s, (const cplx<double>*)a, (const cplx<double>*)b, (size_t)n, (size_t)n, nrhs,
+                                dd::full_mask(nrhs), rb);
+        dd::dot_final<double>(s, rb, (cplx<double>*)out, 1, nrhs, dd::full_mask(nrhs));
         return DD_OK;
     });
 }

@@ -70,6 +70,7 @@ _SIGS = {
     'dd_apply_sigma3': (C.c_int, [_ctx, _dp, _dp, C.c_int]),
     'dd_dot': (C.c_int, [_ctx, C.c_longlong, C.c_int, _dp, _dp, _dp]),
     'dd_dot_dev': (C.c_int, [_ctx, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    'dd_dot_dev_noalloc': (C.c_int, [_ctx, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     'dd_create': (C.c_int, [C.POINTER(_ctx), C.c_int, C.c_int, C.c_int]),
     'dd_destroy': (C.c_int, [_ctx]),
     'dd_set_gauge': (C.c_int, [_ctx, _dp]),

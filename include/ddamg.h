@@ -221,6 +221,15 @@ int dd_apply_sigma3(dd_ctx* ctx, const double* in, double* out, int nrhs);
  * hooks, include/fgmres.h:97-98).  dd_dot_dev: device pointers, out on the device. */
 int dd_dot(dd_ctx* ctx, long long n, int nrhs, const double* a, const double* b, double* out);
 int dd_dot_dev(dd_ctx* ctx, long long n, int nrhs, const double* a, const double* b, double* out);
+/* dd_dot_dev for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops); no counterpart in the reference.  Same result as
+ * dd_dot_dev, bit for bit.  It allocates nothing: its partial sums live in a fixed
+ * device array of the library (256 KiB per GPU, kept until the process ends), so nrhs
+ * must be at most 64 (DD_ERR_ARG otherwise).  The kernels go to the stream set with
+ * dd_set_stream (the null stream by default) and the call returns without waiting:
+ * the caller synchronises before reading `out`.  Calls that overlap in time on one GPU
+ * share that array and race. */
+int dd_dot_dev_noalloc(dd_ctx* ctx, long long n, int nrhs, const double* a, const double* b, double* out);
 
 /* grad[x][t][mu] += scale * sum_r Re[ y_r^H (dD/dtheta_mu(x)) x_r ] for U = e^{i theta};
  * with scale = -1 this is dL/dtheta of L = Re<eta, D^-1 b> for y = D^-dagger eta,
