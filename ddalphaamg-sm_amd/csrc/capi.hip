@@ -317,6 +317,53 @@ int dd_setup(dd_ctx* c, const dd_params* p) {
     });
 }
 
+int dd_setup_storage(dd_ctx* c, const dd_params* p) {
+    if (check_params(p) || check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        c->s->setup_storage(*p);
+        return DD_OK;
+    });
+}
+
+int dd_set_test_vectors(dd_ctx* c, int level, const double* tv, int precision) {
+    if (check(c) || !tv) return DD_ERR_ARG;
+    return guard([&] {
+        auto run = [&](auto tag) {
+            using R = decltype(tag);
+            auto* m = mg<R>(c);
+            if (level < 0 || level >= m->levels() - 1) throw dd::BadShape("test vectors exist below the coarsest level");
+            const dd::LevelGeom& G = m->geom(level);
+            hipStream_t s = c->s->s;
+            const size_t tot = (size_t)G.V() * G.nv * G.dof;
+            Staged a(tv, tot, s);
+            DevMem t(sizeof(cplx<R>) * tot);
+            dd::convert<R, double>(s, t.as<cplx<R>>(), a.p(), tot, 0, 0, 1, 1);
+            m->set_test_vectors(level, t.as<cplx<R>>());
+        };
+        if (precision == 0) run(double());
+        else run(float());
+        return DD_OK;
+    });
+}
+
+int dd_orthonormalize(dd_ctx* c, int level, int precision) {
+    if (check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        if (precision == 0) mg<double>(c)->orthonormalize_level(level);
+        else mg<float>(c)->orthonormalize_level(level);
+        return DD_OK;
+    });
+}
+
+int dd_make_coarse_links(dd_ctx* c, int level, int precision) {
+    if (check(c)) return DD_ERR_ARG;
+    return guard([&] {
+        if (precision == 0) mg<double>(c)->galerkin_level(level);
+        else mg<float>(c)->galerkin_level(level);
+        return DD_OK;
+    });
+}
+
 double dd_setup_seconds(dd_ctx* c) { return (c && c->s) ? c->s->setup_seconds : -1.0; }
 
 int dd_solve_dev(dd_ctx* c, int nrhs, const double* b, double* x, int* iters, dd_solve_info* info) {
@@ -414,6 +461,19 @@ int dd_solve_gmres(dd_ctx* c, int nrhs, const double* b, const double* x0, doubl
         Staged bb(b, n, s), xx(x0, n, s);
         if (!x0) DD_HIP(hipMemsetAsync(xx.p(), 0, sizeof(double) * 2 * n, s));
         const int st = c->s->solve_gmres(nrhs, bb.p(), xx.p(), xx.p(), m0, m, restarts, tol, sap_bx, sap_bt, iters, info);
+        download(x, xx.p(), n, s);
+        return st;
+    });
+}
+
+int dd_sap_fine(dd_ctx* c, int nrhs, const double* v, double* x, int nu, double tol, int sap_bx, int sap_bt,
+                double m0, int* iters, double* relres) {
+    if (check(c) || !v || !x || !iters || nrhs < 1 || nrhs > 64 || nu < 0 || !(tol >= 0)) return DD_ERR_ARG;
+    return guard([&] {
+        hipStream_t s = c->s->s;
+        const size_t n = vsize(c) * nrhs;
+        Staged vv(v, n, s), xx(x, n, s);
+        const int st = c->s->sap_fine(nrhs, vv.p(), xx.p(), nu, tol, sap_bx, sap_bt, m0, iters, relres);
         download(x, xx.p(), n, s);
         return st;
     });
@@ -720,12 +780,16 @@ int dd_get_links(dd_ctx* c, int level, double* out, int precision) {
         auto run = [&](auto tag) {
             using R = decltype(tag);
             auto* m = mg<R>(c);
-            if (level < 1 || level >= m->levels()) throw dd::BadShape("links exist on levels >= 1");
+            if (level < 0 || level >= m->levels()) throw dd::BadShape("bad level");
             const dd::LevelGeom& G = m->geom(level);
             hipStream_t s = c->s->s;
             const size_t tot = (size_t)G.V() * 5 * G.dof * G.dof;
             DevMem o(sizeof(cplx<double>) * tot);
-            if (precision == 2) {
+            if (level == 0) {   // the Wilson-Dirac link matrices (no A)
+                DevMem l0(sizeof(cplx<R>) * tot);
+                m->dirac_links_level0(l0.as<cplx<R>>());
+                dd::convert<double, R>(s, o.as<cplx<double>>(), l0.as<cplx<R>>(), tot, 0, 0, 1, 1);
+            } else if (precision == 2) {
                 if constexpr (sizeof(R) == 4) {
                     if (!m->hlinks(level)) throw dd::BadShape("no binary16 link copy on this level");
                     dd::from_half(s, m->hlinks(level), o.as<double>(), 2 * tot);

@@ -299,4 +299,63 @@ int Solver::solve_gmres(int nrhs, const cplx<double>* b, const cplx<double>* x0,
     return conv == all ? DD_OK : DD_ERR_NOT_CONVERGED;
 }
 
+// SAP_fine_level::SAP(v, x, nu, tol) as a stand-alone solver (src/sap.cpp:81-165;
+// Methods::SAP, src/methods.cpp:61-78): r = v - D x, then per iteration the red blocks,
+// r = v - D x, the black blocks, r = v - D x, and the test ||r|| < tol ||v||, which ends
+// the system's iterations with the count i (0-based), as the reference returns it; nu
+// when it never holds.  Blocks of sap_bx x sap_bt sites, solved by GMRES(5) x 5 to 1e-3
+// from the block right-hand side (the SAPV defaults, src/variables.cpp:100-105).
+int Solver::sap_fine(int nrhs, const cplx<double>* v, cplx<double>* x, int nu, double tol, int sap_bx, int sap_bt,
+                     double m0, int* iters, double* relres) {
+    if (!have_gauge) throw BadShape("SAP before the gauge field");
+    if (nrhs < 1 || nrhs > 64) throw BadShape("SAP: 1..64 systems");
+    if (sap_bx < 1 || sap_bt < 1 || Nx % sap_bx || Nt % sap_bt) throw BadShape("SAP: block size must divide the lattice");
+    const int nbx = Nx / sap_bx, nbt = Nt / sap_bt, S = sap_bx * sap_bt;
+    if ((nbx * nbt) % 2 || ((nbt & 1) && (nbx & 1))) throw BadShape("SAP needs an even number of blocks");
+    if (S > 64 || (S & (S - 1))) throw BadShape("SAP: blocks of a power-of-two number of sites, at most 64");
+    const size_t n = (size_t)Nx * Nt * 2;
+    const cplx<double>* U = U64.as<cplx<double>>();
+    DevMem rbuf(sizeof(cplx<double>) * n * nrhs), nrm(sizeof(cplx<double>) * 2 * nrhs),
+        part(dot_scratch_bytes(nrhs));
+    PinMem nh;
+    nh.alloc(sizeof(cplx<double>) * 2 * nrhs);
+    RedBuf rb;
+    rb.partial = part.as<void>();
+    cplx<double>* r = rbuf.as<cplx<double>>();
+    cplx<double>* nd = nrm.as<cplx<double>>();
+    cplx<double>* hn = nh.as<cplx<double>>();
+    uint64_t run = full_mask(nrhs);
+    dot_partial<double>(s, v, v, n, n, nrhs, run, rb);
+    dot_final<double>(s, rb, nd, 2, nrhs, run);
+    wilson<double>(s, U, x, r, v, Nx, Nt, m0, false, nrhs, run, n);
+    DD_HIP(hipMemcpyAsync(hn, nd, sizeof(cplx<double>) * 2 * nrhs, hipMemcpyDeviceToHost, s));
+    DD_HIP(hipStreamSynchronize(s));
+    std::vector<double> vnorm(nrhs), err(nrhs, 0.0);
+    for (int q = 0; q < nrhs; q++) {
+        vnorm[q] = std::sqrt(hn[2 * q].x);
+        iters[q] = nu;
+    }
+    for (int i = 0; i < nu && run; i++) {
+        for (int colour = 0; colour < 2; colour++) {
+            sap_blocks0<double>(s, U, r, x, Nx, Nt, sap_bx, sap_bt, colour, m0, 5, 5, 1e-3, nrhs, run, n, nullptr);
+            wilson<double>(s, U, x, r, v, Nx, Nt, m0, false, nrhs, run, n);
+        }
+        dot_partial<double>(s, r, r, n, n, nrhs, run, rb);
+        dot_final<double>(s, rb, nd + 1, 2, nrhs, run);
+        DD_HIP(hipMemcpyAsync(hn, nd, sizeof(cplx<double>) * 2 * nrhs, hipMemcpyDeviceToHost, s));
+        DD_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < nrhs; q++) {
+            if (!((run >> q) & 1)) continue;
+            err[q] = std::sqrt(hn[2 * q + 1].x);
+            if (err[q] < tol * vnorm[q]) {
+                iters[q] = i;
+                run &= ~(1ull << q);
+            }
+        }
+    }
+    if (relres)
+        for (int q = 0; q < nrhs; q++) relres[q] = vnorm[q] > 0 ? err[q] / vnorm[q] : 0.0;
+    return run ? DD_ERR_NOT_CONVERGED : DD_OK;
+}
+
 }  // namespace dd

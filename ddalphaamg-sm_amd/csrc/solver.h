@@ -97,6 +97,18 @@ class Multigrid {
     // l+1 .. from the prolongators.  Needs a setup with p.bootstrap > 0 (which keeps
     // those vectors).
     void bootstrap(int passes);
+    // the per-level steps of the setup phase, for callers that build the hierarchy step
+    // by step (Level::orthonormalize / makeCoarseLinks, src/level.cpp:193-284, 431-542):
+    // zero every level's test vectors and coarse operators; install level l's test
+    // vectors from tv [nv][V][dof]; orthonormalise them per aggregate and spin; build
+    // level l+1's operator as the Galerkin product of level l's
+    void clear_storage();
+    void set_test_vectors(int l, const cplx<R>* tv);
+    void orthonormalize_level(int l);
+    void galerkin_level(int l);
+    // level-0 link matrices of the Wilson-Dirac operator in the coarse-link layout
+    // [V][5][2][2] (A = 0), as Level::makeDirac builds them (src/level.cpp:286-331)
+    void dirac_links_level0(cplx<R>* out);
     void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
@@ -170,6 +182,9 @@ class Solver {
     ~Solver();
     void set_gauge(const cplx<double>* U_dev);
     void setup(const dd_params& p);
+    // allocates the hierarchy of p without running the setup phase (test vectors and
+    // coarse operators zero); the caller fills it with the Multigrid per-level steps
+    void setup_storage(const dd_params& p);
     // b, x: device pointers, nrhs systems of V*2 entries back to back
     // use_x0: x holds the initial guess on entry (otherwise x0 = 0)
     int solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, dd_solve_info* info,
@@ -192,6 +207,12 @@ class Solver {
                        int max_iter, double tol, int* iters, dd_solve_info* info);
     int solve_gmres(int nrhs, const cplx<double>* b, const cplx<double>* x0, cplx<double>* x, double m0, int m,
                     int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info);
+    //   sap_fine:        SAP_fine_level::SAP(v, x, nu, tol) as a stand-alone solver on
+    //                    sap_bx x sap_bt site blocks (src/sap.cpp:81-165), x in/out;
+    //                    iters[r] is the reference's return value, relres[r] the last
+    //                    ||v - D x|| / ||v||
+    int sap_fine(int nrhs, const cplx<double>* v, cplx<double>* x, int nu, double tol, int sap_bx, int sap_bt,
+                 double m0, int* iters, double* relres);
     int Nx, Nt, device;
     hipStream_t s = nullptr;
     DevMem U64, U32, theta;

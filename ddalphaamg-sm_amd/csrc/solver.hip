@@ -570,6 +570,54 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
     }
 }
 
+template <typename R>
+void Multigrid<R>::clear_storage() {
+    for (int l = 0; l < levels(); l++) {
+        if (W_[l].bytes()) zero<R>(s_, W_[l].as<cplx<R>>(), W_[l].bytes() / sizeof(cplx<R>), 0, 1, 1);
+        if (links_[l].bytes()) zero<R>(s_, links_[l].as<cplx<R>>(), links_[l].bytes() / sizeof(cplx<R>), 0, 1, 1);
+        if (hlinks_[l].bytes()) DD_HIP(hipMemsetAsync(hlinks_[l].as<void>(), 0, hlinks_[l].bytes(), s_));
+    }
+    DD_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename R>
+void Multigrid<R>::set_test_vectors(int l, const cplx<R>* tv) {
+    if (l < 0 || l >= levels() - 1) throw BadShape("test vectors exist on levels below the coarsest");
+    const LevelGeom& G = g_[l];
+    transpose_tv<R>(s_, tv, W_[l].as<cplx<R>>(), G.V(), G.nv, G.dof, true);
+    DD_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename R>
+void Multigrid<R>::orthonormalize_level(int l) {
+    if (l < 0 || l >= levels() - 1) throw BadShape("test vectors exist on levels below the coarsest");
+    const LevelGeom& G = g_[l];
+    orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
+    DD_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename R>
+void Multigrid<R>::dirac_links_level0(cplx<R>* out) {
+    dirac_links<R>(s_, U_, out, g_[0].Nx, g_[0].Nt);
+}
+
+template <typename R>
+void Multigrid<R>::galerkin_level(int l) {
+    if (l < 0 || l >= levels() - 1) throw BadShape("no coarser level to build");
+    const LevelGeom& G = g_[l];
+    DevMem L0;
+    if (l == 0) {
+        L0.alloc(sizeof(cplx<R>) * (size_t)G.V() * 5 * 4);
+        dirac_links_level0(L0.as<cplx<R>>());
+    }
+    const cplx<R>* Lf = (l == 0) ? L0.as<cplx<R>>() : links(l);
+    galerkin<R>(s_, Lf, W(l), links_[l + 1].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv, l > 0);
+    if (hlinks_[l + 1].bytes())
+        to_half(s_, reinterpret_cast<const float*>(links(l + 1)), hlinks_[l + 1].as<_Float16>(),
+                hlinks_[l + 1].bytes() / sizeof(_Float16));
+    DD_HIP(hipStreamSynchronize(s_));
+}
+
 // Bootstrap adaptivity as the reference's documentation describes it (AMG_Documentation,
 // section 3: "use the current multilevel method defined by P to approximately solve
 // D x = (w_i - D w_i) and update w_i <- x; the resultant test vectors are orthonormalized
@@ -800,6 +848,23 @@ void Solver::setup(const dd_params& p) {
     }
     DD_HIP(hipStreamSynchronize(s));
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    have_setup = true;
+}
+
+void Solver::setup_storage(const dd_params& p) {
+    if (!have_gauge) throw BadShape("setup before the gauge field");
+    prm = p;
+    mg64.reset();
+    mg32.reset();
+    outer_.reset();
+    if (p.precision == 0) {
+        mg64.reset(new Multigrid<double>(s, Nx, Nt, p, U64.as<cplx<double>>()));
+        mg64->clear_storage();
+    } else {
+        mg32.reset(new Multigrid<float>(s, Nx, Nt, p, U32.as<cplx<float>>()));
+        mg32->clear_storage();
+    }
+    setup_seconds = 0;
     have_setup = true;
 }
 

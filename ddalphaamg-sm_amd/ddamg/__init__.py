@@ -70,6 +70,11 @@ _SIGS = {
     'dd_apply_sigma3': (C.c_int, [_ctx, _dp, _dp, C.c_int]),
     'dd_dot': (C.c_int, [_ctx, C.c_longlong, C.c_int, _dp, _dp, _dp]),
     'dd_dot_dev': (C.c_int, [_ctx, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    'dd_setup_storage': (C.c_int, [_ctx, C.POINTER(Params)]),
+    'dd_set_test_vectors': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
+    'dd_orthonormalize': (C.c_int, [_ctx, C.c_int, C.c_int]),
+    'dd_make_coarse_links': (C.c_int, [_ctx, C.c_int, C.c_int]),
+    'dd_sap_fine': (C.c_int, [_ctx, C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _dp]),
     'dd_dot_dev_noalloc': (C.c_int, [_ctx, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     'dd_create': (C.c_int, [C.POINTER(_ctx), C.c_int, C.c_int, C.c_int]),
     'dd_destroy': (C.c_int, [_ctx]),
@@ -434,6 +439,33 @@ class Context:
         out = np.zeros((nv, Nx, Nt, d), complex)
         _chk(lib.dd_get_test_vectors(self._h, level, _cptr(out), precision))
         return out
+
+    def setup_storage(self, prm: Params):
+        """Allocates the hierarchy of prm without running the setup phase."""
+        _chk(lib.dd_setup_storage(self._h, C.byref(prm)))
+        self.prm = prm
+
+    def set_test_vectors(self, level, tv, precision=0):
+        tv = _c128(tv)
+        _chk(lib.dd_set_test_vectors(self._h, level, _cptr(tv), precision))
+
+    def orthonormalize(self, level, precision=0):
+        _chk(lib.dd_orthonormalize(self._h, level, precision))
+
+    def make_coarse_links(self, level, precision=0):
+        _chk(lib.dd_make_coarse_links(self._h, level, precision))
+
+    def sap_fine(self, v, x0, nu, tol, block, m0):
+        """Stand-alone red-black SAP (dd_sap_fine) on level-0 vectors; returns (x, iters,
+        relres) per system."""
+        v, x = _c128(v), _c128(x0).copy()
+        vb = v if v.ndim == 4 else v[None]
+        xb = x if x.ndim == 4 else x[None]
+        its = (C.c_int * vb.shape[0])()
+        rr = np.zeros(vb.shape[0])
+        _chk(lib.dd_sap_fine(self._h, vb.shape[0], _cptr(vb), _cptr(xb), nu, tol, block[0], block[1], m0, its,
+                             _cptr(rr)), allow=(DD_OK, DD_ERR_NOT_CONVERGED))
+        return (xb if v.ndim == 4 else xb[0]), np.array(list(its)), rr
 
     def links(self, level, precision=0):
         """(A, B[2], C[2]) of level >= 1, each (Nx, Nt, dof, dof); precision 0 FP64, 1 FP32,

@@ -7,15 +7,43 @@
 #ifndef VARIABLES_H_INCLUDED
 #define VARIABLES_H_INCLUDED
 #include <complex>
+#include <cstdlib>
 #include <iomanip>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "config.h"
+#include "ddamg_mpi.h"
 
 typedef std::complex<double> c_double;
 extern double pi;
 extern c_double I_number;
+
+// Floating-point operation counts per complex128 operation (include/variables.h:16-25);
+// the host-side algorithms (FGMRES and SAP_C over caller-defined hooks) add them to
+// localFLOPS as the reference does.  The device paths add nothing.
+constexpr long long int ca = 2;    // complex addition
+constexpr long long int cm = 6;    // complex multiplication
+constexpr long long int cd = 11;   // complex division
+constexpr long long int da = 1;    // double addition
+constexpr long long int dm = 1;    // double multiplication
+constexpr long long int dd = 1;    // double division
+constexpr long long int dcm = 2;   // double * complex
+constexpr long long int dcd = 7;   // double / complex
+constexpr long long int dsq = 1;   // sqrt(double)
+
+extern long long int FLOPS;
+extern long long int localFLOPS;
+
+// Iteration and FLOP counters of the comparison runs (include/variables.h:30-50).
+namespace iter_counters {
+extern long long int kCycleIt, vCycleIt, CGIt, BiCGIt, GMRESIt, SAPIt, FGMRES_SAPIt;
+}
+namespace flop_counters {
+extern long long int kCycleFlops, vCycleFlops, kCycleSetUpFlops, vCycleSetUpFlops, CGFlops, BiCGFLops, GMRESFlops,
+    SAPFlops, FGMRES_SAPFlops;
+}
 
 namespace mass {
 extern double m0;
@@ -23,6 +51,7 @@ extern double m0;
 
 namespace mpi {
 extern int rank, size, maxSize, maxSizeH, sitesH, ranks_x, ranks_t, width_x, width_t, rank2d;
+extern MPI_Comm cart_comm;   // MPI_COMM_WORLD (one process holds the lattice)
 }
 
 namespace LV {

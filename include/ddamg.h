@@ -157,6 +157,22 @@ int dd_apply_D_dev(dd_ctx* ctx, const double* in, double* out, int nrhs, double 
 int dd_setup(dd_ctx* ctx, const dd_params* p);
 double dd_setup_seconds(dd_ctx* ctx);
 
+/* The setup phase step by step, for callers that build the hierarchy themselves as the
+ * reference's Level objects allow (include/level.h:147-233, 334-467):
+ * dd_setup_storage allocates the hierarchy of p (test vectors and coarse operators
+ * zero, no setup phase run); dd_set_test_vectors installs level l's test vectors
+ * (complex128 [nv][V][dof], the layout of dd_get_test_vectors); dd_orthonormalize
+ * orthonormalises them per aggregate and spin (Level::orthonormalize,
+ * src/level.cpp:193-284); dd_make_coarse_links builds level l+1's operator as the
+ * Galerkin product of level l's operator with them (Level::makeCoarseLinks,
+ * src/level.cpp:431-542; level 0's operator is the Wilson-Dirac operator of the gauge
+ * field, Level::makeDirac, src/level.cpp:286-331).  precision selects the FP64 (0) or
+ * FP32 (1) hierarchy, as p->precision made it. */
+int dd_setup_storage(dd_ctx* ctx, const dd_params* p);
+int dd_set_test_vectors(dd_ctx* ctx, int level, const double* tv, int precision);
+int dd_orthonormalize(dd_ctx* ctx, int level, int precision);
+int dd_make_coarse_links(dd_ctx* ctx, int level, int precision);
+
 /* Solves D x = b for nrhs right-hand sides (x0 = 0) with FGMRES preconditioned by the
  * K- or V-cycle.  Replaces FGMRES_AMG_{k,v}_cycle::fgmres, include/amg.h:240-270 and
  * src/fgmres.cpp:5-97.  iters (nullable, nrhs ints) receives each system's count.
@@ -205,6 +221,18 @@ int dd_solve_bicgstab(dd_ctx* ctx, int nrhs, const double* b, const double* x0, 
 int dd_solve_gmres(dd_ctx* ctx, int nrhs, const double* b, const double* x0, double* x, double m0, int m,
                    int restarts, double tol, int sap_bx, int sap_bt, int* iters, dd_solve_info* info);
 
+/* Red-black SAP as a stand-alone solver on the FP64 D(m0) of the context's gauge field
+ * (no dd_setup needed): x (in: initial guess, out: result) after at most nu iterations,
+ * each system stopping once ||v - D x|| < tol ||v||; blocks of sap_bx x sap_bt sites (a
+ * power of two, at most 64, an even number of blocks), each solved by GMRES(5) x 5 to
+ * 1e-3.  iters[r] is SAP_C::SAP's return value (the 0-based iteration at which the test
+ * held, or nu); relres (nullable) the final ||v - D x|| / ||v||.  Replaces
+ * SAP_fine_level::SAP, include/sap.h:192-234 and src/sap.cpp:81-165, as Methods::SAP
+ * calls it (src/methods.cpp:61-78; there SAPV-style blocks of Nx/(ranks_x*xblocks) sites).
+ * Returns DD_ERR_NOT_CONVERGED if a system did not reach tol. */
+int dd_sap_fine(dd_ctx* ctx, int nrhs, const double* v, double* x, int nu, double tol, int sap_bx, int sap_bt,
+                double m0, int* iters, double* relres);
+
 /* Adjoint solve D^dagger y = eta on the same hierarchy through sigma3-hermiticity:
  * y = sigma3 D^-1 sigma3 eta.  The reference has D_dagger_phi only
  * (src/dirac_operator.cpp:56-85); no adjoint solver. */
@@ -250,7 +278,8 @@ int dd_prolong(dd_ctx* ctx, int level, const double* in, double* out, int nrhs, 
 int dd_sap(dd_ctx* ctx, int level, const double* v, double* x, int nu, double tol, int nrhs, int precision);
 /* Test vectors of level l (after setup): out[nv][V][dof]. */
 int dd_get_test_vectors(dd_ctx* ctx, int level, double* out, int precision);
-/* Links of level l >= 1 as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1);
+/* Links of level l as [V][5][dof][dof] row-major (A, B_0, B_1, C_0, C_1; at level 0 the
+   Wilson-Dirac ones, A = 0, B_mu = (1 - sigma_mu) U_mu / 2, C_mu = (1 + sigma_mu) U*_mu / 2);
    precision 0 FP64, 1 FP32, 2 the FP32 hierarchy's binary16 copy read by the
    matrix-core coarse SAP (coarse_sap_links = 1). */
 int dd_get_links(dd_ctx* ctx, int level, double* out, int precision);

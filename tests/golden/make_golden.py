@@ -25,6 +25,8 @@ Fixtures:
               residual and the solution at 512 sampled sites
               (both need the field as .npy: python tests/golden/make_golden.py
                g256_solve=theta_256.npy g512_solve=theta_512.npy)
+  g64_defaults g64_solve's inputs with the reference's default knobs (nu2=2, FGMRES(20)):
+              K- and V-cycle iteration counts, as the reference's main() runs them
   g32_methods the comparison solvers of src/methods.cpp on a 32^2 field (m0=-0.1, tol
               1e-10, 2x2 ranks so that FGMRES_SAP's 4x4 blocks per rank are 4x4 sites):
               CG on the normal equations, BiCGStab, GMRES(20) and FGMRES(20)+SAP --
@@ -119,6 +121,28 @@ def big_solve(name, N, theta_path, cycles):
     save(name, meta, **arr)
 
 
+def g64_defaults():
+    """The g64_solve field and right-hand side solved with the reference's default knobs
+    (nu2 = 2, FGMRES(20), src/variables.cpp:90-134), as its own main() runs them: K- and
+    V-cycle iteration counts and residuals (JSON only; the inputs are g64_solve's)."""
+    meta0 = json.load(open(os.path.join(OUT, 'g64_solve.json')))
+    a = dict(np.load(os.path.join(OUT, 'g64_solve.npz')))
+    U = np.exp(1j * a['theta'])
+    levels = [orc.LevelSpec(tuple(l['agg']), tuple(l['sap_block']), l['nv']) for l in meta0['levels']]
+    meta = dict(N=meta0['N'], m0=meta0['m0'], ranks=meta0['ranks'], levels=meta0['levels'], nu1=0, nu2=2,
+                fgmres_m=20, fgmres_restarts=50, tol=1e-10, inputs='g64_solve')
+    for cyc in ('K', 'V'):
+        wd, _ = run_ref.run('solve', U, a['rhs'], levels, meta0['m0'], ranks=tuple(meta0['ranks']), nu2=2,
+                            fgmres_m=20, cycle=cyc, dump_x=0)
+        res = run_ref.result(wd)
+        meta[f'{cyc}_iterations'] = res['iterations']
+        meta[f'{cyc}_relres'] = res['relres']
+        run_ref.cleanup(wd)
+    with open(os.path.join(OUT, 'g64_defaults.json'), 'w') as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print('wrote g64_defaults', meta)
+
+
 def main(names=()):
     if names:
         for nm in names:
@@ -127,7 +151,7 @@ def main(names=()):
             elif nm.startswith('g512_solve='):
                 big_solve('g512_solve', 512, nm.split('=', 1)[1], ('K',))
             else:
-                {'g32_methods': g32_methods}[nm]()
+                {'g32_methods': g32_methods, 'g64_defaults': g64_defaults}[nm]()
         return
     g32_methods()
     # ---- 16^2 operator fixtures -------------------------------------------------------
