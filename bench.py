@@ -65,10 +65,15 @@ def main():
                     help='skip the secondary rows (256^2 plain FP64, 256^2 32-RHS and the ragged batch)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='only run three device copies of 256 MiB (scale of the HBM byte counters)')
+    ap.add_argument('--probe', choices=['wilson', 'sap0'], default=None,
+                    help='only run one kernel family on 32 FP64 systems at --N (counter runs): the Wilson-Dirac '
+                         'stencil 5 times, or one level-0 SAP iteration (4x4-site blocks)')
     ap.add_argument('--coarse-basis', type=int, choices=[0, 1], default=None,
                     help='storage of the coarsest GMRES basis in the FP32 hierarchy (0 FP32, 1 binary16)')
     ap.add_argument('--coarse-sap-links', type=int, choices=[0, 1], default=None,
                     help='links read by the matrix-core coarse SAP of the FP32 hierarchy (0 FP32, 1 binary16 copy)')
+    ap.add_argument('--skip-fp32-basis', action='store_true',
+                    help='skip the row that repeats the batch with an FP32 coarsest Krylov basis')
     ap.add_argument('--bootstrap', type=int, default=0,
                     help='bootstrap passes after the setup (0: the reference setup only)')
     args = ap.parse_args()
@@ -90,6 +95,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log('calibration copy: 3 x (268435456 bytes read + 268435456 bytes written)')
+        return
+
+    if args.probe:
+        probe(ddamg, torch, args.probe, args.N, args.nrhs, args.seed)
         return
 
     N, nrhs = args.N, args.nrhs
@@ -164,6 +173,23 @@ def main():
     nsolve = args.warmup + args.steps
     coarsest_mixed = {k: (v / nsolve if k not in ('persistent',) else v) for k, v in cst.items()}
     log('coarsest solver per outer solve: ' + json.dumps(coarsest_mixed))
+    # the same batch with the coarsest Krylov basis stored in FP32 instead of binary16
+    fp32_basis = None
+    if not args.skip_mixed and not args.skip_fp32_basis and (COARSE_BASIS is None or COARSE_BASIS == 1):
+        p32 = hierarchy_params(ddamg, N, 1, nrhs, m0)
+        p32.coarse_basis = 0
+        ctx.setup(p32)
+        x32 = torch.zeros_like(b_dev)
+        ctx.solve_dev(b_dev.data_ptr(), x32.data_ptr(), nrhs)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its32, info32 = ctx.solve_dev(b_dev.data_ptr(), x32.data_ptr(), nrhs)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t0
+        fp32_basis = {'ms_per_rhs': t32 * 1e3 / nrhs, 'outer_iterations': [int(its32.min()), int(its32.max())],
+                      'max_relres': info32.relres,
+                      'ms_per_outer_iteration': t32 * 1e3 / max(int(its32.max()), 1)}
+        log(f'mixed, FP32 coarsest basis: {t32 * 1e3:.1f} ms per solve, its {its32.min()}..{its32.max()}')
     ctx.close()
 
     # ---- plain FP64, one RHS -------------------------------------------------------------
@@ -176,41 +202,50 @@ def main():
         if not args.skip_mixed:
             ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        its_plain, info_p = ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
-        torch.cuda.synchronize()
-        t_plain = time.perf_counter() - t0
+        plain_times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            its_plain, info_p = ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
+            torch.cuda.synchronize()
+            plain_times.append(time.perf_counter() - t0)
+        t_plain = float(np.median(plain_times))
         x64 = xp.cpu().numpy().view(np.complex128).reshape(N, N, 2)
         err_vs_fp64 = float(np.linalg.norm(x_mixed[0] - x64) / np.linalg.norm(x64))
         log(f'plain FP64: setup {setup_plain:.3f}s, solve {t_plain * 1e3:.1f} ms, its {its_plain[0]}, '
             f'relres {info_p.relres:.3e}, mixed-vs-FP64 rel. error {err_vs_fp64:.3e}')
         cst = ctx.coarsest_stats()
-        nsolve = 1 if args.skip_mixed else 2
+        nsolve = 3 if args.skip_mixed else 4
         coarsest_plain = {k: (v / nsolve if k not in ('persistent',) else v) for k, v in cst.items()}
         log('coarsest solver per plain solve: ' + json.dumps(coarsest_plain))
         ctx.close()
 
-    secondary = None if args.no_secondary else secondary_rows(ddamg, torch, args.sweeps, args.seed)
+    secondary, field256 = (None, None) if args.no_secondary else secondary_rows(ddamg, torch, args.sweeps, args.seed)
 
     # ---- CPU baseline: the reference's own solve on the host cores --------------------------
     cpu = None
     if args.cpu_baseline == 'reference':
         cpu = cpu_reference(theta, B[0], N, m0, args.cpu_ranks)
+        if field256 is not None:   # the 256^2 plain FP64 configuration on the same host cores
+            cpu['cfg_256'] = cpu_reference(field256['theta'], field256['b0'], 256, -0.18, args.cpu_ranks)
 
     ms_rhs = t_mixed * 1e3 / nrhs
     line = {
         'metric': 'ms/RHS to 1e-10 rel. residual at 512^2, mixed+batched vs plain FP64 on GPU',
         'value': ms_rhs, 'unit': 'ms/RHS', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': t_mixed * 1e3, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f32 hierarchy / f64 outer (complex)', 'data': 'synthetic (seeded Metropolis gauge field, Gaussian RHS)',
+        'dtype': 'c64 hierarchy (FP32 SAP, coarse operators, test vectors) with a binary16-stored coarsest Krylov '
+                 'basis (products in FP32), c128 outer FGMRES and residual', 'data': 'synthetic (seeded Metropolis gauge field, Gaussian RHS)',
         'config': {'workload': f'DD-alphaAMG K-cycle solve, {N}^2 Schwinger Wilson-Dirac, beta=2, m0={m0}, '
                                f'3 levels, 4x4 aggregates, Nv=12, 4x4 SAP blocks, FGMRES(30), tol 1e-10, '
                                f'{nrhs} RHS per solve', 'lattice': [N, N], 'nrhs': nrhs},
         'outer_iterations': [int(its.min()), int(its.max())],
+        'ms_per_outer_iteration': t_mixed * 1e3 / max(int(its.max()), 1),
+        'mixed_fp32_coarsest_basis': fp32_basis,
         'max_relres': float(relres.max()),
         'setup_s_mixed': setup_mixed,
         'autotuned_coarse_sap_variant': tune[0],
         'plain_fp64_ms_per_rhs': None if t_plain is None else t_plain * 1e3,
+        'plain_fp64_ms_samples': None if t_plain is None else [t * 1e3 for t in plain_times],
         'plain_fp64_iterations': None if its_plain is None else int(its_plain[0]),
         'setup_s_fp64': setup_plain,
         'speedup_vs_plain_fp64': None if t_plain is None else t_plain * 1e3 / ms_rhs,
@@ -225,6 +260,37 @@ def main():
     print(json.dumps(line))
 
 
+def probe(ddamg, torch, which, N, nrhs, seed):
+    """One kernel family alone, for counter runs: k_wilson (5 launches of D on nrhs FP64
+    vectors) or one level-0 SAP iteration (k_sap0 on the red, then the black blocks, with
+    the Wilson residuals between).  Prints the launches' algorithmic bytes as the library
+    counts them."""
+    rng = np.random.default_rng(seed)
+    ctx = ddamg.Context(N, N)
+    ctx.generate_gauge(2.0, 200, seed)
+    v = torch.from_numpy((rng.standard_normal((nrhs, N, N, 2)) + 1j * rng.standard_normal((nrhs, N, N, 2)))
+                         .view(np.float64).copy()).cuda()
+    out = torch.zeros_like(v)
+    kid = ddamg.K_WILSON if which == 'wilson' else ddamg.K_SAP0
+    ddamg.timer_enable(kid, True)
+    ddamg.timer_reset()
+    torch.cuda.synchronize()
+    if which == 'wilson':
+        for _ in range(5):
+            ctx.apply_D_dev(v.data_ptr(), out.data_ptr(), nrhs, -0.18)
+    else:
+        vb = v.cpu().numpy().view(np.complex128).reshape(nrhs, N, N, 2)
+        ctx.sap_fine(vb, np.zeros_like(vb), 1, 1e-12, (4, 4), -0.18)
+    torch.cuda.synchronize()
+    ms, n, by = ddamg.timer_read(kid)
+    ddamg.timer_enable(kid, False)
+    log(f'probe {which}: {n} launches, {ms / max(n, 1) * 1e3:.1f} us each, '
+        f'{by / max(n, 1):.6e} algorithmic bytes per launch, {by / (ms * 1e-3) / 1e9:.1f} GB/s')
+    print(json.dumps({'probe': which, 'N': N, 'nrhs': nrhs, 'launches': n, 'us_per_launch': ms / max(n, 1) * 1e3,
+                      'algorithmic_bytes_per_launch': by / max(n, 1)}))
+    ctx.close()
+
+
 def secondary_rows(ddamg, torch, sweeps, seed):
     """Further workloads: the 256^2 plain FP64 single-RHS solve, 32 RHS on one 256^2 field
     (mixed, batched; K-cycle, and V-cycle eager and from captured graphs), and a ragged
@@ -236,7 +302,7 @@ def secondary_rows(ddamg, torch, sweeps, seed):
 
     def field_ctx(N, s):
         ctx = ddamg.Context(N, N)
-        ctx.generate_gauge(2.0, sweeps, s)
+        ctx.theta = ctx.generate_gauge(2.0, sweeps, s)
         return ctx
 
     def rhs(nr, N):
@@ -246,6 +312,7 @@ def secondary_rows(ddamg, torch, sweeps, seed):
     # 256^2 plain FP64, 1 RHS, and mixed 32 RHS on the same field
     ctx = field_ctx(256, seed + 11)
     b, B = rhs(32, 256)
+    field256 = {'theta': ctx.theta, 'b0': B[0]}
     x = torch.zeros_like(b)
     for prec, nr in ((0, 1), (1, 32)):
         ctx.setup(hierarchy_params(ddamg, 256, prec, nr, -0.18))
@@ -305,18 +372,35 @@ def secondary_rows(ddamg, torch, sweeps, seed):
         f'its {[i.iterations for i in infos]}')
     for c in ctxs:
         c.close()
-    return out
+    return out, field256
+
+
+def sources_sha256(files):
+    """SHA-256 over the contents of the listed source files (paths relative to the
+    repository root), in the listed order."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in files:
+        with open(os.path.join(ROOT, rel), 'rb') as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def measured_traffic(kernel, N, nrhs):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    measurement (profiles/), when it covers this workload; None otherwise."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
-                        f'{kernel}_{N}_mixed{nrhs}_pmc.json')
+    measurement (profiles/), when it covers this workload and was taken on the kernel's
+    current sources (the file records the SHA-256 of the sources it measured); None
+    otherwise."""
+    path = os.path.join(ROOT, 'profiles', f'{kernel}_{N}_mixed{nrhs}_pmc.json')
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)['traffic_bytes_per_dispatch']
+        rec = json.load(f)
+    srcs = rec.get('sources')
+    if not srcs or sources_sha256(srcs) != rec.get('sources_sha256'):
+        log(f'traffic of {kernel}: {os.path.basename(path)} measured other sources; reported as null')
+        return None
+    return rec['traffic_bytes_per_dispatch']
 
 
 def cpu_reference(theta, b, N, m0, ranks_s):
@@ -340,10 +424,16 @@ def cpu_reference(theta, b, N, m0, ranks_s):
     except Exception as e:  # noqa: BLE001
         return {'error': str(e)[-400:]}
     log(f'cpu reference: {res} ({time.perf_counter() - t0:.1f}s wall)')
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
     return {'value': res['solve_s'] * 1e3, 'unit': 'ms/RHS', 'cores': rx * rt, 'kind': 'reference',
             'sample': f'one full reference solve (FGMRES(30)+K-cycle, {N}^2, same field and RHS 0) under '
                       f'mpirun -n {rx * rt} ({rx}x{rt} ranks); setup {res["setup_s"]:.2f}s excluded',
-            'iterations': res['iterations'], 'relres': res['relres'], 'setup_s': res['setup_s']}
+            'iterations': res['iterations'], 'relres': res['relres'], 'setup_s': res['setup_s'],
+            'host_cpus_visible': os.cpu_count(), 'host_cpus_affinity': affinity,
+            'host_cpu_share': os.environ.get('OMP_NUM_THREADS')}
 
 
 if __name__ == '__main__':

@@ -97,18 +97,25 @@ static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1)
 static inline double nact(uint64_t mask, int nrhs) { return (double)__builtin_popcountll(mask & ((nrhs >= 64) ? ~0ull : ((1ull << nrhs) - 1))); }
 
 // ======================================================================================
-// Level-0 Wilson-Dirac stencil.  One thread per site and system.
+// Level-0 Wilson-Dirac stencil.  One thread per site and group of up to RB systems: the
+// four link matrices of the site are loaded once for the group.  Work-groups are numbered
+// XCD by XCD: the hardware deals consecutive work-group ids round-robin to the 8 XCDs, so
+// work-group id p works on logical block (p % 8) * (G / 8) + p / 8 (when 8 divides the
+// number of work-groups G).  Each XCD then sweeps a contiguous band of sites, and the
+// x-neighbour rows a block reads were fetched by the blocks just before it into the
+// same XCD's L2.
 // ======================================================================================
-template <typename R, bool DAG, bool RES>
+__device__ __forceinline__ int xcd_block(int p, int G) { return (G & 7) ? p : (p & 7) * (G >> 3) + (p >> 3); }
+
+template <typename R, bool DAG, bool RES, int RB>
 __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
-                                                int Nt, R m0, uint64_t mask, size_t ld) {
-    const int r = blockIdx.y;
-    if (!rhs_on(mask, r)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+                                                int Nt, R m0, uint64_t mask, size_t ld, int nsb, int nrhs) {
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int sb = L % nsb, r0 = (L / nsb) * RB;
+    const int n = sb * blockDim.x + threadIdx.x;
     if (n >= Nx * Nt) return;
     const int x = n / Nt, t = n - x * Nt;
-    const cplx<R>* v = in + r * ld;
     const int ntp = (t == Nt - 1) ? n - (Nt - 1) : n + 1;
     const int ntm = (t == 0) ? n + (Nt - 1) : n - 1;
     const int nxp = (x == Nx - 1) ? t : n + Nt;
@@ -117,47 +124,61 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U, c
     const R sm = (t == 0) ? R(-1) : R(1);
     const cplx<R> U0 = U[2 * n], U1 = U[2 * n + 1];
     const cplx<R> U0m = conj(U[2 * ntm]), U1m = conj(U[2 * nxm + 1]);
-    const cplx<R> p0 = v[2 * n], p1 = v[2 * n + 1];
-    const cplx<R> tp0 = v[2 * ntp], tp1 = v[2 * ntp + 1];
-    const cplx<R> xp0 = v[2 * nxp], xp1 = v[2 * nxp + 1];
-    const cplx<R> tm0 = v[2 * ntm], tm1 = v[2 * ntm + 1];
-    const cplx<R> xm0 = v[2 * nxm], xm1 = v[2 * nxm + 1];
-    cplx<R> a0, a1;
-    if (!DAG) {
-        a0 = U0 * (sp * (tp0 - tp1)) + U1 * (xp0 + times_i(xp1)) + U0m * (sm * (tm0 + tm1)) +
-             U1m * (xm0 - times_i(xm1));
-        a1 = U0 * (sp * (tp1 - tp0)) + U1 * (xp1 - times_i(xp0)) + U0m * (sm * (tm0 + tm1)) +
-             U1m * (times_i(xm0) + xm1);
-    } else {
-        a0 = U0m * (sm * (tm0 - tm1)) + U1m * (xm0 + times_i(xm1)) + U0 * (sp * (tp0 + tp1)) +
-             U1 * (xp0 - times_i(xp1));
-        a1 = U0m * (sm * (tm1 - tm0)) + U1m * (xm1 - times_i(xm0)) + U0 * (sp * (tp0 + tp1)) +
-             U1 * (times_i(xp0) + xp1);
-    }
     const R mass = m0 + R(2);
-    cplx<R> o0 = mass * p0 - R(0.5) * a0;
-    cplx<R> o1 = mass * p1 - R(0.5) * a1;
-    cplx<R>* o = out + r * ld;
-    if (RES) {
-        const cplx<R>* bb = b + r * ld;
-        o0 = bb[2 * n] - o0;
-        o1 = bb[2 * n + 1] - o1;
+#pragma unroll
+    for (int q = 0; q < RB; q++) {
+        const int r = r0 + q;
+        if (r >= nrhs || !rhs_on(mask, r)) continue;
+        const cplx<R>* v = in + r * ld;
+        const cplx<R> p0 = v[2 * n], p1 = v[2 * n + 1];
+        const cplx<R> tp0 = v[2 * ntp], tp1 = v[2 * ntp + 1];
+        const cplx<R> xp0 = v[2 * nxp], xp1 = v[2 * nxp + 1];
+        const cplx<R> tm0 = v[2 * ntm], tm1 = v[2 * ntm + 1];
+        const cplx<R> xm0 = v[2 * nxm], xm1 = v[2 * nxm + 1];
+        cplx<R> a0, a1;
+        if (!DAG) {
+            a0 = U0 * (sp * (tp0 - tp1)) + U1 * (xp0 + times_i(xp1)) + U0m * (sm * (tm0 + tm1)) +
+                 U1m * (xm0 - times_i(xm1));
+            a1 = U0 * (sp * (tp1 - tp0)) + U1 * (xp1 - times_i(xp0)) + U0m * (sm * (tm0 + tm1)) +
+                 U1m * (times_i(xm0) + xm1);
+        } else {
+            a0 = U0m * (sm * (tm0 - tm1)) + U1m * (xm0 + times_i(xm1)) + U0 * (sp * (tp0 + tp1)) +
+                 U1 * (xp0 - times_i(xp1));
+            a1 = U0m * (sm * (tm1 - tm0)) + U1m * (xm1 - times_i(xm0)) + U0 * (sp * (tp0 + tp1)) +
+                 U1 * (times_i(xp0) + xp1);
+        }
+        cplx<R> o0 = mass * p0 - R(0.5) * a0;
+        cplx<R> o1 = mass * p1 - R(0.5) * a1;
+        cplx<R>* o = out + r * ld;
+        if (RES) {
+            const cplx<R>* bb = b + r * ld;
+            o0 = bb[2 * n] - o0;
+            o1 = bb[2 * n + 1] - o1;
+        }
+        o[2 * n] = o0;
+        o[2 * n + 1] = o1;
     }
-    o[2 * n] = o0;
-    o[2 * n + 1] = o1;
+}
+
+template <typename R, bool DAG, bool RES>
+static void launch_wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx,
+                          int Nt, double m0, int nrhs, uint64_t mask, size_t ld) {
+    constexpr int RB = 4;
+    const int nsb = (int)cdivu((size_t)Nx * Nt, 256), ng = (nrhs + RB - 1) / RB;
+    k_wilson<R, DAG, RES, RB><<<dim3((unsigned)nsb * ng), 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld, nsb,
+                                                                      nrhs);
 }
 
 template <typename R>
 void wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, int Nx, int Nt,
             double m0, bool dagger, int nrhs, uint64_t mask, size_t ld) {
     timer_begin(s, K_WILSON);
-    dim3 grid(cdivu((size_t)Nx * Nt, 256), nrhs);
     if (dagger) {
-        if (b) k_wilson<R, true, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
-        else k_wilson<R, true, false><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+        if (b) launch_wilson<R, true, true>(s, U, in, out, b, Nx, Nt, m0, nrhs, mask, ld);
+        else launch_wilson<R, true, false>(s, U, in, out, b, Nx, Nt, m0, nrhs, mask, ld);
     } else {
-        if (b) k_wilson<R, false, true><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
-        else k_wilson<R, false, false><<<grid, 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld);
+        if (b) launch_wilson<R, false, true>(s, U, in, out, b, Nx, Nt, m0, nrhs, mask, ld);
+        else launch_wilson<R, false, false>(s, U, in, out, b, Nx, Nt, m0, nrhs, mask, ld);
     }
     timer_end(s, K_WILSON, (double)Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + nact(mask, nrhs) * (b ? 3 : 2)));
     DD_LAUNCH_CHECK();
@@ -205,7 +226,8 @@ template <typename R, int DOF, bool RES, int NT>
 __global__ __launch_bounds__(256) void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
                                                 int Nt, R m0, uint64_t mask, size_t ld, int nrhs) {
-    const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // XCD by XCD, as k_wilson: the neighbour sites' vectors come from the same XCD's L2
+    const int n = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (n >= Nx * Nt) return;   // uniform per wave
     coarse_site<R, DOF, RES, NT>(L, in, out, b, Nx, Nt, m0, mask, ld, nrhs, n, blockIdx.y * NT, threadIdx.x & 63);
 }

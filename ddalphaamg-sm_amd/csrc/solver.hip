@@ -684,24 +684,36 @@ void Multigrid<R>::autotune() {
         cplx<R>* x = sapr_[l].as<cplx<R>>();
         k_fill_hash<R><<<(unsigned)((n * nr + 255) / 256), 256, 0, s_>>>(r, n * nr, 12345 + l);
         DD_HIP(hipGetLastError());
+        // Each variant is timed once on the blocks of the first quarter of the level's x
+        // extent (a prefix of the arrays: every access stays inside them), after one launch
+        // on a single pair of blocks that loads its code.  The quarter keeps a quarter of the
+        // work-groups of a full colour, so relative speeds are those of the full sweep.
+        int nxq = G.Nx;
+        for (int q = 4; q >= 2; q /= 2)
+            if ((G.Nx / q) % G.bx == 0 && ((G.Nx / q / G.bx) * (G.Nt / G.bt)) % 2 == 0 && G.Nx / q >= 2 * G.bx) {
+                nxq = G.Nx / q;
+                break;
+            }
+        const int nbt = G.Nt / G.bt;
+        const bool pair_ok = (nbt % 2 == 0);   // one block row of the first x block: two colours
         double best = 1e300;
         int bestv = -1;
         for (int v : sapc_variants<R>()) {
             if (!sapc_supported<R>(v, G.dof, G.bx * G.bt)) continue;
             if (v != 32 && v != 33 && v / 2 > 1 && nr == 1) continue;   // multi-system vector variants only for batches
             if (v == 33 && !hlinks(l)) continue;                          // no binary16 link copy
-            double t = 0;
-            for (int rep = 0; rep < 2; rep++) {
-                zero<R>(s_, x, n * nr, 0, 1, 1);
-                DD_HIP(hipEventRecord(e0, s_));
-                sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, 0, p_.m0, p_.sap_m,
+            zero<R>(s_, x, n * nr, 0, 1, 1);
+            if (pair_ok)   // code loading: a lattice of one x block row and two t blocks
+                sap_blocksc<R>(s_, links(l), r, x, G.bx, 2 * G.bt, G.dof, G.bx, G.bt, 0, p_.m0, p_.sap_m,
                                p_.sap_restarts, p_.sap_tol, nr, full_mask(nr), n, nullptr, v, hlinks(l));
-                DD_HIP(hipEventRecord(e1, s_));
-                DD_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                DD_HIP(hipEventElapsedTime(&ms, e0, e1));
-                t = ms;   // keep the second (warm) run
-            }
+            DD_HIP(hipEventRecord(e0, s_));
+            sap_blocksc<R>(s_, links(l), r, x, nxq, G.Nt, G.dof, G.bx, G.bt, 0, p_.m0, p_.sap_m, p_.sap_restarts,
+                           p_.sap_tol, nr, full_mask(nr), n, nullptr, v, hlinks(l));
+            DD_HIP(hipEventRecord(e1, s_));
+            DD_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            DD_HIP(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms * (double)G.Nx / nxq;   // scaled to the full half-sweep
             if (t < best) {
                 best = t;
                 bestv = v;
@@ -716,7 +728,7 @@ void Multigrid<R>::autotune() {
         sapc_var_[l] = bestv;
         sapc_ms_[l] = best;
     }
-    // coarsest engine (ortho_coarse = 3): the same 160 Arnoldi steps (one cycle, no
+    // coarsest engine (ortho_coarse = 3): the same 96 Arnoldi steps (one cycle, no
     // convergence test) on max_rhs pseudo-random systems with each engine; keep the faster
     if (coarse_dev_ && coarse_) {
         const LevelGeom& G = g_[L - 1];
@@ -726,7 +738,7 @@ void Multigrid<R>::autotune() {
         k_fill_hash<R><<<(unsigned)((n * nr + 255) / 256), 256, 0, s_>>>(e, n * nr, 777);
         DD_HIP(hipGetLastError());
         double t[2] = {0, 0};
-        const int msteps = std::min(160, p_.coarse_m);
+        const int msteps = std::min(96, p_.coarse_m);
         for (int eng = 0; eng < 2; eng++) {
             use_dev_ = eng == 1;
             // a short untimed run first (loads the kernels' code), then the timed one

@@ -68,3 +68,23 @@ def test_methods_batched_equal_single(golden, method):
             x1, it1, _ = ctx.solve_method(method, B[r], meta['m0'], **kw)
             assert int(it1[0]) == int(itb[r])
             assert rel(xb[r], x1) < 1e-12
+
+
+@pytest.mark.parametrize('nu,tol', [(3, 1e-12), (40, 1e-6)])
+def test_sap_standalone_matches_oracle(golden, nu, tol):
+    """SAP_fine_level::SAP as a solver (dd_sap_fine, src/sap.cpp:81-165) equals the oracle's
+    SAP on the same field: the reference's return value (the iteration at which
+    ||v - D x|| < tol ||v|| first held, or nu) and the result; batched equals single."""
+    meta, a = golden('g32_methods')
+    U = np.exp(1j * a['theta'])
+    m0 = meta['m0']
+    rng = np.random.default_rng(8)
+    x0 = 0.1 * (rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape))
+    ref, its_ref = orc.sap(orc.dirac_stencil(U, m0), orc.SapBlocks(32, 32, 4, 4), a['rhs'], x0, nu, tol)
+    with _ctx(meta, a) as ctx:
+        x, its, rr = ctx.sap_fine(a['rhs'], x0, nu, tol, (4, 4), m0)
+        xb, itsb, _ = ctx.sap_fine(np.stack([a['rhs'], a['rhs']]), np.stack([x0, x0]), nu, tol, (4, 4), m0)
+    print(f'SAP nu={nu} tol={tol}: {its[0]} (oracle {its_ref}), relres {rr[0]:.3e}, vs oracle {rel(x, ref):.2e}')
+    assert int(its[0]) == its_ref
+    assert rel(x, ref) < 1e-12
+    assert int(itsb[1]) == its_ref and rel(xb[1], x) < 1e-14
