@@ -198,6 +198,7 @@ int dd_default_params_sized(dd_params* p, int size, int levels) {
     p->coarse_sap_links = 0;
     p->bootstrap = 0;
     p->graph = 0;
+    p->sap_fused = 1;
     return DD_OK;
 }
 

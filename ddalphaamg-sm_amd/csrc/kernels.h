@@ -67,6 +67,31 @@ template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,
                  int bt, int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask,
                  size_t ld, const int* gate, int variant = -1, const _Float16* Lh = nullptr);
+// State of one fused coarse-SAP iteration (sap_fused_iteration): per (block, tile) the
+// launch epoch at which the block's red solve stored its iterate, two counters that are
+// zero when a launch starts (black solves done, norm slices done), a sticky failure word
+// (a bounded wait timed out), the norm partial sums, the per-system gate of the
+// iterations and ||v||^2 per system (stride 2).
+struct SapFuse {
+    unsigned* done = nullptr;           // [nbx*nbt][ntile]
+    unsigned* ctr = nullptr;            // [2]
+    unsigned* fail = nullptr;
+    float* part = nullptr;              // [slices][64]
+    int* gate = nullptr;                // [64]
+    const cplx<double>* vnorm = nullptr;
+    unsigned epoch = 0;
+    double tol = 0;
+};
+// One SAP iteration on the matrix cores in one launch (FP32, coarse-SAP variants 32/33):
+// red blocks, black blocks, ||v - D x|| and the gate update, with each block's
+// right-hand side v - D x formed from the iterate (src/sap.cpp:103-156).  Returns false,
+// launching nothing, where it does not apply (FP64, another variant, odd numbers of
+// blocks in a direction, blocks above 16 sites or dof above 24).  nslice: work-groups of
+// the norm.
+template <typename R>
+bool sap_fused_iteration(hipStream_t s, const cplx<R>* L, const _Float16* Lh, const cplx<R>* v, cplx<R>* x, int Nx,
+                         int Nt, int dof, int bx, int bt, double m0, int m, int restarts, double tol, int nrhs,
+                         uint64_t mask, size_t ld, int variant, const SapFuse& fz, int nslice);
 template <typename R>
 std::vector<int> sapc_variants();
 template <typename R>

@@ -9,4 +9,7 @@ template void sap_blocksc<double>(hipStream_t, const cplx<double>*, const cplx<d
                                  const _Float16*);
 template std::vector<int> sapc_variants<double>();
 template bool sapc_supported<double>(int, int, int);
+template bool sap_fused_iteration<double>(hipStream_t, const cplx<double>*, const _Float16*, const cplx<double>*, cplx<double>*,
+                                      int, int, int, int, int, double, int, int, double, int, uint64_t, size_t, int,
+                                      const SapFuse&, int);
 }  // namespace dd

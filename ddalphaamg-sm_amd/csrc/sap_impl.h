@@ -562,12 +562,30 @@ void dispatch_sapc(int variant, hipStream_t s, const _Float16* Lh, const cplx<R>
 #undef DD_V
 }
 
+template <typename R, int D>
+bool dispatch_sap_fused(hipStream_t s, const cplx<R>* L, const _Float16* Lh, const cplx<R>* v, cplx<R>* x, int Nx,
+                        int Nt, int bx, int bt, double m0, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
+                        int variant, const SapFuse& fz, int nslice) {
+    if constexpr (sizeof(R) == 4 && D <= 24) {
+        if (variant != 32 && variant != 33) return false;
+        if (variant == 33 && !Lh) return false;
+        launch_sapm_fused<D>(s, L, variant == 33 ? Lh : nullptr, v, x, Nx, Nt, bx, bt, m0, restarts, tol, nrhs, mask,
+                             ld, fz, nslice);
+        return true;
+    } else {
+        return false;
+    }
+}
+
 // Each (precision, dof) pair of the coarse SAP dispatch is compiled in its own
 // translation unit (sap_<prec>_d<dof>.hip) to keep builds parallel.
 #define DD_SAPC_DECL(R, D)                                                                                         \
     extern template void dispatch_sapc<R, D>(int, hipStream_t, const _Float16*, const cplx<R>*, const cplx<R>*,     \
                                              cplx<R>*, int, int,                                                 \
-                                             int, int, int, double, int, double, int, uint64_t, size_t, const int*);
+                                             int, int, int, double, int, double, int, uint64_t, size_t, const int*); \
+    extern template bool dispatch_sap_fused<R, D>(hipStream_t, const cplx<R>*, const _Float16*, const cplx<R>*,     \
+                                                  cplx<R>*, int, int, int, int, double, int, double, int, uint64_t,  \
+                                                  size_t, int, const SapFuse&, int);
 #ifndef DD_SAPC_UNIT
 DD_SAPC_DECL(float, 8)
 DD_SAPC_DECL(float, 16)
@@ -578,6 +596,39 @@ DD_SAPC_DECL(double, 16)
 DD_SAPC_DECL(double, 24)
 DD_SAPC_DECL(double, 32)
 #endif
+
+template <typename R>
+bool sap_fused_iteration(hipStream_t s, const cplx<R>* L, const _Float16* Lh, const cplx<R>* v, cplx<R>* x, int Nx,
+                         int Nt, int dof, int bx, int bt, double m0, int m, int restarts, double tol, int nrhs,
+                         uint64_t mask, size_t ld, int variant, const SapFuse& fz, int nslice) {
+    const int S = bx * bt;
+    if (m != kSapM || nrhs > 64 || bx < 1 || bt < 1 || Nx % bx || Nt % bt) return false;
+    const int nbx = Nx / bx, nbt = Nt / bt;
+    if ((nbx & 1) || (nbt & 1) || nbx < 2 || nbt < 2) return false;   // a true checkerboard with periodic wrap
+    if (variant < 0) variant = sapc_default_variant<R>(nrhs, dof, S, Lh != nullptr);
+    if ((variant != 32 && variant != 33) || !sapc_supported<R>(variant, dof, S)) return false;
+    bool done = false;
+    timer_begin(s, K_SAPC);
+    switch (dof) {
+#define DD_CASE(D)                                                                                              \
+    case D:                                                                                                     \
+        done = dispatch_sap_fused<R, D>(s, L, Lh, v, x, Nx, Nt, bx, bt, m0, restarts, tol, nrhs, mask, ld, variant, \
+                                        fz, nslice);                                                            \
+        break;
+        DD_DOF_LIST(DD_CASE)
+#undef DD_CASE
+        default:
+            break;
+    }
+    // both colours' block solves (links once per colour) and the norm pass (links and
+    // the iterate once)
+    const double lb = variant == 33 ? 2.0 * sizeof(_Float16) : (double)sizeof(cplx<R>);
+    timer_end(s, K_SAPC, done ? Nx * Nt * dof * (5.0 * dof * (lb + sizeof(cplx<R>)) +
+                                               3.0 * nact(mask, nrhs) * sizeof(cplx<R>))
+                              : 0.0);
+    DD_LAUNCH_CHECK();
+    return done;
+}
 
 template <typename R>
 void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int dof, int bx,

@@ -42,11 +42,35 @@ constexpr size_t sapm_lds_bytes(int S) {
            sizeof(float) * 16 + sizeof(int) * 20;
 }
 
-template <int DOF, int SPW, typename LT>
-__global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cplx<float>* __restrict__ rr,
-                                              cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
-                                              int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
-                                              const int* __restrict__ gate, int nrhs, int ntile) {
+constexpr unsigned kFuseSpins = 1u << 24;
+
+// waits (thread 0 of the work-group) until *w == want; false after kFuseSpins polls or
+// when the sticky failure word is set
+__device__ __forceinline__ bool fuse_wait_eq(const unsigned* w, unsigned want, unsigned* fail) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+        if ((++spins & 63u) == 0 &&
+            (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || spins > kFuseSpins)) {
+            __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// One block solve of a colour for one 16-system tile (task `gid` of the colour).
+// FUSED = false: the block right-hand side is read from rr (the residual v - D x).
+// FUSED = true (one launch per SAP iteration, k_sapm_fused): rr is v, and the block
+// right-hand side v - D x is formed here from the iterate xx and the FP32 links L32 at
+// the block's sites; a black block first waits until its four (red) neighbour blocks of
+// the same tile have stored their iterates, and every block publishes its own when done.
+template <int DOF, int SPW, typename LT, bool FUSED>
+__device__ __forceinline__ void sapm_task(const LT* __restrict__ L, const float* __restrict__ L32,
+                                          const cplx<float>* __restrict__ rr, cplx<float>* __restrict__ xx, int Nx,
+                                          int Nt, int bx, int bt, int colour, float m0, int restarts, float tol,
+                                          uint64_t mask, size_t ld, const int* __restrict__ gate, int nrhs, int ntile,
+                                          int gid, const SapFuse& fz) {
     using R = float;
     using acc_t = Mfma<R>::acc_t;
     using Vec = SapmVec<DOF, SPW>;
@@ -69,7 +93,7 @@ __global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cp
     const bool lead = (wv == 0) && (g == 0);   // lane j of wave 0: scalars of system j
     const int nbx = Nx / bx, nbt = Nt / bt;
     // work-group id -> (block, system tile): 8 consecutive blocks per group of 8 ntile ids
-    const int gid = blockIdx.x, grp = gid / (8 * ntile), rr8 = gid % (8 * ntile);
+    const int grp = gid / (8 * ntile), rr8 = gid % (8 * ntile);
     const int blk = grp * 8 + (rr8 & 7), tile = rr8 >> 3;
     if (blk >= (nbx * nbt) / 2) return;
     int BX, BT;
@@ -80,6 +104,32 @@ __global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cp
     for (int q = 0; q < 16; q++) {
         const int rh = r0sys + q;
         if (rh < nrhs && rhs_on(mask, rh) && (!gate || gate[rh] != 0)) onm |= 1u << q;
+    }
+    const int gblk = BX * nbt + BT;   // block index over both colours
+    if (FUSED) {
+        // the iterates of the neighbour blocks this block's residual reads: red blocks read
+        // black ones, which no task of this launch changes before this block is done (a
+        // black block waits for its red neighbours); black blocks wait for the red ones
+        if (colour == 1) {
+            if (threadIdx.x == 0) {
+                const int nb[4] = {((BX + 1) % nbx) * nbt + BT, ((BX + nbx - 1) % nbx) * nbt + BT,
+                                   BX * nbt + (BT + 1) % nbt, BX * nbt + (BT + nbt - 1) % nbt};
+                for (int k = 0; k < 4; k++)
+                    if (!fuse_wait_eq(fz.done + (size_t)nb[k] * ntile + tile, fz.epoch, fz.fail)) break;
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();
+        }
+        if (!onm) {   // nothing to solve: publish at once
+            if (threadIdx.x == 0) {
+                if (colour == 0)
+                    __hip_atomic_store(fz.done + (size_t)gblk * ntile + tile, fz.epoch, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    __hip_atomic_fetch_add(fz.ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            return;
+        }
     }
     if (!onm) return;
     const int rh = r0sys + j;
@@ -201,7 +251,47 @@ __global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cp
 
     // right-hand sides (initial guess = right-hand side)
     Vec w, V[M];
-    {
+    if (FUSED) {   // v - D x at the owned sites (the global stencil: neighbours anywhere, t sign)
+        const cplx<R>* rv = rr + (size_t)(on ? rh : 0) * ld;
+        const R* xs = reinterpret_cast<const R*>(xx + (size_t)(on ? rh : 0) * ld);
+#pragma unroll
+        for (int u = 0; u < SPW; u++) {
+            const int n = gn[u], x0 = n / Nt, t0 = n - x0 * Nt;
+            acc_t acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
+            const float* Ln = L32 + (size_t)n * 5 * DOF * DOF * 2;
+#pragma unroll 1
+            for (int hop = 0; hop < 5; hop++) {
+                const int nh = hop == 0 ? n
+                             : hop == 1 ? ((t0 == Nt - 1) ? n - (Nt - 1) : n + 1)
+                             : hop == 2 ? ((x0 == Nx - 1) ? t0 : n + Nt)
+                             : hop == 3 ? ((t0 == 0) ? n + (Nt - 1) : n - 1)
+                                        : ((x0 == 0) ? (Nx - 1) * Nt + t0 : n - Nt);
+                const R sh = (hop == 1 && t0 == Nt - 1) || (hop == 3 && t0 == 0) ? R(-1) : R(1);
+                R bv[KS];
+#pragma unroll
+                for (int s2 = 0; s2 < KS; s2++) bv[s2] = sh * xs[(size_t)nh * DOF * 2 + 4 * s2 + g];
+                mfma_hop<R, DOF, (DOF / 2) % 4 == 0 ? 4 : DOF / 2, float>(acc, Ln + (size_t)hop * DOF * DOF * 2, ll,
+                                                                          bv);
+            }
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const size_t o = (size_t)n * DOF + 8 * t + 2 * g + h;
+                    const cplx<R> xv = xx[(size_t)(on ? rh : 0) * ld + o], vv = rv[o];
+                    const cplx<R> ph = (own[u] && on) ? mk<R>(vv.x - (mass * xv.x - acc[t][2 * h]),
+                                                              vv.y - (mass * xv.y - acc[t][2 * h + 1]))
+                                                      : czero<R>();
+                    if (own[u]) {
+                        sphi[lidx(u, t, h)] = ph;
+                        sx[lidx(u, t, h)] = ph;
+                    }
+                    w.e[(u * MT + t) * 2 + h] = ph;
+                }
+        }
+    } else {
         const cplx<R>* rv = rr + (size_t)(on ? rh : 0) * ld;
 #pragma unroll
         for (int u = 0; u < SPW; u++)
@@ -336,6 +426,133 @@ __global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cp
                         xv[o] = xv[o] + sx[lidx(u, t, h)];
                     }
     }
+    if (FUSED) {   // publish this block's iterate: every wave's stores, then one release
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (colour == 0)
+                __hip_atomic_store(fz.done + (size_t)gblk * ntile + tile, fz.epoch, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            else
+                __hip_atomic_fetch_add(fz.ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int DOF, int SPW, typename LT>
+__global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cplx<float>* __restrict__ rr,
+                                              cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
+                                              int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
+                                              const int* __restrict__ gate, int nrhs, int ntile) {
+    const SapFuse fz{};
+    sapm_task<DOF, SPW, LT, false>(L, nullptr, rr, xx, Nx, Nt, bx, bt, colour, m0, restarts, tol, mask, ld, gate,
+                                   nrhs, ntile, blockIdx.x, fz);
+}
+
+// One SAP iteration (src/sap.cpp:103-156) in one launch: work-groups [0, nt) solve the red
+// blocks, [nt, 2nt) the black blocks, [2nt, 2nt + ns) form ||v - D x||^2 per system over
+// slices of the sites, and the last slice to finish updates the gate (iterations stop for
+// the systems with ||v - D x|| < tol ||v||).  Work-groups start in the order of their
+// ids, so a work-group only ever waits for lower ids that are already running: black
+// blocks for their red neighbours, norm slices for every black block.  Needs even
+// numbers of blocks in both directions (no block borders one of its own colour).
+template <int DOF, int SPW, typename LT>
+__global__ __launch_bounds__(512) void k_sapm_fused(const LT* __restrict__ L, const float* __restrict__ L32,
+                                                    const cplx<float>* __restrict__ v, cplx<float>* __restrict__ xx,
+                                                    int Nx, int Nt, int bx, int bt, float m0, int restarts, float tol,
+                                                    uint64_t mask, size_t ld, int nrhs, int ntile, int ntask,
+                                                    int nslice, SapFuse fz) {
+    using R = float;
+    using acc_t = Mfma<R>::acc_t;
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+    const int gid = blockIdx.x;
+    if (gid < 2 * ntask) {
+        const int colour = gid < ntask ? 0 : 1;
+        sapm_task<DOF, SPW, LT, true>(L, L32, v, xx, Nx, Nt, bx, bt, colour, m0, restarts, tol, mask, ld, fz.gate,
+                                      nrhs, ntile, gid - colour * ntask, fz);
+        return;
+    }
+    // norm slice: ||v - D x||^2 per system over sites [n0, n1), once every black block is done
+    const int sl = gid - 2 * ntask, V = Nx * Nt;
+    const int n0 = (int)((long long)V * sl / nslice), n1 = (int)((long long)V * (sl + 1) / nslice);
+    __shared__ float red_s[16][64];   // one row per wave (at most 16 waves)
+    __shared__ int ok_s;
+    if (threadIdx.x == 0) {
+        const unsigned nblack = (unsigned)((Nx / bx) * (Nt / bt) / 2 * ntile);   // black tasks that exist
+        ok_s = fuse_wait_eq(fz.ctr, nblack, fz.fail) ? 1 : 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, W = blockDim.x >> 6;
+    const LinkLane ll = LinkLane::make<R, DOF>(lane);
+    const R mass = m0 + R(2);
+    float sumsq[4] = {0, 0, 0, 0};   // per tile k < ntile (at most 4 tiles: 64 systems)
+    for (int n = n0 + wv; n < n1 && ok_s; n += W) {
+        const int x0 = n / Nt, t0 = n - x0 * Nt;
+        const float* Ln = L32 + (size_t)n * 5 * DOF * DOF * 2;
+        for (int k = 0; k < ntile; k++) {
+            const int j = k * 16 + (lane & 15);
+            const bool on = j < nrhs && rhs_on(mask, j);
+            const R* xs = reinterpret_cast<const R*>(xx + (size_t)(on ? j : 0) * ld);
+            acc_t acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) acc[t] = acc_t{0, 0, 0, 0};
+#pragma unroll 1
+            for (int hop = 0; hop < 5; hop++) {
+                const int nh = hop == 0 ? n
+                             : hop == 1 ? ((t0 == Nt - 1) ? n - (Nt - 1) : n + 1)
+                             : hop == 2 ? ((x0 == Nx - 1) ? t0 : n + Nt)
+                             : hop == 3 ? ((t0 == 0) ? n + (Nt - 1) : n - 1)
+                                        : ((x0 == 0) ? (Nx - 1) * Nt + t0 : n - Nt);
+                const R sh = (hop == 1 && t0 == Nt - 1) || (hop == 3 && t0 == 0) ? R(-1) : R(1);
+                R bv[KS];
+#pragma unroll
+                for (int s2 = 0; s2 < KS; s2++) bv[s2] = sh * xs[(size_t)nh * DOF * 2 + 4 * s2 + g];
+                mfma_hop<R, DOF, (DOF / 2) % 4 == 0 ? 4 : DOF / 2, float>(acc, Ln + (size_t)hop * DOF * DOF * 2,
+                                                                          ll, bv);
+            }
+            float p = 0;
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const size_t o = (size_t)(on ? j : 0) * ld + (size_t)n * DOF + 8 * t + 2 * g + h;
+                    const cplx<R> xv = xx[o], vv = v[o];
+                    const R re = vv.x - (mass * xv.x - acc[t][2 * h]), im = vv.y - (mass * xv.y - acc[t][2 * h + 1]);
+                    p += re * re + im * im;
+                }
+            sumsq[k] += on ? p : 0.0f;
+        }
+    }
+    // per system: the four lane groups, then the eight waves in order, then the slices
+    for (int k = 0; k < 4; k++) {
+        float p = sumsq[k];
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        if (g == 0) red_s[wv][k * 16 + (lane & 15)] = p;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float t = 0;
+        for (int w = 0; w < W; w++) t += red_s[w][threadIdx.x];
+        fz.part[(size_t)sl * 64 + threadIdx.x] = t;
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    __shared__ unsigned last_s;
+    if (threadIdx.x == 0)
+        last_s = __hip_atomic_fetch_add(fz.ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                 (unsigned)nslice - 1;
+    __syncthreads();
+    if (last_s && threadIdx.x < 64) {   // the last slice: totals in slice order, gate update
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int j = threadIdx.x;
+        if (j < nrhs && rhs_on(mask, j)) {
+            double t = 0;
+            for (int q = 0; q < nslice; q++) t += (double)fz.part[(size_t)q * 64 + j];
+            if (sqrt(t) < fz.tol * sqrt(fz.vnorm[2 * j].x)) fz.gate[j] = 0;
+        }
+    }
 }
 
 template <int DOF, int SPW, typename LT>
@@ -350,6 +567,40 @@ static void launch_sapm_k(hipStream_t s, dim3 grid, int threads, size_t sh, cons
     }
     k_sapm<DOF, SPW, LT><<<grid, threads, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
                                                     mask, ld, gate, nrhs, ntile);
+}
+
+// One fused SAP iteration (k_sapm_fused) for the systems in mask; L the FP32 links, Lh
+// (nullable) the binary16 copy the block solves then read.
+template <int DOF>
+static void launch_sapm_fused(hipStream_t s, const cplx<float>* L, const _Float16* Lh, const cplx<float>* v,
+                              cplx<float>* x, int Nx, int Nt, int bx, int bt, double m0, int restarts, double tol,
+                              int nrhs, uint64_t mask, size_t ld, const SapFuse& fz, int nslice) {
+    const int S = bx * bt;
+    const size_t sh = sapm_lds_bytes<DOF>(S);
+    const int nblk = (Nx / bx) * (Nt / bt) / 2, ntile = (nrhs + 15) / 16;
+    const int ntask = ((nblk + 7) / 8) * 8 * ntile;
+    const dim3 grid(2 * ntask + nslice);
+    const float* Lf = reinterpret_cast<const float*>(L);
+    auto go = [&](auto kern, int threads, const auto* Ls) {
+        static bool attr = false;
+        if (!attr) {   // the dynamic LDS may take what the kernel's static LDS leaves of 160 KB
+            hipFuncAttributes fa;
+            DD_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+            DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(160 * 1024 - fa.sharedSizeBytes)));
+            attr = true;
+        }
+        if (sh + 4200 > 160 * 1024) throw BadShape("fused coarse SAP: block does not fit the LDS");
+        kern<<<grid, threads, sh, s>>>(Ls, Lf, v, x, Nx, Nt, bx, bt, (float)m0, restarts, (float)tol, mask, ld, nrhs,
+                                       ntile, ntask, nslice, fz);
+    };
+    if (S > 8) {
+        if (Lh) go(&k_sapm_fused<DOF, 2, _Float16>, 64 * ((S + 1) / 2), Lh);
+        else go(&k_sapm_fused<DOF, 2, float>, 64 * ((S + 1) / 2), Lf);
+    } else {
+        if (Lh) go(&k_sapm_fused<DOF, 1, _Float16>, 64 * S, Lh);
+        else go(&k_sapm_fused<DOF, 1, float>, 64 * S, Lf);
+    }
 }
 
 // L: FP32 links [V][5][DOF][DOF] complex, or (Lh non-null) their binary16 copy, which is

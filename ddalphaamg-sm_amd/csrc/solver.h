@@ -119,6 +119,8 @@ class Multigrid {
     bool vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     // reports (throws) a timed-out persistent coarsest solve among the graph replays
     void check_graph_failures();
+    // reports (throws) a timed-out wait inside a fused SAP iteration since the last check
+    void check_sap_failures();
     int graph_count() const { return (int)graphs_.size(); }
     ~Multigrid();
     void apply(int l, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mask);
@@ -172,6 +174,13 @@ class Multigrid {
     std::vector<int> sapc_var_;
     std::vector<double> sapc_ms_;
     bool capturing_ = false, graphs_ok_ = true;
+    // fused SAP iterations (sap_fused_iteration) per level: block epochs, counters, norm
+    // partials, launch epoch, and whether the level's shape allows them
+    std::vector<DevMem> fuse_done_, fuse_ctr_, fuse_part_;
+    DevMem fuse_fail_;
+    std::vector<unsigned> fuse_epoch_;
+    std::vector<char> fuse_ok_;
+    int fuse_slices_ = 256;
     std::map<std::tuple<const void*, const void*, uint64_t>, hipGraphExec_t> graphs_;
 };
 

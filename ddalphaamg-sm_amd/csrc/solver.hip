@@ -330,6 +330,31 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
         }
         if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
     }
+    // fused SAP iterations: state for every coarse level that the matrix cores smooth
+    fuse_done_.resize(L);
+    fuse_ctr_.resize(L);
+    fuse_part_.resize(L);
+    fuse_epoch_.assign(L, 0u);
+    fuse_ok_.assign(L, 0);
+    if (sizeof(R) == 4 && p.sap_fused) {
+        int dev = 0, ncu = 0;
+        DD_HIP(hipGetDevice(&dev));
+        DD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        fuse_slices_ = std::max(1, std::min(ncu, 1024));
+        fuse_fail_.alloc(sizeof(unsigned));
+        DD_HIP(hipMemsetAsync(fuse_fail_.as<void>(), 0, sizeof(unsigned), s));
+        for (int l = 1; l < L - 1; l++) {
+            const LevelGeom& G = g_[l];
+            const int nbx = G.Nx / G.bx, nbt = G.Nt / G.bt;
+            if ((nbx & 1) || (nbt & 1) || !sapc_supported<R>(32, G.dof, G.bx * G.bt)) continue;
+            fuse_done_[l].alloc(sizeof(unsigned) * (size_t)nbx * nbt * 4);
+            DD_HIP(hipMemsetAsync(fuse_done_[l].as<void>(), 0, fuse_done_[l].bytes(), s));
+            fuse_ctr_[l].alloc(sizeof(unsigned) * 2);
+            fuse_part_[l].alloc(sizeof(float) * 64 * (size_t)fuse_slices_);
+            fuse_ok_[l] = 1;
+        }
+        DD_HIP(hipStreamSynchronize(s));
+    }
     if (p.ortho_coarse >= 2) {
         const LevelGeom& Gc = g_[L - 1];
         coarse_dev_.reset(new CoarsestGmres<R>(s, Gc.Nx, Gc.Nt, Gc.dof, p.coarse_m, p.coarse_restarts, p.coarse_tol,
@@ -379,6 +404,30 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
     DD_HIP(hipGetLastError());
     dot_partial<R>(s_, v, v, n, n, nr, mask, rb_[l]);
     dot_final<R>(s_, rb_[l], nrm + 0, 2, nr, mask);
+    // one launch per iteration where the level allows it (not inside a graph capture: the
+    // launch epoch is an argument that a replay would repeat)
+    if (l > 0 && fuse_ok_[l] && !capturing_) {
+        SapFuse fz;
+        fz.done = fuse_done_[l].as<unsigned>();
+        fz.ctr = fuse_ctr_[l].as<unsigned>();
+        fz.fail = fuse_fail_.as<unsigned>();
+        fz.part = fuse_part_[l].as<float>();
+        fz.gate = gate;
+        fz.vnorm = nrm;
+        fz.tol = tol;
+        const int var = l < (int)sapc_var_.size() ? sapc_var_[l] : -1;
+        bool ran = true;
+        for (int i = 0; i < nu && ran; i++) {
+            k_fill_int<<<1, 64, 0, s_>>>(reinterpret_cast<int*>(fz.ctr), 0, 2);
+            DD_HIP(hipGetLastError());
+            fz.epoch = ++fuse_epoch_[l];
+            ran = sap_fused_iteration<R>(s_, links(l), hlinks(l), v, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, p_.m0, p_.sap_m,
+                                         p_.sap_restarts, p_.sap_tol, nr, mask, n, var, fz, fuse_slices_);
+            if (!ran && i > 0) throw BadShape("fused SAP became unavailable between iterations");
+        }
+        if (ran) return;
+        fuse_ok_[l] = 0;   // this level's variant has no fused form: separate launches from now on
+    }
     apply(l, x, r, v, mask);
     for (int i = 0; i < nu; i++) {
         for (int colour = 0; colour < 2; colour++) {
@@ -473,6 +522,18 @@ bool Multigrid<R>::vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask)
 template <typename R>
 void Multigrid<R>::check_graph_failures() {
     if (coarse_dev_) coarse_dev_->check_failure();
+}
+
+template <typename R>
+void Multigrid<R>::check_sap_failures() {
+    if (!fuse_fail_.bytes()) return;
+    unsigned f = 0;
+    DD_HIP(hipMemcpyAsync(&f, fuse_fail_.as<void>(), sizeof(unsigned), hipMemcpyDeviceToHost, s_));
+    DD_HIP(hipStreamSynchronize(s_));
+    if (f) {
+        DD_HIP(hipMemsetAsync(fuse_fail_.as<void>(), 0, sizeof(unsigned), s_));
+        throw BadShape("coarse SAP: a wait inside a fused iteration timed out; results of this call are invalid");
+    }
 }
 
 template <typename R>
@@ -943,6 +1004,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
         if (mg64) mg64->check_graph_failures();
         if (mg32) mg32->check_graph_failures();
     }
+    if (mg32) mg32->check_sap_failures();
     DD_HIP(hipStreamSynchronize(s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     int mx = 0;

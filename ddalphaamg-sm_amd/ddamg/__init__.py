@@ -51,7 +51,7 @@ class Params(C.Structure):
                 ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
                 ('max_rhs', C.c_int), ('autotune', C.c_int),
                 ('sap_variant', C.c_int), ('coarse_basis', C.c_int), ('coarse_sap_links', C.c_int),
-                ('bootstrap', C.c_int), ('graph', C.c_int)]
+                ('bootstrap', C.c_int), ('graph', C.c_int), ('sap_fused', C.c_int)]
 
 
 class SolveInfo(C.Structure):

@@ -96,6 +96,12 @@ typedef struct dd_params {
                                       (SAP, transfers, persistent coarsest GMRES) as a
                                       HIP graph per batch mask and replays it; 0 (the
                                       default) runs it as separate launches.        */
+    int sap_fused;                 /* 1 (default): on coarse levels smoothed on the matrix
+                                      cores, each SAP iteration (red blocks, black blocks,
+                                      residual norm, stopping test) is one launch whose
+                                      blocks form their right-hand sides v - D x from the
+                                      iterate; 0: separate launches with full-lattice
+                                      residuals between the colours.                */
 } dd_params;
 
 typedef struct dd_solve_info {

@@ -421,3 +421,46 @@ def test_ragged_batch_equals_separate_solves():
             res = np.linalg.norm(Bi[r] - orc.D_phi(U[i], xi[r], m0)) / np.linalg.norm(Bi[r])
             assert res < 1e-10
         c.close()
+
+
+@pytest.mark.parametrize('links16', [0, 1])
+def test_fused_sap_iteration_equals_separate_launches(links16):
+    """The fused SAP iteration (one launch: red blocks, black blocks, residual norm and
+    stopping test, each block forming v - D x itself) gives the result of the separate
+    launches (block solves with full-lattice residuals between the colours), to FP32
+    rounding, for a batch of 20 systems (two 16-system tiles, the second partial; block
+    GMRES tolerance 1e-12, which FP32 cannot reach, so that rounding cannot move a block
+    solve's break and both forms run the same iterations); with a
+    loose tolerance the same systems stop early (src/sap.cpp:150-156), which the
+    iterates show: a system that stops after one iteration keeps its one-iteration
+    result."""
+    rng = np.random.default_rng(44)
+    N = 64
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.4)
+    res = {}
+    for fused in (1, 0):
+        prm = ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[12, 8], m0=-0.2, precision=1,
+                           max_rhs=20, rng_ranks=(2, 2), sap_variant=33 if links16 else 32,
+                           coarse_sap_links=links16, sap_fused=fused, sap_tol=1e-12)
+        with ddamg.Context(N, N) as ctx:
+            ctx.set_gauge(U)
+            ctx.setup(prm)
+            shape = ctx.level_shape(1)
+            if fused:
+                v = rng.standard_normal((20,) + shape) + 1j * rng.standard_normal((20,) + shape)
+                x0 = 0.3 * (rng.standard_normal((20,) + shape) + 1j * rng.standard_normal((20,) + shape))
+                # a system that starts near its solution: its residual test holds after the
+                # first iteration
+                v[5] = ctx.level_apply(1, x0[5], precision=1) + 1e-3 * (rng.standard_normal(shape) +
+                                                                         1j * rng.standard_normal(shape))
+            res[fused] = (ctx.sap(1, v, x0, 3, tol=1e-12, precision=1), ctx.sap(1, v, x0, 3, tol=0.05, precision=1),
+                          ctx.sap(1, v, x0, 1, tol=1e-12, precision=1))
+    for k in range(3):
+        d = max(rel(res[1][k][r], res[0][k][r]) for r in range(20))
+        print(f'fused vs separate launches (case {k}): {d:.2e}')
+        assert d < 1e-5
+    # the loose tolerance stops some systems after one iteration in both forms
+    one_step = [r for r in range(20) if rel(res[1][1][r], res[1][2][r]) < 1e-6]
+    one_step0 = [r for r in range(20) if rel(res[0][1][r], res[0][2][r]) < 1e-6]
+    print('systems stopped after one iteration:', one_step)
+    assert one_step == one_step0 and 5 in one_step
