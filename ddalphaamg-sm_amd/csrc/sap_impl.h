@@ -378,6 +378,12 @@ __global__ __launch_bounds__(512) void k_sapc(const cplx<R>* __restrict__ L, con
     unsigned livem = onm;
     for (int kk = 0; kk < restarts && livem; kk++) {
         __syncthreads();   // errs of the last residual visible to all
+        if (kk == 0) {     // a zero block right-hand side has the solution zero: no iterations
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                if (nphis[q] == R(0)) livem &= ~(1u << q);
+            if (!livem) break;
+        }
         unsigned runm = livem;
 #pragma unroll
         for (int q = 0; q < NB; q++)

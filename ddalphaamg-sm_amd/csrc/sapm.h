@@ -315,6 +315,8 @@ __device__ __forceinline__ void sapm_task(const LT* __restrict__ L, const float*
     if (lead) maxIts[j] = M;
 
     unsigned livem = onm;
+    // a zero block right-hand side has the solution zero: no iterations (no 1/0 scaling)
+    livem &= ~((unsigned)__ballot(g == 0 && nphi == R(0)) & 0xffffu);
     for (int kk = 0; kk < restarts && livem; kk++) {
         unsigned runm = livem;
         const bool live = (livem >> j) & 1;
