@@ -76,6 +76,8 @@ def main():
                     help='skip the row that repeats the batch with an FP32 coarsest Krylov basis')
     ap.add_argument('--bootstrap', type=int, default=0,
                     help='bootstrap passes after the setup (0: the reference setup only)')
+    ap.add_argument('--setup-only', action='store_true',
+                    help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
     global COARSE_BASIS, SAP_LINKS, BOOTSTRAP
     BOOTSTRAP = args.bootstrap
@@ -110,6 +112,17 @@ def main():
     theta = ctx.generate_gauge(2.0, args.sweeps, args.seed)
     log(f'gauge {N}^2 {args.sweeps} sweeps: {time.perf_counter() - t0:.2f}s, '
         f'plaquette {np.mean(np.cos(theta[..., 0] + np.roll(theta[..., 1], -1, 1) - np.roll(theta[..., 0], -1, 0) - theta[..., 1])):.4f}')
+    if args.setup_only:
+        # setup phase alone (test vectors, smoothing, orthonormalisation, Galerkin products),
+        # twice per precision: the second build runs on loaded code
+        out = {}
+        for prec, name in ((1, 'mixed'), (0, 'fp64')):
+            ts = [ctx.setup(hierarchy_params(ddamg, N, prec, nrhs if prec else 1, m0)) for _ in range(2)]
+            out[name] = ts
+            log(f'setup {name}: ' + ', '.join(f'{t:.3f}s' for t in ts))
+        ctx.close()
+        print(json.dumps({'setup_only': True, 'N': N, 'setup_s': out}))
+        return
     rng = np.random.default_rng(args.seed + 1)
     B = rng.standard_normal((nrhs, N, N, 2)) + 1j * rng.standard_normal((nrhs, N, N, 2))
     b_dev = torch.from_numpy(B.view(np.float64).copy()).to(dev)
