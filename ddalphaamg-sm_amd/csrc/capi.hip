@@ -747,6 +747,7 @@ int dd_sap(dd_ctx* c, int level, const double* v, double* x, int nu, double tol,
             with_level_io<R>(
                 c, v, x, n, n, nrhs,
                 [&](const cplx<R>* a, cplx<R>* b) { m->sap(level, a, b, nu, tol, dd::full_mask(nrhs)); }, x);
+            m->check_sap_failures();
         };
         if (precision == 0) run(double());
         else run(float());

@@ -532,7 +532,7 @@ void Multigrid<R>::check_sap_failures() {
     DD_HIP(hipStreamSynchronize(s_));
     if (f) {
         DD_HIP(hipMemsetAsync(fuse_fail_.as<void>(), 0, sizeof(unsigned), s_));
-        throw BadShape("coarse SAP: a wait inside a fused iteration timed out; results of this call are invalid");
+        throw BadShape("SAP: a wait inside a fused iteration timed out; results of this call are invalid");
     }
 }
 
@@ -629,6 +629,7 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
                     hlinks_[l + 1].bytes() / sizeof(_Float16));
         DD_HIP(hipStreamSynchronize(s_));
     }
+    check_sap_failures();
 }
 
 template <typename R>
@@ -1004,6 +1005,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
         if (mg64) mg64->check_graph_failures();
         if (mg32) mg32->check_graph_failures();
     }
+    if (mg64) mg64->check_sap_failures();
     if (mg32) mg32->check_sap_failures();
     DD_HIP(hipStreamSynchronize(s));
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
