@@ -147,6 +147,24 @@ __device__ __forceinline__ cplx<float> narrow(chalf* d, cplx<float> v) {
     return widen(h);
 }
 
+// Complex multiply-accumulates as four fused multiply-adds each (a stored binary16
+// operand is widened inside the FMA): acc += conj(q) v and acc -= c q.
+template <typename R>
+__device__ __forceinline__ void cmac_conj(cplx<R>& acc, cplx<R> q, cplx<R> v) {
+    acc.x = fma(q.x, v.x, fma(q.y, v.y, acc.x));
+    acc.y = fma(q.x, v.y, fma(-q.y, v.x, acc.y));
+}
+template <typename R>
+__device__ __forceinline__ void cmsub(cplx<R>& acc, cplx<R> c, cplx<R> q) {
+    acc.x = fma(-c.x, q.x, fma(c.y, q.y, acc.x));
+    acc.y = fma(-c.x, q.y, fma(-c.y, q.x, acc.y));
+}
+template <typename R>
+__device__ __forceinline__ void cmadd(cplx<R>& acc, cplx<R> c, cplx<R> q) {
+    acc.x = fma(c.x, q.x, fma(-c.y, q.y, acc.x));
+    acc.y = fma(c.x, q.y, fma(c.y, q.x, acc.y));
+}
+
 template <typename R, typename S>
 struct CArgs {
     const cplx<R>* L;       // links of the coarsest level
@@ -521,13 +539,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
 #pragma unroll
                 for (int gi = 0; gi < kGV; gi++) {
                     const int ii = g0 + gi;
+                    // no mask on the basis values: lanes past the slice hold zero rows of u
+                    // and w, and the sums of clamped duplicates (i0 + ii >= j) are not stored
                     cplx<R> pa = czero<R>(), pb = czero<R>();
-                    const bool ok = i0 + ii < j && lrow;
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
-                        const cplx<R> qq = ok ? widen(qv[ii][k]) : czero<R>();
-                        pa = pa + cmul(qq, ur[k]);
-                        pb = pb + cmul(qq, wr[k]);
+                        const cplx<R> qq = widen(qv[ii][k]);
+                        cmac_conj(pa, qq, ur[k]);
+                        cmac_conj(pb, qq, wr[k]);
                     }
                     F[4 * gi] = pa.x;
                     F[4 * gi + 1] = pa.y;
@@ -842,8 +861,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
 #pragma unroll
                     for (int k = 0; k < RPL; k++) {
                         const cplx<R> qq = widen(qv[ii][k]);
-                        qj[k] = qj[k] - aa * qq;
-                        un[k] = un[k] - cc * qq;
+                        cmsub(qj[k], aa, qq);
+                        cmsub(un[k], cc, qq);
                     }
                 }
             };
@@ -941,7 +960,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 for (int ii = 0; ii < kCU; ii++) {
                     const cplx<R> ee = i0 + ii < nIo ? e[ii] : czero<R>();
 #pragma unroll
-                    for (int k = 0; k < RPL; k++) xr[k] = xr[k] + ee * widen(qv[ii][k]);
+                    for (int k = 0; k < RPL; k++) cmadd(xr[k], ee, widen(qv[ii][k]));
                 }
             }
             cplx<R>* xo = a.x + (size_t)q * n + row0 + lb;
