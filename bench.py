@@ -36,6 +36,7 @@ def log(*a):
 COARSE_BASIS = None   # None: the library default; 0 FP32, 1 binary16 (--coarse-basis)
 SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-sap-links)
 BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
+SAP_FUSED = None      # None: the library default; 0 separate launches per colour, 1 one launch per iteration
 
 
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
@@ -46,6 +47,8 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
     if SAP_LINKS is not None:
         p.coarse_sap_links = SAP_LINKS
     p.bootstrap = BOOTSTRAP
+    if SAP_FUSED is not None:
+        p.sap_fused = SAP_FUSED
     return p
 
 
@@ -77,11 +80,14 @@ def main():
                     help='skip the row that repeats the batch with an FP32 coarsest Krylov basis')
     ap.add_argument('--bootstrap', type=int, default=0,
                     help='bootstrap passes after the setup (0: the reference setup only)')
+    ap.add_argument('--sap-fused', type=int, choices=[0, 1], default=None,
+                    help='coarse SAP iterations in one launch (1) or per colour with separate residuals (0)')
     ap.add_argument('--setup-only', action='store_true',
                     help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
-    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP
+    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED
     BOOTSTRAP = args.bootstrap
+    SAP_FUSED = args.sap_fused
     COARSE_BASIS = args.coarse_basis
     SAP_LINKS = args.coarse_sap_links
     if args.gpus != 1:

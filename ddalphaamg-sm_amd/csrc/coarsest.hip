@@ -51,6 +51,9 @@ constexpr int kSpwMax = 8;   // sites per work-group at most
 constexpr int kGmax = 256;   // work-groups at most (one per compute unit)
 constexpr int kMaxM = 432;   // restart length at most (LDS state below 64 KB)
 constexpr int kBB = 32;      // block of the triangular solve
+// diagnostic clocks per work-group: 9 phases, then each wave's own reduction and update
+// pass times (kNW each)
+constexpr int kClk = 9 + 2 * (512 / 64);
 
 enum : int { F_LIVE = 1, F_RUN = 2 };
 
@@ -183,7 +186,8 @@ struct CArgs {
     double* stats;          // [2 + 8]: algorithmic bytes, Arnoldi steps, phase clocks of
                             // work-group 0 (A, barrier, B, barrier, C, barrier, cycle end)
     GridBar* bar;           // grid barrier state (zero at launch)
-    double* wgclk;          // [G][9] per-work-group phase clocks (diagnostics), or null
+    double* wgclk;          // [G][kClk] per-work-group phase clocks and per-wave pass times
+                            // (diagnostics), or null
     uint64_t mask;
     int Nx, Nt, nsys, m, restarts, spw;
     int lds_links;          // 1: the slice's links are cached in LDS (after the scalar state)
@@ -414,7 +418,7 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (tid == 0) {
             const uint64_t t = wall_clock64();
             if (wg == 0) a.stats[2 + slot] += (double)(t - tclk);
-            if (a.wgclk) a.wgclk[wg * 9 + slot] += (double)(t - tclk);
+            if (a.wgclk) a.wgclk[wg * kClk + slot] += (double)(t - tclk);
             tclk = t;
         }
     };
@@ -1024,10 +1028,12 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (hw) stencil(a.u, a.w, nullptr, run, false);
             __syncthreads();
             clk(7);
+            const uint64_t tw0 = a.wgclk ? wall_clock64() : 0;
             for (int qb = wv; qb * SPI < a.nsys; qb += kNW) {
                 const unsigned pm = pair_mask(run, qb);
                 if (pm) reduce_step(qb, j, hw, pm);
             }
+            if (a.wgclk && lane == 0) a.wgclk[wg * kClk + 9 + wv] += (double)(wall_clock64() - tw0);
             clk(0);
             // publish this slice's partial sums of the step; only the owners of running
             // systems wait for every work-group's (a grid barrier's worth of ordering for
@@ -1047,7 +1053,9 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (!si[2]) return;
             clk(3);
             if (tid == 0 && j >= 1 && owner && ((run >> sys) & 1ull)) givens(j);
+            const uint64_t tw1 = a.wgclk ? wall_clock64() : 0;
             update(j, cont);
+            if (a.wgclk && lane == 0) a.wgclk[wg * kClk + 9 + kNW + wv] += (double)(wall_clock64() - tw1);
             clk(4);
             const int nc = __builtin_popcountll(cont);
             count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * (sQ * j + sR * (hw ? 2 : 1)) +
@@ -1221,8 +1229,8 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     a.bar = bar_.as<GridBar>();
     static const bool verbose = std::getenv("DDAMG_VERBOSE") != nullptr;
     if (verbose && !deferred && !wgclk_.bytes()) {
-        wgclk_.alloc(sizeof(double) * 9 * G_);
-        DD_HIP(hipMemsetAsync(wgclk_.as<void>(), 0, sizeof(double) * 9 * G_, s_));
+        wgclk_.alloc(sizeof(double) * kClk * G_);
+        DD_HIP(hipMemsetAsync(wgclk_.as<void>(), 0, sizeof(double) * kClk * G_, s_));
     }
     a.wgclk = (verbose && !deferred) ? wgclk_.as<double>() : nullptr;
     a.mask = mask;
@@ -1262,15 +1270,16 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     double* st = reinterpret_cast<double*>(ch + cap_);
     DD_HIP(hipMemcpyAsync(ch, a.ctrl, sizeof(Ctrl) * nsys, hipMemcpyDeviceToHost, s_));
     DD_HIP(hipMemcpyAsync(st, a.stats, sizeof(double) * 12, hipMemcpyDeviceToHost, s_));
-    GridBar* bh = reinterpret_cast<GridBar*>(st + 12);
-    DD_HIP(hipMemcpyAsync(bh, a.bar, sizeof(GridBar), hipMemcpyDeviceToHost, s_));
+    // only the fail word of the barrier state comes back
+    unsigned* bfail = reinterpret_cast<unsigned*>(st + 12);
+    DD_HIP(hipMemcpyAsync(bfail, &a.bar->fail, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
     DD_HIP(hipStreamSynchronize(s_));
-    if (bh->fail) {
+    if (*bfail) {
         fprintf(stderr, "coarsest GMRES: grid barrier timed out\n");
         throw HipFailure(hipErrorLaunchTimeOut);
     }
     if (verbose && wgclk_.bytes() && ++solves_ % 19 == 0) {   // spread of the phase times over the work-groups
-        std::vector<double> h((size_t)9 * G_);
+        std::vector<double> h((size_t)kClk * G_);
         DD_HIP(hipMemcpy(h.data(), a.wgclk, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
         const char* nm[9] = {"reduce", "pub-part", "scalar", "wait-cf", "update", "pub-upd", "cycle", "nbr+sten",
                              "gather"};
@@ -1278,7 +1287,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
             double mn = 1e300, mx = 0, sum = 0;
             int imx = 0;
             for (int g = 0; g < G_; g++) {
-                const double v = h[(size_t)g * 9 + k];
+                const double v = h[(size_t)g * kClk + k];
                 mn = std::min(mn, v);
                 if (v > mx) {
                     mx = v;
@@ -1288,6 +1297,25 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
             }
             fprintf(stderr, "coarsest wg clocks %-8s min %.3g mean %.3g max %.3g (wg %d)\n", nm[k], mn, sum / G_, mx,
                     imx);
+        }
+        // each wave's own pass times: mean over the work-groups per wave index, and the mean
+        // spread (slowest minus fastest wave) within a work-group
+        for (int pass = 0; pass < 2; pass++) {
+            double spread = 0;
+            std::vector<double> per(kNW, 0.0);
+            for (int g = 0; g < G_; g++) {
+                double mn = 1e300, mx = 0;
+                for (int w = 0; w < kNW; w++) {
+                    const double v = h[(size_t)g * kClk + 9 + pass * kNW + w];
+                    per[w] += v / G_;
+                    mn = std::min(mn, v);
+                    mx = std::max(mx, v);
+                }
+                spread += (mx - mn) / G_;
+            }
+            fprintf(stderr, "coarsest wave clocks %s:", pass ? "update" : "reduce");
+            for (int w = 0; w < kNW; w++) fprintf(stderr, " %.3g", per[w]);
+            fprintf(stderr, " | mean spread in a work-group %.3g\n", spread);
         }
     }
     for (int k = 0; k < 9; k++) phase_clk[k] += st[2 + k];
