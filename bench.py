@@ -37,6 +37,7 @@ COARSE_BASIS = None   # None: the library default; 0 FP32, 1 binary16 (--coarse-
 SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-sap-links)
 BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
 SAP_FUSED = None      # None: the library default; 0 separate launches per colour, 1 one launch per iteration
+GRAPH = 0             # 1: the cycles replayed from captured HIP graphs (--graph)
 
 
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
@@ -49,6 +50,7 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
     p.bootstrap = BOOTSTRAP
     if SAP_FUSED is not None:
         p.sap_fused = SAP_FUSED
+    p.graph = GRAPH
     return p
 
 
@@ -82,11 +84,14 @@ def main():
                     help='bootstrap passes after the setup (0: the reference setup only)')
     ap.add_argument('--sap-fused', type=int, choices=[0, 1], default=None,
                     help='coarse SAP iterations in one launch (1) or per colour with separate residuals (0)')
+    ap.add_argument('--graph', type=int, choices=[0, 1], default=0,
+                    help='replay the cycles from captured HIP graphs (K-cycle FGMRES device-controlled)')
     ap.add_argument('--setup-only', action='store_true',
                     help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
-    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED
+    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED, GRAPH
     BOOTSTRAP = args.bootstrap
+    GRAPH = args.graph
     SAP_FUSED = args.sap_fused
     COARSE_BASIS = args.coarse_basis
     SAP_LINKS = args.coarse_sap_links
@@ -330,7 +335,8 @@ def probe(ddamg, torch, which, N, nrhs, seed):
 
 def secondary_rows(ddamg, torch, sweeps, seed):
     """Further workloads: the 256^2 plain FP64 single-RHS solve, 32 RHS on one 256^2 field
-    (mixed, batched; K-cycle, and V-cycle eager and from captured graphs), and a ragged
+    (mixed, batched; K-cycle eager and from captured graphs, V-cycle eager and from captured
+    graphs), and a ragged
     batch of 8 fields of sizes {64,96,128,128,192,256,256,256}^2 with m0 drawn from
     [-0.18, -0.10] and 4 RHS each, all solved at the same time (dd_solve_many_dev).
     Gauge fields: seeded quenched Metropolis, beta = 2."""
@@ -363,10 +369,11 @@ def secondary_rows(ddamg, torch, sweeps, seed):
         out[key] = {'ms_per_rhs': t * 1e3 / nr, 'iterations': [int(its.min()), int(its.max())],
                     'max_relres': info.relres}
         log(f'{key}: {t * 1e3 / nr:.2f} ms/RHS, its {its.min()}..{its.max()}')
-    # the same 32 RHS with V-cycles, eager and replayed from captured HIP graphs
-    for graph in (0, 1):
+    # the same 32 RHS with the K-cycle replayed from captured HIP graphs (its FGMRES solves
+    # device-controlled), then with V-cycles, eager and replayed from captured graphs
+    for cyc, graph in ((1, 1), (0, 0), (0, 1)):
         p = hierarchy_params(ddamg, 256, 1, 32, -0.18)
-        p.cycle = 0
+        p.cycle = cyc
         p.graph = graph
         ctx.setup(p)
         ctx.solve_dev(b.data_ptr(), x.data_ptr(), 32)   # warm-up (captures the graphs)
@@ -375,7 +382,8 @@ def secondary_rows(ddamg, torch, sweeps, seed):
         its, info = ctx.solve_dev(b.data_ptr(), x.data_ptr(), 32)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
-        key = 'mixed_256_32rhs_vcycle_graph' if graph else 'mixed_256_32rhs_vcycle'
+        key = ('mixed_256_32rhs_kcycle_graph' if cyc else
+               'mixed_256_32rhs_vcycle_graph' if graph else 'mixed_256_32rhs_vcycle')
         out[key] = {'ms_per_rhs': t * 1e3 / 32, 'iterations': [int(its.min()), int(its.max())],
                     'max_relres': info.relres, 'graphs': ctx.graph_count()}
         log(f'{key}: {t * 1e3 / 32:.2f} ms/RHS, its {its.min()}..{its.max()}, graphs {ctx.graph_count()}')

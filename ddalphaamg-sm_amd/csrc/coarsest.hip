@@ -189,6 +189,7 @@ struct CArgs {
     double* wgclk;          // [G][kClk] per-work-group phase clocks and per-wave pass times
                             // (diagnostics), or null
     uint64_t mask;
+    const uint64_t* dmask;  // device word ANDed into mask at the start (nullptr: none)
     int Nx, Nt, nsys, m, restarts, spw;
     int lds_links;          // 1: the slice's links are cached in LDS (after the scalar state)
     R m0;
@@ -350,6 +351,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     constexpr int kCH = Chunk<S, RPL>::RED, kCU = Chunk<S, RPL>::UPD, kGV = kCH < 8 ? kCH : 8, NVAL = 4 * kGV,
                   SH = ilog2c(LPS) - ilog2c(NVAL);
     static_assert(kCH % kGV == 0 && (kGV == 8 || kGV == 4), "chunk of 4, 8 or a multiple of 8 vectors");
+    // systems decided on the device (a device-controlled caller): every work-group reads
+    // the same word; a launch with no system left ends at once
+    if (a.dmask) a.mask &= *a.dmask;
+    if (!a.mask) return;
     unsigned bar_gen = 0;
     extern __shared__ unsigned char smem_raw[];
     const int m = a.m, VALS = 2 * m + 2, CST = 2 * m + 2;
@@ -1234,6 +1239,7 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     }
     a.wgclk = (verbose && !deferred) ? wgclk_.as<double>() : nullptr;
     a.mask = mask;
+    a.dmask = dev_mask();
     a.Nx = Nx_;
     a.Nt = Nt_;
     a.nsys = nsys;

@@ -27,6 +27,21 @@ bool timer_any();
 // since the last reset (synchronises the events)
 double timer_read(int id, long long* count, double* bytes);
 
+// ---- device-side system masks ---------------------------------------------------------
+// While a DevMaskScope is alive on the calling host thread, every launcher below that takes
+// a system mask also hands its kernel the device word m, and the kernel ANDs its mask with
+// *m when it starts: the systems a launch serves can then be decided on the device (the
+// run masks of a device-controlled Krylov solve inside a captured graph).  Scopes nest;
+// without one (nullptr) launches use their host masks only.
+const uint64_t* dev_mask();
+struct DevMaskScope {
+    const uint64_t* prev;
+    explicit DevMaskScope(const uint64_t* m);
+    ~DevMaskScope();
+    DevMaskScope(const DevMaskScope&) = delete;
+    DevMaskScope& operator=(const DevMaskScope&) = delete;
+};
+
 // Maximum block-GMRES restart length compiled into the SAP block kernels.
 constexpr int kSapM = 5;
 
@@ -165,6 +180,8 @@ void to_half(hipStream_t s, const float* in, _Float16* out, size_t n);
 void from_half(hipStream_t s, const _Float16* in, double* out, size_t n);
 template <typename R>
 void sigma3(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, int nrhs, uint64_t mask);
+// gate[r] = 1 for the 64 systems, or (inside a DevMaskScope) bit r of the device mask
+void gate_init(hipStream_t s, int* gate);
 // gate[r] = 0 where sqrt(err2[r*stride].x) < tol * sqrt(v2[r*stride].x)
 void gate_update(hipStream_t s, int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
                  int nrhs, uint64_t mask);

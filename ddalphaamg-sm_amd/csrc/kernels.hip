@@ -93,6 +93,14 @@ double timer_read(int id, long long* count, double* bytes) {
     return tot;
 }
 
+// ---- device-side system mask of the calling thread's launches ---------------------------
+namespace {
+thread_local const uint64_t* tl_dmask = nullptr;
+}  // namespace
+const uint64_t* dev_mask() { return tl_dmask; }
+DevMaskScope::DevMaskScope(const uint64_t* m) : prev(tl_dmask) { tl_dmask = m; }
+DevMaskScope::~DevMaskScope() { tl_dmask = prev; }
+
 static inline unsigned cdivu(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 static inline double nact(uint64_t mask, int nrhs) { return (double)__builtin_popcountll(mask & ((nrhs >= 64) ? ~0ull : ((1ull << nrhs) - 1))); }
 
@@ -110,7 +118,8 @@ __device__ __forceinline__ int xcd_block(int p, int G) { return (G & 7) ? p : (p
 template <typename R, bool DAG, bool RES, int RB>
 __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
-                                                int Nt, R m0, uint64_t mask, size_t ld, int nsb, int nrhs) {
+                                                int Nt, R m0, uint64_t mask, size_t ld, int nsb, int nrhs, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int L = xcd_block(blockIdx.x, gridDim.x);
     const int sb = L % nsb, r0 = (L / nsb) * RB;
     const int n = sb * blockDim.x + threadIdx.x;
@@ -166,7 +175,7 @@ static void launch_wilson(hipStream_t s, const cplx<R>* U, const cplx<R>* in, cp
     constexpr int RB = 4;
     const int nsb = (int)cdivu((size_t)Nx * Nt, 256), ng = (nrhs + RB - 1) / RB;
     k_wilson<R, DAG, RES, RB><<<dim3((unsigned)nsb * ng), 256, 0, s>>>(U, in, out, b, Nx, Nt, (R)m0, mask, ld, nsb,
-                                                                      nrhs);
+                                                                      nrhs, dev_mask());
 }
 
 template <typename R>
@@ -225,7 +234,8 @@ void dirac_links(hipStream_t s, const cplx<R>* U, cplx<R>* L, int Nx, int Nt) {
 template <typename R, int DOF, bool RES, int NT>
 __global__ __launch_bounds__(256) void k_coarse(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx,
-                                                int Nt, R m0, uint64_t mask, size_t ld, int nrhs) {
+                                                int Nt, R m0, uint64_t mask, size_t ld, int nrhs, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     // XCD by XCD, as k_wilson: the neighbour sites' vectors come from the same XCD's L2
     const int n = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (n >= Nx * Nt) return;   // uniform per wave
@@ -248,11 +258,11 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
 #define DD_CASE(D)                                                                                            \
     case D:                                                                                                   \
         if (two) {                                                                                            \
-            if (b) k_coarse<R, D, true, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
-            else k_coarse<R, D, false, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+            if (b) k_coarse<R, D, true, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs, dev_mask()); \
+            else k_coarse<R, D, false, 2><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs, dev_mask()); \
         } else {                                                                                              \
-            if (b) k_coarse<R, D, true, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
-            else k_coarse<R, D, false, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs); \
+            if (b) k_coarse<R, D, true, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs, dev_mask()); \
+            else k_coarse<R, D, false, 1><<<grid, 64 * kWaves, 0, s>>>(L, in, out, b, Nx, Nt, (R)m0, mask, ld, nrhs, dev_mask()); \
         }                                                                                                     \
         break;
         DD_DOF_LIST(DD_CASE)
@@ -269,7 +279,8 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
 // ======================================================================================
 template <typename R>
 __global__ void k_restrict(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
-                           int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf, size_t ldc) {
+                           int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf, size_t ldc, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     const int Ntc = Nt / at, Vc = (Nx / ax) * Ntc;
@@ -297,7 +308,7 @@ void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, 
     timer_begin(s, K_RESTRICT);
     const size_t tot = (size_t)(Nx / ax) * (Nt / at) * 2 * nv;
     dim3 grid(cdivu(tot, 256), nrhs);
-    k_restrict<R><<<grid, 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    k_restrict<R><<<grid, 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc, dev_mask());
     timer_end(s, K_RESTRICT, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
@@ -305,7 +316,8 @@ void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, 
 template <typename R>
 __global__ void k_prolong(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ vc, const cplx<R>* y,
                           cplx<R>* out, int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf,
-                          size_t ldc) {
+                          size_t ldc, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -326,7 +338,7 @@ void prolong(hipStream_t s, const cplx<R>* W, const cplx<R>* vc, const cplx<R>* 
              int dof, int ax, int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
     timer_begin(s, K_PROLONG);
     dim3 grid(cdivu((size_t)Nx * Nt * dof, 256), nrhs);
-    k_prolong<R><<<grid, 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc);
+    k_prolong<R><<<grid, 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc, dev_mask());
     timer_end(s, K_PROLONG, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + (y ? 2 : 1) * nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
@@ -497,7 +509,8 @@ void transpose_tv(hipStream_t s, const cplx<R>* in, cplx<R>* out, int V, int nv,
 template <typename R>
 __global__ __launch_bounds__(256) void k_dot_partial(const cplx<R>* __restrict__ a, const cplx<R>* __restrict__ b,
                                                      size_t n, size_t ld, uint64_t mask,
-                                                     cplx<double>* __restrict__ partial) {
+                                                     cplx<double>* __restrict__ partial, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     __shared__ R scratch[32];
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
@@ -520,7 +533,7 @@ void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, si
     timer_begin(s, K_DOT);
     rb.nblk = red_blocks(n);
     dim3 grid(rb.nblk, nrhs);
-    k_dot_partial<R><<<grid, 256, 0, s>>>(a, b, n, ld, mask, (cplx<double>*)rb.partial);
+    k_dot_partial<R><<<grid, 256, 0, s>>>(a, b, n, ld, mask, (cplx<double>*)rb.partial, dev_mask());
     timer_end(s, K_DOT, nact(mask, nrhs) * (double)n * 2 * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
@@ -535,7 +548,8 @@ __device__ __forceinline__ cplx<double> sum_partials(const cplx<double>* p, int 
 }
 
 __global__ void k_dot_final(const cplx<double>* __restrict__ partial, int nblk, cplx<double>* out, int ostride,
-                            uint64_t mask) {
+                            uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.x;
     if (!rhs_on(mask, r)) return;
     const cplx<double> s = sum_partials(partial + (size_t)r * nblk, nblk);
@@ -544,14 +558,15 @@ __global__ void k_dot_final(const cplx<double>* __restrict__ partial, int nblk, 
 
 template <typename R>
 void dot_final(hipStream_t s, const RedBuf& rb, cplx<double>* out, int ostride, int nrhs, uint64_t mask) {
-    k_dot_final<<<nrhs, 64, 0, s>>>((const cplx<double>*)rb.partial, rb.nblk, out, ostride, mask);
+    k_dot_final<<<nrhs, 64, 0, s>>>((const cplx<double>*)rb.partial, rb.nblk, out, ostride, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_mgs_update(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ v, size_t n,
                                                     size_t ld, uint64_t mask, const cplx<double>* __restrict__ partial,
-                                                    int nblk, cplx<double>* H, int hstride) {
+                                                    int nblk, cplx<double>* H, int hstride, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     __shared__ cplx<double> hs;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
@@ -576,14 +591,15 @@ template <typename R>
 void mgs_update(hipStream_t s, cplx<R>* w, const cplx<R>* v, size_t n, size_t ld, int nrhs, uint64_t mask,
                 const RedBuf& rb, cplx<double>* H, int hstride) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_mgs_update<R><<<grid, 256, 0, s>>>(w, v, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride);
+    k_mgs_update<R><<<grid, 256, 0, s>>>(w, v, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_norm_scale(cplx<R>* __restrict__ out, const cplx<R>* __restrict__ w, size_t n,
                                                     size_t ld, uint64_t mask, const cplx<double>* __restrict__ partial,
-                                                    int nblk, cplx<double>* H, int hstride) {
+                                                    int nblk, cplx<double>* H, int hstride, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     __shared__ double hs;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
@@ -610,14 +626,15 @@ template <typename R>
 void norm_scale(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
                 const RedBuf& rb, cplx<double>* H, int hstride) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_norm_scale<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride);
+    k_norm_scale<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_scale_inv(cplx<R>* __restrict__ out, const cplx<R>* __restrict__ w, size_t n,
                                                    size_t ld, uint64_t mask, const cplx<double>* __restrict__ beta,
-                                                   int bstride) {
+                                                   int bstride, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     const R bt = (R)beta[(size_t)r * bstride].x;
@@ -632,13 +649,14 @@ template <typename R>
 void scale_inv(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,
                const cplx<double>* beta, int bstride) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_scale_inv<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, beta, bstride);
+    k_scale_inv<R><<<grid, 256, 0, s>>>(out, w, n, ld, mask, beta, bstride, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_axpy(cplx<R>* out, const cplx<R>* x, const cplx<R>* y, R alpha, size_t n,
-                                              size_t ld, uint64_t mask) {
+                                              size_t ld, uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -649,7 +667,7 @@ template <typename R>
 void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, double alpha, size_t n, size_t ld, int nrhs,
           uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_axpy<R><<<grid, 256, 0, s>>>(out, x, y, (R)alpha, n, ld, mask);
+    k_axpy<R><<<grid, 256, 0, s>>>(out, x, y, (R)alpha, n, ld, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
@@ -658,7 +676,8 @@ void axpy(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, doubl
 template <typename R>
 __global__ __launch_bounds__(256) void k_lincomb(cplx<R>* out, const cplx<R>* x, const cplx<R>* y, const cplx<R>* z,
                                                  const cplx<double>* __restrict__ c, size_t n, size_t ld,
-                                                 uint64_t mask) {
+                                                 uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     const cplx<double> a = c[3 * r], b = c[3 * r + 1], d = c[3 * r + 2];
@@ -677,7 +696,7 @@ template <typename R>
 void lincomb(hipStream_t s, cplx<R>* out, const cplx<R>* x, const cplx<R>* y, const cplx<R>* z,
              const cplx<double>* c, size_t n, size_t ld, int nrhs, uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_lincomb<R><<<grid, 256, 0, s>>>(out, x, y, z, c, n, ld, mask);
+    k_lincomb<R><<<grid, 256, 0, s>>>(out, x, y, z, c, n, ld, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
@@ -685,7 +704,8 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_update_x(cplx<R>* __restrict__ x, const cplx<R>* __restrict__ Z,
                                                   size_t zstride, const cplx<double>* __restrict__ eta,
                                                   const int* __restrict__ nit, int m, size_t n, size_t ld,
-                                                  uint64_t mask) {
+                                                  uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     const int k = nit[r];
@@ -704,7 +724,7 @@ template <typename R>
 void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const cplx<double>* eta, const int* nit,
               int m, size_t n, size_t ld, int nrhs, uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_update_x<R><<<grid, 256, 0, s>>>(x, Z, zstride, eta, nit, m, n, ld, mask);
+    k_update_x<R><<<grid, 256, 0, s>>>(x, Z, zstride, eta, nit, m, n, ld, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
@@ -713,7 +733,8 @@ void update_x(hipStream_t s, cplx<R>* x, const cplx<R>* Z, size_t zstride, const
 template <typename R>
 __global__ __launch_bounds__(256) void k_multidot(const cplx<R>* __restrict__ w, const cplx<R>* __restrict__ V,
                                                   size_t vstride, int nvec, size_t n, size_t ld, uint64_t mask,
-                                                  cplx<double>* __restrict__ partial, int nblk) {
+                                                  cplx<double>* __restrict__ partial, int nblk, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     // grid: (nblk * nvec, nrhs); block b = (vector i, chunk c)
     __shared__ R scratch[32];
     const int r = blockIdx.y;
@@ -734,7 +755,7 @@ void multidot(hipStream_t s, const cplx<R>* w, const cplx<R>* V, size_t vstride,
     timer_begin(s, K_MULTIDOT);
     rb.nblk = red_blocks(n);
     dim3 grid(rb.nblk * nvec, nrhs);
-    k_multidot<R><<<grid, 256, 0, s>>>(w, V, vstride, nvec, n, ld, mask, (cplx<double>*)rb.partial, rb.nblk);
+    k_multidot<R><<<grid, 256, 0, s>>>(w, V, vstride, nvec, n, ld, mask, (cplx<double>*)rb.partial, rb.nblk, dev_mask());
     timer_end(s, K_MULTIDOT, nact(mask, nrhs) * (double)n * (nvec + 1) * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
@@ -743,7 +764,8 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_multi_axpy(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ V,
                                                     size_t vstride, int nvec, size_t n, size_t ld, uint64_t mask,
                                                     const cplx<double>* __restrict__ partial, int nblk,
-                                                    cplx<double>* H, int hstride, int accumulate) {
+                                                    cplx<double>* H, int hstride, int accumulate, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     extern __shared__ unsigned char smem_raw[];
     cplx<double>* hs = reinterpret_cast<cplx<double>*>(smem_raw);
     const int r = blockIdx.y;
@@ -774,14 +796,15 @@ void multi_axpy(hipStream_t s, cplx<R>* w, const cplx<R>* V, size_t vstride, int
     dim3 grid(ew_blocks(n), nrhs);
     const size_t sh = sizeof(cplx<double>) * nvec;
     k_multi_axpy<R><<<grid, 256, sh, s>>>(w, V, vstride, nvec, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H,
-                                          hstride, accumulate ? 1 : 0);
+                                          hstride, accumulate ? 1 : 0, dev_mask());
     timer_end(s, K_MULTIAXPY, nact(mask, nrhs) * (double)n * (nvec + 2) * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
 template <typename To, typename From>
 __global__ __launch_bounds__(256) void k_convert(cplx<To>* __restrict__ out, const cplx<From>* __restrict__ in, size_t n,
-                                                 size_t ldo, size_t ldi, uint64_t mask) {
+                                                 size_t ldo, size_t ldi, uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -791,7 +814,7 @@ __global__ __launch_bounds__(256) void k_convert(cplx<To>* __restrict__ out, con
 template <typename R>
 void copy(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ldo, size_t ldi, int nrhs, uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_convert<R, R><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask);
+    k_convert<R, R><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
@@ -799,7 +822,7 @@ template <typename To, typename From>
 void convert(hipStream_t s, cplx<To>* out, const cplx<From>* in, size_t n, size_t ldo, size_t ldi, int nrhs,
              uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_convert<To, From><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask);
+    k_convert<To, From><<<grid, 256, 0, s>>>(out, in, n, ldo, ldi, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
@@ -824,7 +847,8 @@ void from_half(hipStream_t s, const _Float16* in, double* out, size_t n) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_zero(cplx<R>* out, size_t n, size_t ld, uint64_t mask) {
+__global__ __launch_bounds__(256) void k_zero(cplx<R>* out, size_t n, size_t ld, uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -834,12 +858,13 @@ __global__ __launch_bounds__(256) void k_zero(cplx<R>* out, size_t n, size_t ld,
 template <typename R>
 void zero(hipStream_t s, cplx<R>* out, size_t n, size_t ld, int nrhs, uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_zero<R><<<grid, 256, 0, s>>>(out, n, ld, mask);
+    k_zero<R><<<grid, 256, 0, s>>>(out, n, ld, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_sigma3(cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, uint64_t mask) {
+__global__ __launch_bounds__(256) void k_sigma3(cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, uint64_t mask, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -851,20 +876,31 @@ __global__ __launch_bounds__(256) void k_sigma3(cplx<R>* out, const cplx<R>* in,
 template <typename R>
 void sigma3(hipStream_t s, cplx<R>* out, const cplx<R>* in, size_t n, size_t ld, int nrhs, uint64_t mask) {
     dim3 grid(ew_blocks(n), nrhs);
-    k_sigma3<R><<<grid, 256, 0, s>>>(out, in, n, ld, mask);
+    k_sigma3<R><<<grid, 256, 0, s>>>(out, in, n, ld, mask, dev_mask());
     DD_LAUNCH_CHECK();
 }
 
 __global__ void k_gate(int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
-                       uint64_t mask, int nrhs) {
+                       uint64_t mask, int nrhs, const uint64_t* __restrict__ dm) {
+    if (dm) mask &= *dm;
     const int r = threadIdx.x;
     if (r >= nrhs || !rhs_on(mask, r)) return;
     if (sqrt(err2[r * stride].x) < tol * sqrt(v2[r * stride].x)) gate[r] = 0;
 }
 
+__global__ void k_gate_init(int* gate, const uint64_t* __restrict__ dm) {
+    const int r = threadIdx.x;
+    gate[r] = dm ? (int)((*dm >> r) & 1ull) : 1;
+}
+
+void gate_init(hipStream_t s, int* gate) {
+    k_gate_init<<<1, 64, 0, s>>>(gate, dev_mask());
+    DD_LAUNCH_CHECK();
+}
+
 void gate_update(hipStream_t s, int* gate, const cplx<double>* err2, const cplx<double>* v2, int stride, double tol,
                  int nrhs, uint64_t mask) {
-    k_gate<<<1, 64, 0, s>>>(gate, err2, v2, stride, tol, mask, nrhs);
+    k_gate<<<1, 64, 0, s>>>(gate, err2, v2, stride, tol, mask, nrhs, dev_mask());
     DD_LAUNCH_CHECK();
 }
 

@@ -37,6 +37,16 @@ class Krylov {
     // Returns a mask of the systems that converged; iters[r] (cap entries) per system.
     uint64_t solve(const cplx<R>* phi, const cplx<R>* x0, cplx<R>* x, uint64_t mask, const Func& A,
                    const Prec* M, int* iters, double* relres);
+    // The same solve with x0 = phi and every control decision on the device, for a
+    // captured graph: no host read-back.  The systems are those of mask that the calling
+    // scope's device mask runs (DevMaskScope).  The Givens rotations, the break on
+    // |g_{j+1}| < tol ||phi||, maxIt kept across restarts and the true-residual test of
+    // each cycle are made per system by one-wave control kernels.  Every launch is masked
+    // on the device by the cycle's live or the step's run systems, so all m * restarts
+    // steps are issued and the systems that have stopped do nothing in them.
+    void solve_dc(const cplx<R>* phi, cplx<R>* x, uint64_t mask, const Func& A, const Prec* M);
+    // allocates the device control state of solve_dc (outside any graph capture)
+    void alloc_ctl();
     size_t n() const { return n_; }
     long long steps = 0;   // Arnoldi steps taken (max over the batch per step), cumulative
     // restart length (at most the constructed one), cycles and tolerance of later solves
@@ -53,7 +63,7 @@ class Krylov {
     int m_, restarts_, cap_, ortho_, m_max_;
     double tol_;
     bool flex_;
-    DevMem V_, Z_, r_, w_, H_, nrm_, eta_, nit_, part_;
+    DevMem V_, Z_, r_, w_, H_, nrm_, eta_, nit_, part_, ctl_;
     PinMem Hh_, nrmh_, etah_, nith_;
     RedBuf rb_;
 };
@@ -117,6 +127,9 @@ class Multigrid {
     // when the cycle cannot be captured (no persistent coarsest solver, kernel timers on,
     // or the runtime refused the capture once); the caller then runs it eagerly.
     bool vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    // the same for the K-cycle: its FGMRES solves at levels 1 .. L-2 run device-controlled
+    // (Krylov::solve_dc) inside the capture
+    bool kcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     // reports (throws) a timed-out persistent coarsest solve among the graph replays
     void check_graph_failures();
     // reports (throws) a timed-out wait inside a fused SAP iteration since the last check
@@ -174,6 +187,8 @@ class Multigrid {
     std::vector<int> sapc_var_;
     std::vector<double> sapc_ms_;
     bool capturing_ = false, graphs_ok_ = true;
+    bool dc_ = false;   // K-cycle FGMRES solves device-controlled (set while a K-cycle is captured)
+    bool capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool kcyc);
     // fused SAP iterations (sap_fused_iteration) per level: block epochs, counters, norm
     // partials, launch epoch, and whether the level's shape allows them
     std::vector<DevMem> fuse_done_, fuse_ctr_, fuse_part_;
@@ -181,7 +196,7 @@ class Multigrid {
     std::vector<unsigned> fuse_epoch_;
     std::vector<char> fuse_ok_;
     int fuse_slices_ = 256;
-    std::map<std::tuple<const void*, const void*, uint64_t>, hipGraphExec_t> graphs_;
+    std::map<std::tuple<const void*, const void*, uint64_t, bool>, hipGraphExec_t> graphs_;
 };
 
 // ---- top-level solver --------------------------------------------------------------------

@@ -40,6 +40,8 @@ void PinMem::release() {
     p_ = nullptr;
 }
 
+size_t kview_bytes(int m);   // bytes of the device control state of Krylov::solve_dc
+
 __global__ void k_fill_int(int* p, int v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -80,6 +82,11 @@ Krylov<R>::Krylov(hipStream_t s, size_t n, int m, int restarts, double tol, int 
     nrmh_.alloc(sizeof(cplx<double>) * cap * 4);
     etah_.alloc(sizeof(cplx<double>) * cap * m);
     nith_.alloc(sizeof(int) * cap);
+}
+
+template <typename R>
+void Krylov<R>::alloc_ctl() {
+    if (!ctl_.bytes()) ctl_.alloc(kview_bytes(m_max_));
 }
 
 template <typename R>
@@ -229,6 +236,217 @@ uint64_t Krylov<R>::solve(const cplx<R>* phi, const cplx<R>* x0, cplx<R>* x, uin
     return mask & done;
 }
 
+// ---- device-controlled solve (Krylov::solve_dc) ---------------------------------------
+// Control state: the masks of the systems still to solve in this cycle (live), still in
+// the Arnoldi loop (run) and finished (done), then per system ||phi||, the last true
+// residual norm, maxIt, the iteration count, the least-squares right-hand side g, the
+// rotations (cn, sn) and the rotated Hessenberg matrix Hr[i*m + j] (src/fgmres.cpp:5-127).
+struct KCtl {
+    uint64_t live, run, done, pad;
+};
+struct KView {
+    KCtl* h;
+    double *nphi, *err;
+    int *maxit, *iters;
+    cplx<double> *g, *cn, *sn, *Hr;
+    int m;
+    __host__ __device__ static size_t bytes(int cap, int m) {
+        return sizeof(KCtl) + (size_t)cap * (2 * sizeof(double) + 2 * sizeof(int)) +
+               sizeof(cplx<double>) * (size_t)cap * ((m + 1) + 2 * m + (m + 1) * m) + 64;
+    }
+    __host__ __device__ static KView at(void* base, int cap, int m) {
+        KView v;
+        unsigned char* p = static_cast<unsigned char*>(base);
+        v.h = reinterpret_cast<KCtl*>(p);
+        p += sizeof(KCtl);
+        v.nphi = reinterpret_cast<double*>(p);
+        v.err = v.nphi + cap;
+        p += 2 * sizeof(double) * cap;
+        v.maxit = reinterpret_cast<int*>(p);
+        v.iters = v.maxit + cap;
+        p += 2 * sizeof(int) * cap;
+        p = reinterpret_cast<unsigned char*>((reinterpret_cast<uintptr_t>(p) + 15) & ~(uintptr_t)15);
+        v.g = reinterpret_cast<cplx<double>*>(p);
+        v.cn = v.g + (size_t)cap * (m + 1);
+        v.sn = v.cn + (size_t)cap * m;
+        v.Hr = v.sn + (size_t)cap * m;
+        v.m = m;
+        return v;
+    }
+};
+
+// one lane per system: ||phi||, ||phi - A x0||, maxIt = m, iters = restarts*m; a system
+// outside the batch (mask, and the caller's device mask) or with a zero residual is done
+__global__ void k_kc_init(KView v, const cplx<double>* nrm, uint64_t mask, const uint64_t* parent, int restarts) {
+    const int q = threadIdx.x;
+    const uint64_t batch = mask & (parent ? *parent : ~0ull);
+    const bool in = (batch >> q) & 1ull;
+    bool dn = !in;
+    if (in) {
+        v.nphi[q] = sqrt(nrm[4 * q].x);
+        v.err[q] = sqrt(nrm[4 * q + 1].x);
+        v.maxit[q] = v.m;
+        v.iters[q] = v.err[q] == 0.0 ? 0 : restarts * v.m;
+        dn = v.err[q] == 0.0;
+    }
+    const uint64_t done = __ballot(dn);
+    if (q == 0) {
+        v.h->done = done;
+        v.h->live = 0;
+        v.h->run = 0;
+    }
+}
+
+// cycle start: live = run = the systems not done; g_0 = ||r||
+__global__ void k_kc_start(KView v) {
+    const int q = threadIdx.x;
+    const uint64_t live = ~v.h->done;
+    if ((live >> q) & 1ull) v.g[(size_t)q * (v.m + 1)] = mk<double>(v.err[q], 0.0);
+    __syncthreads();
+    if (q == 0) {
+        v.h->live = live;
+        v.h->run = live;
+    }
+}
+
+// Givens rotation of column j (src/fgmres.cpp:99-115) and the break test, per running system
+__global__ void k_kc_givens(KView v, const cplx<double>* Hj, int hstride, int j, double tol) {
+    const int q = threadIdx.x, m = v.m;
+    const bool on = (v.h->run >> q) & 1ull;
+    bool stop = false;
+    if (on) {
+        cplx<double>* cn = v.cn + (size_t)q * m;
+        cplx<double>* sn = v.sn + (size_t)q * m;
+        cplx<double>* g = v.g + (size_t)q * (m + 1);
+        cplx<double>* H = v.Hr + (size_t)q * (m + 1) * m;
+        const cplx<double>* col = Hj + (size_t)q * hstride;
+        for (int i = 0; i <= j + 1; i++) H[(size_t)i * m + j] = col[i];
+        for (int i = 0; i < j; i++) {
+            const cplx<double> a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+            const cplx<double> temp = cmul(cn[i], a) + cmul(sn[i], b);
+            H[(size_t)(i + 1) * m + j] = cn[i] * b - sn[i] * a;
+            H[(size_t)i * m + j] = temp;
+        }
+        const cplx<double> hjj = H[(size_t)j * m + j], hs = H[(size_t)(j + 1) * m + j];
+        const double den = sqrt(norm2(hjj) + norm2(hs));
+        sn[j] = mk<double>(hs.x / den, hs.y / den);
+        cn[j] = mk<double>(hjj.x / den, hjj.y / den);
+        H[(size_t)j * m + j] = cmul(cn[j], hjj) + cmul(sn[j], hs);
+        H[(size_t)(j + 1) * m + j] = czero<double>();
+        g[j + 1] = czero<double>() - sn[j] * g[j];
+        g[j] = cmul(cn[j], g[j]);
+        if (sqrt(norm2(g[j + 1])) < tol * v.nphi[q]) {
+            v.maxit[q] = j + 1;
+            stop = true;
+        }
+    }
+    const uint64_t st = __ballot(stop);
+    if (q == 0) v.h->run &= ~st;
+}
+
+// back substitution (src/fgmres.cpp:118-127) of every live system: eta [q*m + i], nit [q]
+__global__ void k_kc_backsub(KView v, cplx<double>* eta, int* nit, int cap) {
+    const int q = threadIdx.x, m = v.m;
+    if (q >= cap) return;
+    if (!((v.h->live >> q) & 1ull)) {
+        nit[q] = 0;
+        return;
+    }
+    const int nI = v.maxit[q];
+    const cplx<double>* H = v.Hr + (size_t)q * (m + 1) * m;
+    const cplx<double>* g = v.g + (size_t)q * (m + 1);
+    cplx<double>* e = eta + (size_t)q * m;
+    for (int i = nI - 1; i >= 0; i--) {
+        cplx<double> t = g[i];
+        for (int jj = i + 1; jj < nI; jj++) t = t - H[(size_t)i * m + jj] * e[jj];
+        e[i] = cdiv(t, H[(size_t)i * m + i]);
+    }
+    nit[q] = nI;
+}
+
+// cycle end: the true residual norm of every live system; converged ones are done with
+// k*m + maxIt iterations
+__global__ void k_kc_end(KView v, const cplx<double>* nrm, int k, double tol) {
+    const int q = threadIdx.x;
+    bool dn = false;
+    if ((v.h->live >> q) & 1ull) {
+        v.err[q] = sqrt(nrm[4 * q + 1].x);
+        if (v.err[q] < tol * v.nphi[q]) {
+            v.iters[q] = k * v.m + v.maxit[q];
+            dn = true;
+        }
+    }
+    const uint64_t d = __ballot(dn);
+    if (q == 0) v.h->done |= d;
+}
+
+size_t kview_bytes(int m) { return KView::bytes(64, m); }
+
+template <typename R>
+void Krylov<R>::solve_dc(const cplx<R>* phi, cplx<R>* x, uint64_t mask, const Func& A, const Prec* M) {
+    const int nr = mask_count(mask);
+    if (nr > cap_) throw BadShape("Krylov: batch larger than capacity");
+    const size_t n = n_, vs = n_ * cap_;
+    if (!ctl_.bytes()) throw BadShape("Krylov: device control state not allocated (alloc_ctl)");
+    const KView kv = KView::at(ctl_.as<void>(), 64, m_max_);
+    KView v = kv;
+    v.m = m_;
+    cplx<R>* V = V_.as<cplx<R>>();
+    cplx<R>* Z = flex_ ? Z_.as<cplx<R>>() : V;
+    cplx<R>* r = r_.as<cplx<R>>();
+    cplx<R>* w = w_.as<cplx<R>>();
+    cplx<double>* nrm = nrm_.as<cplx<double>>();
+    const uint64_t* parent = dev_mask();   // the systems the caller runs (nullptr: all of mask)
+    // setZeros (include/fgmres.h:116-146), x0 = phi, r = phi - A x0, ||phi||, ||r||
+    for (int i = 0; i <= m_; i++) zero<R>(s_, V + i * vs, n, n, nr, mask);
+    if (flex_)
+        for (int i = 0; i < m_; i++) zero<R>(s_, Z + i * vs, n, n, nr, mask);
+    copy<R>(s_, x, phi, n, n, n, nr, mask);
+    A(x, w, mask);
+    axpy<R>(s_, r, phi, w, -1.0, n, n, nr, mask);
+    dot_partial<R>(s_, phi, phi, n, n, nr, mask, rb_);
+    dot_final<R>(s_, rb_, nrm + 0, 4, nr, mask);
+    dot_partial<R>(s_, r, r, n, n, nr, mask, rb_);
+    dot_final<R>(s_, rb_, nrm + 1, 4, nr, mask);
+    k_kc_init<<<1, 64, 0, s_>>>(v, nrm, mask, parent, restarts_);
+    DD_HIP(hipGetLastError());
+    for (int k = 0; k < restarts_; k++) {
+        k_kc_start<<<1, 64, 0, s_>>>(v);
+        DD_HIP(hipGetLastError());
+        {   // V_0 = r / ||r|| from the partial sums of the last residual norm
+            DevMaskScope sc(&v.h->live);
+            norm_scale<R>(s_, V, r, n, n, nr, mask, rb_, nrm + 2, 4);
+        }
+        for (int j = 0; j < m_; j++) {
+            cplx<double>* Hj = H_.as<cplx<double>>() + (size_t)j * cap_ * (m_ + 1);
+            {
+                DevMaskScope sc(&v.h->run);
+                cplx<R>* Vj = V + j * vs;
+                cplx<R>* Zj = Z + j * vs;
+                if (flex_) (*M)(Vj, Zj, mask);
+                A(Zj, w, mask);
+                ortho(j, w, mask, nr);
+                dot_partial<R>(s_, w, w, n, n, nr, mask, rb_);
+                norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, mask, rb_, Hj + (j + 1), m_ + 1);
+            }
+            k_kc_givens<<<1, 64, 0, s_>>>(v, Hj, m_ + 1, j, tol_);
+            DD_HIP(hipGetLastError());
+        }
+        k_kc_backsub<<<1, 64, 0, s_>>>(v, eta_.as<cplx<double>>(), nit_.as<int>(), cap_);
+        DD_HIP(hipGetLastError());
+        {
+            DevMaskScope sc(&v.h->live);
+            update_x<R>(s_, x, Z, vs, eta_.as<cplx<double>>(), nit_.as<int>(), m_, n, n, nr, mask);
+            A(x, w, mask);
+            axpy<R>(s_, r, phi, w, -1.0, n, n, nr, mask);
+            dot_partial<R>(s_, r, r, n, n, nr, mask, rb_);
+            dot_final<R>(s_, rb_, nrm + 1, 4, nr, mask);
+        }
+        k_kc_end<<<1, 64, 0, s_>>>(v, nrm, k, tol_);
+        DD_HIP(hipGetLastError());
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Geometry and test-vector streams
 // ---------------------------------------------------------------------------------------
@@ -327,6 +545,7 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
             if (sizeof(R) == 4 && p.coarse_sap_links == 1 && l < L - 1 && sapc_supported<R>(32, G.dof, G.bx * G.bt))
                 hlinks_[l].alloc(sizeof(_Float16) * 2 * (size_t)G.V() * 5 * G.dof * G.dof);
             kry_[l].reset(new Krylov<R>(s, G.n(), p.kcycle_m, p.kcycle_restarts, p.kcycle_tol, cap, true, p.ortho_inner));
+            if (p.graph) kry_[l]->alloc_ctl();   // device-controlled solves of captured K-cycles
         }
         if (l < L - 1) W_[l].alloc(sizeof(cplx<R>) * (size_t)G.V() * G.nv * G.dof);
     }
@@ -400,8 +619,7 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
     int* gate = gate_[l].as<int>();
     cplx<double>* nrm = nrm_[l].as<cplx<double>>();
     const size_t n = G.n();
-    k_fill_int<<<1, 64, 0, s_>>>(gate, 1, 64);
-    DD_HIP(hipGetLastError());
+    gate_init(s_, gate);   // open for the systems of the mask (and of the device mask, if any)
     dot_partial<R>(s_, v, v, n, n, nr, mask, rb_[l]);
     dot_final<R>(s_, rb_[l], nrm + 0, 2, nr, mask);
     // one launch per iteration where the level allows it (not inside a graph capture: the
@@ -486,8 +704,22 @@ Multigrid<R>::~Multigrid() {
 
 template <typename R>
 bool Multigrid<R>::vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    return capture_(eta, psi, mask, false);
+}
+
+template <typename R>
+bool Multigrid<R>::kcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+    return capture_(eta, psi, mask, true);
+}
+
+// psi = cycle(eta) from level 0 with zero initial guess, replayed from a HIP graph that is
+// captured on the first call for these buffers, this mask and this cycle.  A K-cycle is
+// captured with its FGMRES solves device-controlled (every step issued, the systems masked
+// on the device), so that the graph holds no host decision.
+template <typename R>
+bool Multigrid<R>::capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool kcyc) {
     if (!graphs_ok_ || !use_dev_ || timer_any()) return false;
-    const auto key = std::make_tuple(static_cast<const void*>(eta), static_cast<const void*>(psi), mask);
+    const auto key = std::make_tuple(static_cast<const void*>(eta), static_cast<const void*>(psi), mask, kcyc);
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
         const size_t n = g_[0].n();
@@ -495,14 +727,17 @@ bool Multigrid<R>::vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask)
         hipGraphExec_t ex = nullptr;
         DD_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
         capturing_ = true;
+        dc_ = kcyc;
         bool ok = true;
         try {
             zero<R>(s_, psi, n, n, mask_count(mask), mask);
-            vcycle(0, eta, psi, mask);
+            if (kcyc) kcycle(0, eta, psi, mask);
+            else vcycle(0, eta, psi, mask);
         } catch (...) {
             ok = false;
         }
         capturing_ = false;
+        dc_ = false;
         const hipError_t e = hipStreamEndCapture(s_, &g);
         ok = ok && e == hipSuccess && g != nullptr && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
         if (g) (void)hipGraphDestroy(g);
@@ -510,7 +745,7 @@ bool Multigrid<R>::vcycle_graph(const cplx<R>* eta, cplx<R>* psi, uint64_t mask)
             (void)hipGetLastError();
             if (ex) (void)hipGraphExecDestroy(ex);
             graphs_ok_ = false;
-            if (std::getenv("DDAMG_VERBOSE")) fprintf(stderr, "V-cycle graph capture refused; eager cycles\n");
+            if (std::getenv("DDAMG_VERBOSE")) fprintf(stderr, "cycle graph capture refused; eager cycles\n");
             return false;
         }
         it = graphs_.emplace(key, ex).first;
@@ -553,8 +788,12 @@ void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
         apply(l + 1, in, out, nullptr, m);
     };
     typename Krylov<R>::Prec fM = [this, l](const cplx<R>* in, cplx<R>* io, uint64_t m) { kcycle(l + 1, in, io, m); };
-    std::vector<int> its(64);
-    kry_[l + 1]->solve(e1, e1, p1, mask, fA, &fM, its.data(), nullptr);
+    if (dc_) {
+        kry_[l + 1]->solve_dc(e1, p1, mask, fA, &fM);
+    } else {
+        std::vector<int> its(64);
+        kry_[l + 1]->solve(e1, e1, p1, mask, fA, &fM, its.data(), nullptr);
+    }
     prolong(l, p1, psi, psi, mask);
     if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
 }
@@ -954,7 +1193,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
         if (prm.precision == 1) {
             x32_.alloc(sizeof(cplx<float>) * n * cap);
             y32_.alloc(sizeof(cplx<float>) * n * cap);
-        } else if (prm.graph && prm.cycle == 0) {   // fixed buffers for the captured cycle
+        } else if (prm.graph) {   // fixed buffers for the captured cycle
             x64_.alloc(sizeof(cplx<double>) * n * cap);
             y64_.alloc(sizeof(cplx<double>) * n * cap);
         }
@@ -964,17 +1203,18 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
     Krylov<double>::Func A = [this, n](const cplx<double>* in, cplx<double>* out, uint64_t m) {
         wilson<double>(s, U64.as<cplx<double>>(), in, out, nullptr, Nx, Nt, prm.m0, false, mask_count(m), m, n);
     };
-    // V-cycles (graph = 1): the cycle on fixed buffers is captured once per mask and
-    // replayed; the copies in and out stay outside the graph
+    // graph = 1: the cycle on fixed buffers is captured once per mask and replayed; the
+    // copies in and out stay outside the graph
     Krylov<double>::Prec M = [this, n, &graphs_used](const cplx<double>* in, cplx<double>* io, uint64_t m) {
         const int nr = mask_count(m);
-        const bool graph = prm.graph && prm.cycle == 0;
+        const bool graph = prm.graph != 0;
+        const bool kc = prm.cycle == 1;
         if (prm.precision == 0) {
             if (graph) {
                 cplx<double>* a = x64_.as<cplx<double>>();
                 cplx<double>* y = y64_.as<cplx<double>>();
                 copy<double>(s, a, in, n, n, n, nr, m);
-                if (mg64->vcycle_graph(a, y, m)) {
+                if (kc ? mg64->kcycle_graph(a, y, m) : mg64->vcycle_graph(a, y, m)) {
                     graphs_used = true;
                     copy<double>(s, io, y, n, n, n, nr, m);
                     return;
@@ -987,7 +1227,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             cplx<float>* a = x32_.as<cplx<float>>();
             cplx<float>* y = y32_.as<cplx<float>>();
             convert<float, double>(s, a, in, n, n, n, nr, m);
-            if (graph && mg32->vcycle_graph(a, y, m)) {
+            if (graph && (kc ? mg32->kcycle_graph(a, y, m) : mg32->vcycle_graph(a, y, m))) {
                 graphs_used = true;
             } else {
                 zero<float>(s, y, n, n, nr, m);
