@@ -15,6 +15,7 @@
   U[-0.18, -0.10] and 4 right-hand sides each, mixed precision, solved at the same time
   (dd_solve_many_dev): every residual < 1e-10, and each lattice's result equals its own
   solve of the same right-hand sides to 1e-10.
+- D_W and D^dagger at 512^2 for a 32-system batch against the oracle to 1e-14 (FP64).
 
 Fields come from the library's device Metropolis generator (beta=2, cold start, 2000
 sweeps of +-0.5 rad); the fixtures hold the SHA-256 of the field the reference solved,
@@ -167,3 +168,26 @@ def test_ragged_batch_of_eight_fields():
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_dirac_full_size(golden):
+    """D_W and D^dagger at 512^2 on the fixture's field for 32 systems, the shape of the
+    benchmark's batch (XCD-ordered work-groups, 4 systems per thread): each checked system
+    equals the oracle's D_phi / D_dagger_phi (src/dirac_operator.cpp:24-85) to 1e-14
+    relative in FP64 and 1e-6 in FP32, and batched systems equal single ones bit for bit."""
+    meta, _ = golden('g512_solve')
+    N = meta['N']
+    ctx, U = field_ctx(meta)
+    V = gaussian(N, 77, nrhs=32)
+    with ctx:
+        D64 = ctx.apply_D(V, meta['m0'])
+        Dd = ctx.apply_D(V, meta['m0'], dagger=True)
+        D32 = ctx.apply_D(V, meta['m0'], precision=1)
+        one = ctx.apply_D(V[13], meta['m0'])
+    assert np.array_equal(one, D64[13])
+    for r in (0, 13, 31):
+        ref = orc.D_phi(U, V[r], meta['m0'])
+        refd = orc.D_dagger_phi(U, V[r], meta['m0'])
+        e64, ed, e32 = rel(D64[r], ref), rel(Dd[r], refd), rel(D32[r], ref)
+        print(f'512^2 system {r}: D {e64:.2e}, D^dagger {ed:.2e}, FP32 D {e32:.2e}')
+        assert e64 < 1e-14 and ed < 1e-14 and e32 < 1e-6
