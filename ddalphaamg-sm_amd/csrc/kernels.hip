@@ -277,29 +277,50 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
 // ======================================================================================
 // Transfers
 // ======================================================================================
-template <typename R>
-__global__ void k_restrict(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ v, cplx<R>* __restrict__ out,
-                           int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf, size_t ldc, const uint64_t* __restrict__ dm) {
+// The transfers serve RB systems per thread: every test-vector element is loaded once for
+// the group (the test vectors are the dominant operand), and the work-groups are numbered
+// XCD by XCD as in k_wilson.
+constexpr int kTransferRB = 8;
+
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void k_restrict(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ v,
+                                                  cplx<R>* __restrict__ out, int Nx, int Nt, int dof, int ax, int at,
+                                                  int nv, uint64_t mask, size_t ldf, size_t ldc, int nrhs, int nsb,
+                                                  const uint64_t* __restrict__ dm) {
     if (dm) mask &= *dm;
-    const int r = blockIdx.y;
-    if (!rhs_on(mask, r)) return;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int sb = L % nsb, r0 = (L / nsb) * RB;
     const int Ntc = Nt / at, Vc = (Nx / ax) * Ntc;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = sb * blockDim.x + threadIdx.x;
     if (idx >= Vc * 2 * nv) return;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < RB; k++) any |= r0 + k < nrhs && rhs_on(mask, r0 + k);
+    if (!any) return;
     const int Xc = idx / (2 * nv), q = idx - Xc * 2 * nv, p = q >> 1, sp = q & 1;
     const int X = Xc / Ntc, Tc = Xc - X * Ntc;
     const int col = dof / 2;
-    const cplx<R>* vr = v + r * ldf;
-    cplx<R> sum = czero<R>();
+    cplx<R> sum[RB];
+#pragma unroll
+    for (int k = 0; k < RB; k++) sum[k] = czero<R>();
     for (int lx = 0; lx < ax; lx++)
         for (int lt = 0; lt < at; lt++) {
             const size_t n = (size_t)(X * ax + lx) * Nt + Tc * at + lt;
             for (int c = 0; c < col; c++) {
                 const int d = c * 2 + sp;
-                sum = sum + cmul(W[(n * nv + p) * dof + d], vr[n * dof + d]);
+                const cplx<R> w = W[(n * nv + p) * dof + d];
+#pragma unroll
+                for (int k = 0; k < RB; k++) {
+                    const int r = min(r0 + k, nrhs - 1);
+                    sum[k] = sum[k] + cmul(w, v[(size_t)r * ldf + n * dof + d]);
+                }
             }
         }
-    out[r * ldc + (size_t)Xc * 2 * nv + q] = sum;
+#pragma unroll
+    for (int k = 0; k < RB; k++) {
+        const int r = r0 + k;
+        if (r < nrhs && rhs_on(mask, r)) out[(size_t)r * ldc + (size_t)Xc * 2 * nv + q] = sum[k];
+    }
 }
 
 template <typename R>
@@ -307,38 +328,60 @@ void restrict_(hipStream_t s, const cplx<R>* W, const cplx<R>* v, cplx<R>* out, 
                int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
     timer_begin(s, K_RESTRICT);
     const size_t tot = (size_t)(Nx / ax) * (Nt / at) * 2 * nv;
-    dim3 grid(cdivu(tot, 256), nrhs);
-    k_restrict<R><<<grid, 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc, dev_mask());
+    const int nsb = (int)cdivu(tot, 256), ng = (nrhs + kTransferRB - 1) / kTransferRB;
+    k_restrict<R, kTransferRB><<<dim3((unsigned)nsb * ng), 256, 0, s>>>(W, v, out, Nx, Nt, dof, ax, at, nv, mask, ldf,
+                                                                        ldc, nrhs, nsb, dev_mask());
     timer_end(s, K_RESTRICT, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
-template <typename R>
-__global__ void k_prolong(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ vc, const cplx<R>* y,
-                          cplx<R>* out, int Nx, int Nt, int dof, int ax, int at, int nv, uint64_t mask, size_t ldf,
-                          size_t ldc, const uint64_t* __restrict__ dm) {
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void k_prolong(const cplx<R>* __restrict__ W, const cplx<R>* __restrict__ vc,
+                                                 const cplx<R>* y, cplx<R>* out, int Nx, int Nt, int dof, int ax,
+                                                 int at, int nv, uint64_t mask, size_t ldf, size_t ldc, int nrhs,
+                                                 int nsb, const uint64_t* __restrict__ dm) {
     if (dm) mask &= *dm;
-    const int r = blockIdx.y;
-    if (!rhs_on(mask, r)) return;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int sb = L % nsb, r0 = (L / nsb) * RB;
+    const size_t idx = (size_t)sb * blockDim.x + threadIdx.x;
     if (idx >= (size_t)Nx * Nt * dof) return;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < RB; k++) any |= r0 + k < nrhs && rhs_on(mask, r0 + k);
+    if (!any) return;
     const int n = (int)(idx / dof), d = (int)(idx - (size_t)n * dof), sp = d & 1;
     const int x = n / Nt, t = n - x * Nt;
     const int Xc = (x / ax) * (Nt / at) + t / at;
-    const cplx<R>* vcr = vc + r * ldc + (size_t)Xc * 2 * nv + sp;
     const cplx<R>* Wn = W + (size_t)n * nv * dof + d;
-    cplx<R> sum = czero<R>();
-    for (int p = 0; p < nv; p++) sum = sum + Wn[p * dof] * vcr[2 * p];
-    if (y) sum = y[r * ldf + idx] + sum;
-    out[r * ldf + idx] = sum;
+    cplx<R> sum[RB];
+#pragma unroll
+    for (int k = 0; k < RB; k++) sum[k] = czero<R>();
+    for (int p = 0; p < nv; p++) {
+        const cplx<R> w = Wn[p * dof];
+#pragma unroll
+        for (int k = 0; k < RB; k++) {
+            const int r = min(r0 + k, nrhs - 1);
+            sum[k] = sum[k] + w * vc[(size_t)r * ldc + (size_t)Xc * 2 * nv + sp + 2 * p];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RB; k++) {
+        const int r = r0 + k;
+        if (r < nrhs && rhs_on(mask, r)) {
+            cplx<R> o = sum[k];
+            if (y) o = y[(size_t)r * ldf + idx] + o;
+            out[(size_t)r * ldf + idx] = o;
+        }
+    }
 }
 
 template <typename R>
 void prolong(hipStream_t s, const cplx<R>* W, const cplx<R>* vc, const cplx<R>* y, cplx<R>* out, int Nx, int Nt,
              int dof, int ax, int at, int nv, int nrhs, uint64_t mask, size_t ldf, size_t ldc) {
     timer_begin(s, K_PROLONG);
-    dim3 grid(cdivu((size_t)Nx * Nt * dof, 256), nrhs);
-    k_prolong<R><<<grid, 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask, ldf, ldc, dev_mask());
+    const int nsb = (int)cdivu((size_t)Nx * Nt * dof, 256), ng = (nrhs + kTransferRB - 1) / kTransferRB;
+    k_prolong<R, kTransferRB><<<dim3((unsigned)nsb * ng), 256, 0, s>>>(W, vc, y, out, Nx, Nt, dof, ax, at, nv, mask,
+                                                                       ldf, ldc, nrhs, nsb, dev_mask());
     timer_end(s, K_PROLONG, (double)Nx * Nt * dof * sizeof(cplx<R>) * (nv + (y ? 2 : 1) * nact(mask, nrhs)) + nact(mask, nrhs) * (double)(Nx / ax) * (Nt / at) * 2 * nv * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
