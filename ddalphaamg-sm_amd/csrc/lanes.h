@@ -94,6 +94,35 @@ __device__ __forceinline__ void pair_swap(double& A, double& B, int lane) {
     B = __builtin_bit_cast(double, b);
 }
 
+// value of lane i + D (D > 0: row_shl) or i - |D| (D < 0: row_shr) within the lane's
+// 16-lane row; 0 where that lane is outside the row
+template <int D>
+__device__ __forceinline__ float row_shift(float v) {
+    static_assert(D != 0 && D > -16 && D < 16, "row shift");
+    constexpr int ctrl = D > 0 ? 0x100 + D : 0x110 - D;
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xf, 0xf, true));
+}
+template <int D>
+__device__ __forceinline__ double row_shift(double v) {
+    static_assert(D != 0 && D > -16 && D < 16, "row shift");
+    constexpr int ctrl = D > 0 ? 0x100 + D : 0x110 - D;
+    lanes_detail::W2 w = __builtin_bit_cast(lanes_detail::W2, v);
+    w.lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)w.lo, ctrl, 0xf, 0xf, true);
+    w.hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)w.hi, ctrl, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, w);
+}
+
+// sum over each group of 16 lanes, partners at lane offsets 8, 4, 2, 1 in that order (the
+// order of wave_sum(v, 16)); every lane of the group gets the same bits
+template <typename R>
+__device__ __forceinline__ R seg16_sum(R v, int lane) {
+    v += lane_xor<8>(v, lane);
+    v += lane_xor<4>(v, lane);
+    v += lane_xor<2>(v, lane);
+    v += lane_xor<1>(v, lane);
+    return v;
+}
+
 // v + (v of lane ^ o) for a runtime offset o in {1, ..., 32}
 template <typename R>
 __device__ __forceinline__ R add_xor(R v, int o, int lane) {

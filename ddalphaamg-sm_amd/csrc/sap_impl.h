@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "lanes.h"
 
 namespace dd {
 
@@ -143,7 +144,20 @@ __device__ __forceinline__ cplx<R> shfl_c(cplx<R> v, int src) {
     return mk<R>(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
 }
 
-template <typename R, int M>
+template <typename R>
+__device__ __forceinline__ cplx<R> row_shift_c(cplx<R> v, int d) {   // d: +1, -1, +4, -4
+    switch (d) {
+        case 1: return mk<R>(row_shift<1>(v.x), row_shift<1>(v.y));
+        case -1: return mk<R>(row_shift<-1>(v.x), row_shift<-1>(v.y));
+        case 4: return mk<R>(row_shift<4>(v.x), row_shift<4>(v.y));
+        default: return mk<R>(row_shift<-4>(v.x), row_shift<-4>(v.y));
+    }
+}
+
+// B44: blocks of 4 x 4 sites, one per 16-lane row: the neighbours inside the block are DPP
+// row shifts by 1 (t) and 4 (x) and the block sums a DPP butterfly over the row, in the
+// order of wave_sum (the same bits); otherwise LDS permutes for any block shape.
+template <typename R, int M, bool B44>
 __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ rr,
                                              cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt, int colour,
                                              R m0, int restarts, R tol, uint64_t mask, size_t ld,
@@ -173,10 +187,26 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
     (void)base;
     // D_B restricted to the block, zero outside, no boundary sign
     auto A = [&](const Spin2<R>& v) -> Spin2<R> {
-        cplx<R> tp0 = shfl_c(v.a, sTp), tp1 = shfl_c(v.b, sTp);
-        cplx<R> xp0 = shfl_c(v.a, sXp), xp1 = shfl_c(v.b, sXp);
-        cplx<R> tm0 = shfl_c(v.a, sTm), tm1 = shfl_c(v.b, sTm);
-        cplx<R> xm0 = shfl_c(v.a, sXm), xm1 = shfl_c(v.b, sXm);
+        cplx<R> tp0, tp1, xp0, xp1, tm0, tm1, xm0, xm1;
+        if constexpr (B44) {
+            tp0 = row_shift_c(v.a, 1);
+            tp1 = row_shift_c(v.b, 1);
+            xp0 = row_shift_c(v.a, 4);
+            xp1 = row_shift_c(v.b, 4);
+            tm0 = row_shift_c(v.a, -1);
+            tm1 = row_shift_c(v.b, -1);
+            xm0 = row_shift_c(v.a, -4);
+            xm1 = row_shift_c(v.b, -4);
+        } else {
+            tp0 = shfl_c(v.a, sTp);
+            tp1 = shfl_c(v.b, sTp);
+            xp0 = shfl_c(v.a, sXp);
+            xp1 = shfl_c(v.b, sXp);
+            tm0 = shfl_c(v.a, sTm);
+            tm1 = shfl_c(v.b, sTm);
+            xm0 = shfl_c(v.a, sXm);
+            xm1 = shfl_c(v.b, sXm);
+        }
         if (!hasTp) tp0 = tp1 = czero<R>();
         if (!hasXp) xp0 = xp1 = czero<R>();
         if (!hasTm) tm0 = tm1 = czero<R>();
@@ -187,7 +217,10 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
             U0 * (tp1 - tp0) + U1 * (xp1 - times_i(xp0)) + U0m * (tm0 + tm1) + U1m * (times_i(xm0) + xm1);
         return {mass * v.a - R(0.5) * a0, mass * v.b - R(0.5) * a1};
     };
-    auto red = [&](cplx<R> v) -> cplx<R> { return wave_sum(v, S); };
+    auto red = [&](cplx<R> v) -> cplx<R> {
+        if constexpr (B44) return mk<R>(seg16_sum(v.x, lane), seg16_sum(v.y, lane));
+        else return wave_sum(v, S);
+    };
     const cplx<R>* rv = rr + rh * ld;
     const Spin2<R> phi = {rv[2 * n], rv[2 * n + 1]};
     const Spin2<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
@@ -206,8 +239,12 @@ void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, 
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
     const int nblk = (Nx / bx) * (Nt / bt) / 2;
     dim3 grid(cdivu(nblk, 64 / S), nrhs);
-    k_sap0<R, kSapM><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld, gate,
-                                         nblk);
+    if (bx == 4 && bt == 4)
+        k_sap0<R, kSapM, true><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld,
+                                                   gate, nblk);
+    else
+        k_sap0<R, kSapM, false><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld,
+                                                    gate, nblk);
     timer_end(s, K_SAP0, 0.5 * Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + 3.0 * nact(mask, nrhs)));
     DD_LAUNCH_CHECK();
 }
