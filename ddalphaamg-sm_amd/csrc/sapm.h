@@ -26,6 +26,7 @@
 //     most 2^-11 relative; it serves only the smoother's block solves (the level operator
 //     and the Galerkin products keep the FP32 links).
 #pragma once
+#include "lanes.h"
 #include "mfma.h"
 
 namespace dd {
@@ -198,10 +199,10 @@ __device__ __forceinline__ void sapm_task(const LT* __restrict__ L, const float*
     // lane of the system)
     int buf = 0;
     auto rsum = [&](cplx<R> p) -> cplx<R> {
-        p.x += __shfl_xor(p.x, 16, 64);
-        p.y += __shfl_xor(p.y, 16, 64);
-        p.x += __shfl_xor(p.x, 32, 64);
-        p.y += __shfl_xor(p.y, 32, 64);
+        p.x += lane_xor<16>(p.x, lane);   // permlane swaps: the lane network, not LDS
+        p.y += lane_xor<16>(p.y, lane);
+        p.x += lane_xor<32>(p.x, lane);
+        p.y += lane_xor<32>(p.y, lane);
         cplx<R>* rb = red + buf * 256;
         if (g == 0) rb[wv * 16 + j] = p;
         __syncthreads();
@@ -529,8 +530,8 @@ __global__ __launch_bounds__(512) void k_sapm_fused(const LT* __restrict__ L, co
     // per system: the four lane groups, then the eight waves in order, then the slices
     for (int k = 0; k < 4; k++) {
         float p = sumsq[k];
-        p += __shfl_xor(p, 16, 64);
-        p += __shfl_xor(p, 32, 64);
+        p += lane_xor<16>(p, lane);
+        p += lane_xor<32>(p, lane);
         if (g == 0) red_s[wv][k * 16 + (lane & 15)] = p;
     }
     __syncthreads();
