@@ -38,6 +38,7 @@ SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-
 BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
 SAP_FUSED = None      # None: the library default; 0 separate launches per colour, 1 one launch per iteration
 GRAPH = 0             # 1: the cycles replayed from captured HIP graphs (--graph)
+COARSE_POLY = None    # None: the library default (0); degree of the coarsest GMRES's polynomial preconditioner
 
 
 def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
@@ -51,6 +52,8 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
     if SAP_FUSED is not None:
         p.sap_fused = SAP_FUSED
     p.graph = GRAPH
+    if COARSE_POLY is not None:
+        p.coarse_poly = COARSE_POLY
     return p
 
 
@@ -86,11 +89,14 @@ def main():
                     help='coarse SAP iterations in one launch (1) or per colour with separate residuals (0)')
     ap.add_argument('--graph', type=int, choices=[0, 1], default=0,
                     help='replay the cycles from captured HIP graphs (K-cycle FGMRES device-controlled)')
+    ap.add_argument('--coarse-poly', type=int, default=None,
+                    help='degree of the GMRES polynomial right preconditioner of the coarsest solve (0: plain GMRES)')
     ap.add_argument('--setup-only', action='store_true',
                     help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
-    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED, GRAPH
+    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED, GRAPH, COARSE_POLY
     BOOTSTRAP = args.bootstrap
+    COARSE_POLY = args.coarse_poly
     GRAPH = args.graph
     SAP_FUSED = args.sap_fused
     COARSE_BASIS = args.coarse_basis

@@ -6,6 +6,7 @@
 #include <string>
 #include <thread>
 
+#include "poly.h"
 #include "solver.h"
 
 // Device scratch of dd_dot_dev_noalloc: the partial sums of up to 64 systems x 256
@@ -199,6 +200,7 @@ int dd_default_params_sized(dd_params* p, int size, int levels) {
     p->bootstrap = 0;
     p->graph = 0;
     p->sap_fused = 1;
+    p->coarse_poly = 0;
     return DD_OK;
 }
 
@@ -655,6 +657,42 @@ int dd_level_shape(dd_ctx* c, int level, int* Nx, int* Nt, int* dof) {
         if (Nx) *Nx = g[level].Nx;
         if (Nt) *Nt = g[level].Nt;
         if (dof) *dof = g[level].dof;
+        return (int)DD_OK;
+    });
+}
+
+int dd_get_coarse_poly(dd_ctx* c, int* degree, double* roots) {
+    if (check(c) || !degree) return DD_ERR_ARG;
+    return guard([&] {
+        if (!c->s->have_setup) throw dd::BadShape("no hierarchy");
+        const std::vector<std::complex<double>>& th =
+            c->s->mg64 ? c->s->mg64->poly_roots() : c->s->mg32->poly_roots();
+        *degree = (int)th.size();
+        if (roots)
+            for (size_t i = 0; i < th.size(); i++) {
+                roots[2 * i] = th[i].real();
+                roots[2 * i + 1] = th[i].imag();
+            }
+        return (int)DD_OK;
+    });
+}
+
+int dd_poly_roots(const double* H, int d, double* roots) {
+    if (!H || !roots || d < 1 || d > 64) return DD_ERR_ARG;
+    return guard([&] {
+        std::vector<std::complex<double>> h((size_t)(d + 1) * d);
+        for (size_t i = 0; i < h.size(); i++) h[i] = {H[2 * i], H[2 * i + 1]};
+        std::vector<std::complex<double>> th;
+        try {
+            th = dd::gmres_poly_roots(h, d);
+        } catch (const std::runtime_error& e) {   // singular H_d, or the QR iteration failed
+            set_err(e.what());
+            return (int)DD_ERR_ARG;
+        }
+        for (int i = 0; i < d; i++) {
+            roots[2 * i] = th[i].real();
+            roots[2 * i + 1] = th[i].imag();
+        }
         return (int)DD_OK;
     });
 }

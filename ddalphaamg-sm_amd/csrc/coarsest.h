@@ -9,6 +9,7 @@
 // passes over the basis per step); the Hessenberg matrix, Givens rotations and the
 // triangular solve run on the device, so one launch performs the whole solve.
 #pragma once
+#include <complex>
 #include <vector>
 
 #include "devmem.h"
@@ -25,6 +26,10 @@ class CoarsestGmres {
     // false when the lattice is too large for one co-resident grid (the caller then
     // uses the launch-per-step Krylov engine)
     bool supported() const { return ok_; }
+    // GMRES polynomial right preconditioner p(D) = (1 - prod_i (1 - D / theta_i)) / D of
+    // degree theta.size() (at most 32; empty: none): later solves run the Arnoldi process
+    // on D p(D) and return x0 + p(D) Q y.  The roots are applied in the given order.
+    void set_poly(const std::vector<std::complex<double>>& theta);
     // psi = D_c^-1 eta for the systems in mask (x0 = eta); iters[r] per system
     void solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, int* iters);
     // deferred: solve() only enqueues device work (no host read-back, no stream
@@ -55,6 +60,8 @@ class CoarsestGmres {
     bool half_ = false, ok_ = false;
     DevMem Q_, u_, w_, part_, Hrot_, etals_, coef_, nrm_, ctrl_, bar_, wgclk_;
     DevMem sticky_;   // 1 once a launch's fail flag was set (kept across launches)
+    DevMem pc_, z_;   // polynomial preconditioner: 1/theta_i, two stage vectors per system
+    int npoly_ = 0;
     long long solves_ = 0;
     PinMem ctrlh_;
 };

@@ -72,6 +72,7 @@ struct Ctrl {
 // kernel ends; the host reports it.  All fields are zero when a launch starts.
 constexpr unsigned kBarSpins = 1u << 25;
 constexpr int kFlagStride = 16;   // words between two flags (each flag in its own 64-byte line)
+constexpr int kPolyMax = 32;      // degree of the polynomial preconditioner at most
 
 // Each word that work-groups update or poll sits in its own 256-byte region (one memory
 // channel's interleave unit), so that the barrier's atomics, the polling of its generation
@@ -92,6 +93,9 @@ struct GridBar {
     // per work-group: the last Arnoldi step whose partial sums it has written (release),
     // read by the owners of the running systems (see phaseB)
     unsigned pdone[kGmax * kFlagStride];
+    // per work-group: the number of polynomial-preconditioner stages whose rows it has
+    // written (release), read by the work-groups whose stencil needs them (poly_stages)
+    unsigned zdone[kGmax * kFlagStride];
 };
 
 // Executed by every wave that stored data before the work-group barrier that precedes a
@@ -168,6 +172,28 @@ __device__ __forceinline__ void cmadd(cplx<R>& acc, cplx<R> c, cplx<R> q) {
     acc.y = fma(c.x, q.y, fma(c.y, q.x, acc.y));
 }
 
+// Epilogue of one stage of the polynomial preconditioner's product form: z = v - c D v
+// (c = 1/theta_i, v the stage's input row), stored as out = z; for the last stage of an
+// Arnoldi step (extra = u_j) as out = u_j - z, which is D p(D) u_j; with xacc also
+// xacc += cx z (the solution's accumulation at the end of a cycle).
+template <typename R>
+struct PolyEpi {
+    const cplx<R>* extra;
+    cplx<R>* xacc;
+    cplx<R> c, cx;
+    __device__ __forceinline__ void operator()(cplx<R>* __restrict__ out, size_t i, cplx<R> v, cplx<R> Dv) const {
+        cplx<R> z = v;
+        cmsub(z, c, Dv);
+        if (extra) z = extra[i] - z;
+        out[i] = z;
+        if (xacc) {
+            cplx<R> xv = xacc[i];
+            cmadd(xv, cx, z);
+            xacc[i] = xv;
+        }
+    }
+};
+
 template <typename R, typename S>
 struct CArgs {
     const cplx<R>* L;       // links of the coarsest level
@@ -194,6 +220,12 @@ struct CArgs {
     int lds_links;          // 1: the slice's links are cached in LDS (after the scalar state)
     R m0;
     double tol;
+    // GMRES polynomial right preconditioner p(D) of degree npoly (0: none): the Arnoldi
+    // process runs on D p(D) and the solution is x0 + p(D) Q y
+    int npoly;
+    const cplx<R>* pc;      // [npoly] 1/theta_i of its roots in order of application
+    cplx<R>* z;             // [2][cap][n]: the stages of its product form (ping-pong),
+    size_t zs;              // the second buffer at z + zs
 };
 
 // LDS of the kernel: the scalar state of the owned system and scratch, then (lds_links)
@@ -212,10 +244,12 @@ __host__ __device__ constexpr size_t coarsest_lds(int m) {
 // element (row m = 2r + p, k = 2c + q) is Lr for p == q, Li for p != q, negated for
 // (p, q) = (0, 1); with c = (KS/2) g + s/2 and q = s & 1 its LDS offset is a lane part
 // plus a constant per (s, t).  FP32 only (the accumulator rows m - 16t = i of mfma.h).
-template <typename R, int DOF, bool RES>
+// Without RES the result goes through epi(out, row index, own input value, result).
+template <typename R, int DOF, bool RES, typename Epi = StoreEpi<R>>
 __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out,
                                                 const cplx<R>* __restrict__ b, int Nx, int Nt, R m0, uint64_t mask,
-                                                size_t ld, int nrhs, int n, int tile, int lane) {
+                                                size_t ld, int nrhs, int n, int tile, int lane,
+                                                const Epi& epi = Epi()) {
     using acc_t = typename Mfma<R>::acc_t;
     constexpr int MT = DOF / 8, KS = DOF / 2;
     const int g = lane >> 4;
@@ -282,14 +316,15 @@ __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __re
 #pragma unroll
     for (int tt = 0; tt < MT; tt++) {
         const int r0 = 8 * tt + 2 * g;
-        cplx<R> o0 = mk<R>(mass * own[tt][0].x - acc[0][tt][0], mass * own[tt][0].y - acc[0][tt][1]);
-        cplx<R> o1 = mk<R>(mass * own[tt][1].x - acc[0][tt][2], mass * own[tt][1].y - acc[0][tt][3]);
-        if (RES) {
-            o0 = rhs[tt][0] - o0;
-            o1 = rhs[tt][1] - o1;
+        const cplx<R> o0 = mk<R>(mass * own[tt][0].x - acc[0][tt][0], mass * own[tt][0].y - acc[0][tt][1]);
+        const cplx<R> o1 = mk<R>(mass * own[tt][1].x - acc[0][tt][2], mass * own[tt][1].y - acc[0][tt][3]);
+        if constexpr (RES) {
+            out[base + r0] = rhs[tt][0] - o0;
+            out[base + r0 + 1] = rhs[tt][1] - o1;
+        } else {
+            epi(out, base + r0, own[tt][0], o0);
+            epi(out, base + r0 + 1, own[tt][1], o1);
         }
-        out[base + r0] = o0;
-        out[base + r0 + 1] = o1;
     }
 }
 
@@ -343,9 +378,17 @@ struct Chunk {
 // Lanes and rows: a system's rows of the slice (spw sites of DOF rows) are spread over a
 // group of LPS lanes, RPL = DOF / 8 consecutive rows per lane, so LPS = 32 holds up to 4
 // sites and LPS = 64 up to 8; with LPS = 32 a wave works on two systems at once.
-template <typename R, typename S, int DOF, int LPS>
+// POLY: the variant with the polynomial preconditioner (a.npoly > 0); the plain one
+// compiles without its code.
+template <typename R, typename S, int DOF, int LPS, bool POLY>
 __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     constexpr int RPL = DOF / 8, SPI = 64 / LPS;
+    const int npoly = POLY ? a.npoly : 0;
+    // the stage buffers by address arithmetic (a select between two pointers compiles to
+    // a table in scratch memory)
+    cplx<R>* const zb0 = a.z;
+    const size_t zstride = a.zs;
+    auto zbuf = [zb0, zstride](int i) { return zb0 + (size_t)(i & 1) * zstride; };
     static_assert(DOF % 8 == 0, "rows per lane must divide the site's rows");
     // kCH basis vectors loaded per pass step, reduced in groups of kGV (NVAL sums each)
     constexpr int kCH = Chunk<S, RPL>::RED, kCU = Chunk<S, RPL>::UPD, kGV = kCH < 8 ? kCH : 8, NVAL = 4 * kGV,
@@ -409,6 +452,8 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     auto pidx = [&](int q, int v, int w) -> size_t { return ((size_t)w * a.nsys + q) * VALS + v; };
 
     const double links_bytes = (double)V * 5 * DOF * DOF * sR;
+    // links read by one stencil sweep: none once they sit in LDS (loaded once per launch)
+    const double sweep_links = a.lds_links ? 0.0 : links_bytes;
     auto count = [&](double bytes, double steps) {
         if (wg == 0 && tid == 0) {
             a.stats[0] += bytes;
@@ -494,6 +539,37 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                         coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site,
                                                       tl, lane);
                 }
+            }
+    };
+    // the same sweep with the result going through an epilogue (no residual form)
+    auto stencil_e = [&](const cplx<R>* in, cplx<R>* out, uint64_t mk2, const auto& epi) {
+        if (LPS == 32) {
+            const int ns = s1 - s0;
+            for (int u = wv; u < ns * ntile; u += kNW) {
+                const int site = s0 + u % ns, tl = u / ns;
+                if (!((mk2 >> (16 * tl)) & 0xffffull)) continue;
+                if constexpr (sizeof(R) == 4 && DOF <= 24) {
+                    if (a.lds_links) {
+                        const R* Ls = lnk + (size_t)(site - s0) * 5 * DOF * DOF * 2;
+                        coarse_site_lds<R, DOF, false>(Ls, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                                       lane, epi);
+                        continue;
+                    }
+                }
+                coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane,
+                                              epi);
+            }
+            return;
+        }
+        for (int site = s0 + wv; site < s1; site += kNW)
+            for (int tl = 0; tl < ntile; tl += 2) {
+                if (!((mk2 >> (16 * tl)) & 0xffffffffull)) continue;
+                if (tl + 1 < ntile)
+                    coarse_site<R, DOF, false, 2>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                                  lane, epi);
+                else
+                    coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
+                                                  lane, epi);
             }
     };
     auto load_rows = [&](cplx<R> (&r)[RPL], const cplx<R>* vec) {   // vec: system base + row0
@@ -738,13 +814,13 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     // overwrite rows that a neighbour's stencil still reads: its next update waits for
     // coefficients that need every work-group's next partial sums, which follow that
     // work-group's stencil.
-    auto wait_neighbours = [&](unsigned gstep) {
+    auto wait_nb = [&](const unsigned* flags, unsigned target) {
         __syncthreads();
         if (wv == 0) {
             bool ok = true;
             if (nbw >= 0) {
                 unsigned spins = 0;
-                while (__hip_atomic_load(&a.bar->udone[nbw * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gstep) {
+                while (__hip_atomic_load(&flags[nbw * kFlagStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                     if ((++spins & 63u) == 0 &&
                         (__hip_atomic_load(&a.bar->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
                          spins > kBarSpins)) {
@@ -760,6 +836,33 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (lane == 0) si[2] = bad ? 0 : 1;
         }
         __syncthreads();
+    };
+    auto wait_neighbours = [&](unsigned gstep) { wait_nb(a.bar->udone, gstep); };
+    // w = D p(D) u_j for the systems of mk2 by the product form z_0 = u_j,
+    // z_{i+1} = z_i - D z_i / theta_i, the last stage stored as w = u_j - z_npoly
+    // (= D p(D) u_j).  Between two stages the slice publishes its rows of z_{i+1} and waits
+    // for the neighbours whose rows its stencil reads (the same argument as for the update
+    // flags keeps the ping-pong buffers safe: a work-group overwrites a stage's rows only
+    // after its neighbours have published the next stage, which follows their reads of
+    // them).  false when a wait timed out.
+    unsigned zst = 0;   // stages published in this launch (the same count in every work-group)
+    auto poly_stages = [&](uint64_t mk2) -> bool {
+        const cplx<R>* zin = a.u;
+        for (int i = 0; i < npoly; i++) {
+            const bool fin = i == npoly - 1;
+            cplx<R>* zout = fin ? a.w : zbuf(i + 1);
+            stencil_e(zin, zout, mk2, PolyEpi<R>{fin ? a.u : nullptr, nullptr, a.pc[i], czero<R>()});
+            if (fin) break;
+            ++zst;
+            release_stores();
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_store(&a.bar->zdone[wg * kFlagStride], zst, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            wait_nb(a.bar->zdone, zst);
+            if (!si[2]) return false;
+            zin = zout;
+        }
+        return true;
     };
     // Waits (wave 0, a lane per system) until the owner of every system in `run` has
     // published step gstep's coefficients, then acquires them for the work-group.  Returns
@@ -953,7 +1056,11 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             for (int o = LPS; o < 64; o *= 2) nI = max(nI, __shfl_xor(nI, o, 64));
             const cplx<R>* et = a.etals + (size_t)q * m;
             cplx<R> xr[RPL];
-            load_rows(xr, a.x + (size_t)q * n + row0);
+            if (npoly)   // s = Q eta; x += p(D) s follows
+#pragma unroll
+                for (int k = 0; k < RPL; k++) xr[k] = czero<R>();
+            else
+                load_rows(xr, a.x + (size_t)q * n + row0);
             for (int i0 = 0; i0 < nI; i0 += kCU) {
                 S qv[kCU][RPL];
                 cplx<R> e[kCU];
@@ -973,9 +1080,22 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 }
             }
             cplx<R>* xo = a.x + (size_t)q * n + row0 + lb;
-            if (lrow && act)
+            if (lrow && act) {
+                if (npoly) {   // z_0 = s, x += s / theta_0
+                    cplx<R>* zo = zb0 + (size_t)q * n + row0 + lb;
+                    const cplx<R> c0 = a.pc[0];
 #pragma unroll
-                for (int k = 0; k < RPL; k++) xo[k] = xr[k];
+                    for (int k = 0; k < RPL; k++) {
+                        zo[k] = xr[k];
+                        cplx<R> xv = xo[k];
+                        cmadd(xv, c0, xr[k]);
+                        xo[k] = xv;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < RPL; k++) xo[k] = xr[k];
+                }
+            }
         }
     };
     // ---- x0 = eta, r = eta - A x0, ||eta||, ||r||
@@ -1030,7 +1150,10 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                 wait_neighbours(gstep);
                 if (!si[2]) return;
             }
-            if (hw) stencil(a.u, a.w, nullptr, run, false);
+            if (hw) {
+                if (npoly == 0) stencil(a.u, a.w, nullptr, run, false);
+                else if (!poly_stages(run)) return;
+            }
             __syncthreads();
             clk(7);
             const uint64_t tw0 = a.wgclk ? wall_clock64() : 0;
@@ -1063,8 +1186,11 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             if (a.wgclk && lane == 0) a.wgclk[wg * kClk + 9 + kNW + wv] += (double)(wall_clock64() - tw1);
             clk(4);
             const int nc = __builtin_popcountll(cont);
-            count((hw ? links_bytes + 2.0 * nr * n * sR : 0.0) + nr * n * (sQ * j + sR * (hw ? 2 : 1)) +
-                      nc * n * (sQ * (j + 1) + sR * 3),
+            // stencil: each sweep reads its input and writes its output (the last poly stage
+            // also reads u_j)
+            const double sweeps = npoly ? npoly : 1;
+            count((hw ? sweeps * (sweep_links + 2.0 * nr * n * sR) + (npoly ? nr * n * sR : 0.0) : 0.0) +
+                      nr * n * (sQ * j + sR * (hw ? 2 : 1)) + nc * n * (sQ * (j + 1) + sR * 3),
                   1);
             // publish this slice's update (every thread's stores precede thread 0's release)
             release_stores();
@@ -1075,11 +1201,18 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
         if (owner && ((live >> sys) & 1ull)) backsub();
         if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         xupdate(live);
+        // x += p(D) s: the stages z_{i+1} = z_i - D z_i / theta_i from z_0 = s, each added
+        // to x with 1/theta_{i+1} (z_0's term was added by xupdate)
+        for (int i = 0; i + 1 < npoly; i++) {
+            if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
+            stencil_e(zbuf(i), zbuf(i + 1), live, PolyEpi<R>{nullptr, a.x, a.pc[i], a.pc[i + 1]});
+            count(sweep_links + nl * n * sR * 4, 0);
+        }
         if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         stencil(a.x, a.u, a.eta, live, true);
         __syncthreads();
         reduce_norm(live, 2 * m, a.u);
-        count(links_bytes + nl * n * sR * 6, 0);
+        count(sweep_links + nl * n * sR * 6, 0);
         if (!grid_barrier(a.bar, bar_gen, wg, G, &si[3])) return;
         if (owner && ((live >> sys) & 1ull)) {
             const double err = sqrt(total1(sys, 2 * m).x);
@@ -1110,16 +1243,16 @@ __global__ __launch_bounds__(256) void k_bar_reset(GridBar* __restrict__ b, unsi
     for (size_t i = threadIdx.x; i < sizeof(GridBar) / sizeof(unsigned); i += blockDim.x) w[i] = 0u;
 }
 
-template <typename R, typename S, int DOF, int LPS>
+template <typename R, typename S, int DOF, int LPS, bool POLY>
 void launch_coarsest(const CArgs<R, S>& a, int G, size_t sh, hipStream_t s) {
     static bool attr = false;   // dynamic LDS above 64 KB (links cached in LDS)
     if (!attr) {
-        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS>),
+        DD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS, POLY>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr = true;
     }
     void* args[] = {const_cast<CArgs<R, S>*>(&a)};
-    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS>), dim3(G), dim3(kCT), args,
+    DD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_coarsest<R, S, DOF, LPS, POLY>), dim3(G), dim3(kCT), args,
                                       (unsigned)sh, s));
 }
 
@@ -1198,6 +1331,25 @@ void CoarsestGmres<R>::check_failure() {
 }
 
 template <typename R>
+void CoarsestGmres<R>::set_poly(const std::vector<std::complex<double>>& theta) {
+    if (!ok_) return;
+    if ((int)theta.size() > kPolyMax) throw BadShape("coarsest GMRES: polynomial degree above 32");
+    npoly_ = (int)theta.size();
+    if (!npoly_) return;
+    if (!pc_.bytes()) {
+        pc_.alloc(sizeof(cplx<R>) * kPolyMax);
+        z_.alloc(sizeof(cplx<R>) * 2 * (size_t)Nx_ * Nt_ * dof_ * cap_);
+    }
+    std::vector<cplx<R>> h(kPolyMax, mk<R>(R(0), R(0)));
+    for (int i = 0; i < npoly_; i++) {
+        const std::complex<double> c = 1.0 / theta[i];
+        h[i] = mk<R>((R)c.real(), (R)c.imag());
+    }
+    DD_HIP(hipMemcpyAsync(pc_.as<void>(), h.data(), sizeof(cplx<R>) * kPolyMax, hipMemcpyHostToDevice, s_));
+    DD_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename R>
 void CoarsestGmres<R>::solve(const cplx<R>* L, double m0, const cplx<R>* eta, cplx<R>* psi, uint64_t mask,
                              int* iters) {
     if (!ok_) throw BadShape("coarsest GMRES: persistent kernel not available for this level");
@@ -1248,6 +1400,10 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     a.spw = spw_;
     a.m0 = (R)m0;
     a.tol = tol_;
+    a.npoly = npoly_;
+    a.pc = pc_.as<cplx<R>>();
+    a.z = z_.as<cplx<R>>();
+    a.zs = (size_t)Nx_ * Nt_ * dof_ * cap_;
     // FP32 slices of up to 4 sites keep their links in LDS when they fit beside the state
     const size_t lbytes = (size_t)spw_ * 5 * dof_ * dof_ * sizeof(cplx<R>);
     a.lds_links = (sizeof(R) == 4 && dof_ <= 24 && spw_ <= 4 && coarsest_lds(m_) + lbytes <= 160 * 1024) ? 1 : 0;
@@ -1262,8 +1418,10 @@ void CoarsestGmres<R>::run(const cplx<R>* L, double m0, const cplx<R>* eta, cplx
     switch (dof_) {
 #define DD_CASE(D)                                                   \
     case D:                                                          \
-        if (spw_ <= 4) launch_coarsest<R, S, D, 32>(a, G_, sh, s_);  \
-        else launch_coarsest<R, S, D, 64>(a, G_, sh, s_);            \
+        if (spw_ <= 4 && npoly_) launch_coarsest<R, S, D, 32, true>(a, G_, sh, s_);   \
+        else if (spw_ <= 4) launch_coarsest<R, S, D, 32, false>(a, G_, sh, s_);  \
+        else if (npoly_) launch_coarsest<R, S, D, 64, true>(a, G_, sh, s_);      \
+        else launch_coarsest<R, S, D, 64, false>(a, G_, sh, s_);                 \
         break;
         DD_CASE(8) DD_CASE(16) DD_CASE(24) DD_CASE(32)
 #undef DD_CASE

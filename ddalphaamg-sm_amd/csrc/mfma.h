@@ -108,16 +108,27 @@ __device__ __forceinline__ void mfma_hop_nt(typename Mfma<R>::acc_t (&acc)[NT][D
     }
 }
 
+// Store of a stencil result: out[i] = D v at row i (v = the input's own value there).
+template <typename R>
+struct StoreEpi {
+    __device__ __forceinline__ void operator()(cplx<R>* __restrict__ out, size_t i, cplx<R> v, cplx<R> Dv) const {
+        (void)v;
+        out[i] = Dv;
+    }
+};
+
 // One site of the generic level stencil for NT tiles of 16 systems (one wave):
 //   out = (m0+2) v - A v - sum_mu (B_mu s+ v(x+mu) + C_mu s- v(x-mu))      (RES: b - that)
 // for site n and systems 16*(tile0 + k) + (lane&15), k < NT, that are in `mask` and below
 // nrhs; links [V][5][DOF][DOF] column-major, vectors [system][V][DOF] with stride ld.
 // Every load is unconditional (indices of systems that do not take part are replaced by
-// system 0) and masked afterwards, so no branch separates the loads of a hop.
-template <typename R, int DOF, bool RES, int NT = 1>
+// system 0) and masked afterwards, so no branch separates the loads of a hop.  Without
+// RES the result goes through epi(out, row index, own input value, result).
+template <typename R, int DOF, bool RES, int NT = 1, typename Epi = StoreEpi<R>>
 __device__ __forceinline__ void coarse_site(const cplx<R>* __restrict__ L, const cplx<R>* __restrict__ in,
                                             cplx<R>* __restrict__ out, const cplx<R>* __restrict__ b, int Nx, int Nt,
-                                            R m0, uint64_t mask, size_t ld, int nrhs, int n, int tile0, int lane) {
+                                            R m0, uint64_t mask, size_t ld, int nrhs, int n, int tile0, int lane,
+                                            const Epi& epi = Epi()) {
     using acc_t = typename Mfma<R>::acc_t;
     constexpr int MT = DOF / 8, KS = DOF / 2;
     const int g = lane >> 4;
@@ -178,12 +189,13 @@ __device__ __forceinline__ void coarse_site(const cplx<R>* __restrict__ L, const
             const cplx<R> v0 = in[base + r0], v1 = in[base + r0 + 1];
             cplx<R> o0 = mk<R>(mass * v0.x - acc[k][tt][0], mass * v0.y - acc[k][tt][1]);
             cplx<R> o1 = mk<R>(mass * v1.x - acc[k][tt][2], mass * v1.y - acc[k][tt][3]);
-            if (RES) {
-                o0 = b[base + r0] - o0;
-                o1 = b[base + r0 + 1] - o1;
+            if constexpr (RES) {
+                out[base + r0] = b[base + r0] - o0;
+                out[base + r0 + 1] = b[base + r0 + 1] - o1;
+            } else {
+                epi(out, base + r0, v0, o0);
+                epi(out, base + r0 + 1, v1, o1);
             }
-            out[base + r0] = o0;
-            out[base + r0 + 1] = o1;
         }
     }
 }

@@ -135,6 +135,11 @@ class Multigrid {
     // reports (throws) a timed-out wait inside a fused SAP iteration since the last check
     void check_sap_failures();
     int graph_count() const { return (int)graphs_.size(); }
+    // roots of the coarsest GMRES's polynomial preconditioner in their order of
+    // application (empty without p.coarse_poly or before the coarsest operator exists)
+    const std::vector<std::complex<double>>& poly_roots() const { return poly_theta_; }
+    // out = p(D_c) in on the coarsest level for the systems of mask (in != out)
+    void poly_apply(const cplx<R>* in, cplx<R>* out, uint64_t mask);
     ~Multigrid();
     void apply(int l, const cplx<R>* in, cplx<R>* out, const cplx<R>* b, uint64_t mask);
     void restrict_(int l, const cplx<R>* in, cplx<R>* out, uint64_t mask);
@@ -197,6 +202,12 @@ class Multigrid {
     std::vector<char> fuse_ok_;
     int fuse_slices_ = 256;
     std::map<std::tuple<const void*, const void*, uint64_t, bool>, hipGraphExec_t> graphs_;
+    // GMRES polynomial preconditioner of the coarsest solve (p.coarse_poly = degree):
+    // roots, per-root lincomb coefficients [root][3 forms][64 systems][3], work vectors
+    std::vector<std::complex<double>> poly_theta_;
+    DevMem poly_coef_, poly_r_, poly_t_;
+    // roots from p.coarse_poly Arnoldi steps on the coarsest operator as it is now
+    void poly_build();
 };
 
 // ---- top-level solver --------------------------------------------------------------------

@@ -8,6 +8,7 @@
 //   Multigrid::coarsest      GMRES_level_l, include/level.h:70-140
 //   Multigrid::setup         AlgebraicMG::setUpPhase, src/amg.cpp:4-62
 //   Solver::solve            FGMRES_AMG_{k,v}_cycle, include/amg.h:147-270
+#include "poly.h"
 #include "solver.h"
 
 #include <algorithm>
@@ -574,6 +575,13 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
         }
         DD_HIP(hipStreamSynchronize(s));
     }
+    if (p.coarse_poly < 0 || p.coarse_poly > 32) throw BadShape("coarse_poly: degree outside [0, 32]");
+    if (p.coarse_poly > 0) {
+        const LevelGeom& Gc = g_[L - 1];
+        poly_coef_.alloc(sizeof(cplx<double>) * (size_t)p.coarse_poly * 3 * 64 * 3);
+        poly_r_.alloc(sizeof(cplx<R>) * Gc.n() * cap_[L - 1]);
+        poly_t_.alloc(sizeof(cplx<R>) * Gc.n() * cap_[L - 1]);
+    }
     if (p.ortho_coarse >= 2) {
         const LevelGeom& Gc = g_[L - 1];
         coarse_dev_.reset(new CoarsestGmres<R>(s, Gc.Nx, Gc.Nt, Gc.dof, p.coarse_m, p.coarse_restarts, p.coarse_tol,
@@ -582,8 +590,8 @@ Multigrid<R>::Multigrid(hipStream_t s, int Nx, int Nt, const dd_params& p, const
     }
     // launch-per-step engine: always for 0/1, as the alternative the setup times for 3
     if (!coarse_dev_ || p.ortho_coarse == 3)
-        coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap, false,
-                                    p.ortho_coarse == 0 ? 0 : 1));
+        coarse_.reset(new Krylov<R>(s, g_[L - 1].n(), p.coarse_m, p.coarse_restarts, p.coarse_tol, cap,
+                                    p.coarse_poly > 0, p.ortho_coarse == 0 ? 0 : 1));
     use_dev_ = (bool)coarse_dev_;
     iters_scratch_.resize(64);
 }
@@ -664,6 +672,107 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
     }
 }
 
+// Start vector of the Arnoldi run that fixes the coarsest polynomial preconditioner: the
+// hash fill of this seed (k_fill_hash), the same on every run.
+constexpr uint64_t kPolySeed = 4242;
+
+// The roots of the GMRES polynomial preconditioner (poly.h) from p.coarse_poly Arnoldi
+// steps of the coarsest operator, one system: the operator is applied on the device in
+// the hierarchy's precision to the basis vector rounded to it, and the Gram-Schmidt
+// projections (two passes) run on the host in FP64.
+template <typename R>
+void Multigrid<R>::poly_build() {
+    const int d = p_.coarse_poly;
+    poly_theta_.clear();
+    if (d <= 0) return;
+    const int L = levels();
+    const size_t n = g_[L - 1].n();
+    cplx<R>* v = poly_r_.as<cplx<R>>();
+    cplx<R>* w = poly_t_.as<cplx<R>>();
+    k_fill_hash<R><<<(unsigned)((n + 255) / 256), 256, 0, s_>>>(v, n, kPolySeed);
+    DD_HIP(hipGetLastError());
+    std::vector<cplx<R>> hb(n);
+    auto down = [&](const cplx<R>* src, std::vector<Cd>& dst) {
+        DD_HIP(hipMemcpyAsync(hb.data(), src, sizeof(cplx<R>) * n, hipMemcpyDeviceToHost, s_));
+        DD_HIP(hipStreamSynchronize(s_));
+        for (size_t i = 0; i < n; i++) dst[i] = Cd((double)hb[i].x, (double)hb[i].y);
+    };
+    auto norm = [&](const std::vector<Cd>& x) {
+        double a = 0;
+        for (const Cd& z : x) a += std::norm(z);
+        return std::sqrt(a);
+    };
+    std::vector<std::vector<Cd>> Q(d + 1, std::vector<Cd>(n));
+    std::vector<Cd> H((size_t)(d + 1) * d, 0.0), y(n);
+    down(v, Q[0]);
+    const double n0 = norm(Q[0]);
+    for (Cd& z : Q[0]) z /= n0;
+    for (int j = 0; j < d; j++) {
+        for (size_t i = 0; i < n; i++) hb[i] = mk<R>((R)Q[j][i].real(), (R)Q[j][i].imag());
+        DD_HIP(hipMemcpyAsync(v, hb.data(), sizeof(cplx<R>) * n, hipMemcpyHostToDevice, s_));
+        apply(L - 1, v, w, nullptr, 1ull);
+        down(w, y);
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i <= j; i++) {
+                Cd h = 0.0;
+                for (size_t k = 0; k < n; k++) h += std::conj(Q[i][k]) * y[k];
+                for (size_t k = 0; k < n; k++) y[k] -= h * Q[i][k];
+                H[(size_t)i * d + j] += h;
+            }
+        const double beta = norm(y);
+        if (!(beta > 0.0)) throw BadShape("coarse_poly: Arnoldi breakdown (degree above the operator's grade)");
+        H[(size_t)(j + 1) * d + j] = beta;
+        for (size_t k = 0; k < n; k++) Q[j + 1][k] = y[k] / beta;
+    }
+    try {
+        poly_theta_ = gmres_poly_roots(H, d);
+    } catch (const std::runtime_error&) {
+        throw BadShape("coarse_poly: no harmonic Ritz values (singular Hessenberg matrix or no QR convergence)");
+    }
+    // lincomb coefficients per root i and system r: form 0 (1/theta_i, 0, 0),
+    // form 1 (1, 1/theta_i, 0), form 2 (1, -1/theta_i, 0)
+    std::vector<cplx<double>> c((size_t)d * 3 * 64 * 3, mk<double>(0.0, 0.0));
+    for (int i = 0; i < d; i++) {
+        const Cd inv = 1.0 / poly_theta_[i];
+        for (int r = 0; r < 64; r++) {
+            cplx<double>* f0 = &c[((size_t)(i * 3 + 0) * 64 + r) * 3];
+            cplx<double>* f1 = &c[((size_t)(i * 3 + 1) * 64 + r) * 3];
+            cplx<double>* f2 = &c[((size_t)(i * 3 + 2) * 64 + r) * 3];
+            f0[0] = mk<double>(inv.real(), inv.imag());
+            f1[0] = mk<double>(1.0, 0.0);
+            f1[1] = mk<double>(inv.real(), inv.imag());
+            f2[0] = mk<double>(1.0, 0.0);
+            f2[1] = mk<double>(-inv.real(), -inv.imag());
+        }
+    }
+    DD_HIP(hipMemcpyAsync(poly_coef_.as<void>(), c.data(), sizeof(cplx<double>) * c.size(), hipMemcpyHostToDevice,
+                          s_));
+    DD_HIP(hipStreamSynchronize(s_));
+    if (coarse_dev_) coarse_dev_->set_poly(poly_theta_);
+}
+
+// out = p(D_c) in by the product form: out = in/theta_0, r = in; for i < d - 1:
+// r -= D_c r / theta_i, out += r / theta_{i+1}
+template <typename R>
+void Multigrid<R>::poly_apply(const cplx<R>* in, cplx<R>* out, uint64_t mask) {
+    const int L = levels(), d = (int)poly_theta_.size();
+    const size_t n = g_[L - 1].n();
+    const int nr = mask_count(mask);
+    if (nr > cap_[L - 1]) throw BadShape("coarse_poly: batch larger than capacity");
+    cplx<R>* r = poly_r_.as<cplx<R>>();
+    cplx<R>* t = poly_t_.as<cplx<R>>();
+    const cplx<double>* c = poly_coef_.as<cplx<double>>();
+    auto cf = [&](int i, int f) { return c + (size_t)(i * 3 + f) * 64 * 3; };
+    lincomb<R>(s_, out, in, nullptr, nullptr, cf(0, 0), n, n, nr, mask);
+    const cplx<R>* cur = in;
+    for (int i = 0; i + 1 < d; i++) {
+        apply(L - 1, cur, t, nullptr, mask);
+        lincomb<R>(s_, r, cur, t, nullptr, cf(i, 2), n, n, nr, mask);
+        lincomb<R>(s_, out, out, r, nullptr, cf(i + 1, 1), n, n, nr, mask);
+        cur = r;
+    }
+}
+
 template <typename R>
 void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
     const int L = levels();
@@ -682,14 +791,24 @@ void Multigrid<R>::coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
             use_dev_ = false;
             if (!coarse_)
                 coarse_.reset(new Krylov<R>(s_, g_[L - 1].n(), p_.coarse_m, p_.coarse_restarts, p_.coarse_tol,
-                                            std::max(1, p_.max_rhs), false, 1));
+                                            std::max(1, p_.max_rhs), p_.coarse_poly > 0, 1));
         }
     }
     if (capturing_) return;   // no host read-back inside a capture
     if (!use_dev_) {
         auto A = [this, L](const cplx<R>* in, cplx<R>* out, uint64_t m) { apply(L - 1, in, out, nullptr, m); };
         typename Krylov<R>::Func fA = A;
-        coarse_->solve(eta, eta, psi, mask, fA, nullptr, iters_scratch_.data(), nullptr);
+        typename Krylov<R>::Prec fM;
+        if (p_.coarse_poly > 0) {
+            const size_t n = g_[L - 1].n();
+            if (!poly_theta_.empty())
+                fM = [this](const cplx<R>* in, cplx<R>* out, uint64_t m) { poly_apply(in, out, m); };
+            else   // no roots yet (the coarsest operator was never built): the identity
+                fM = [this, n](const cplx<R>* in, cplx<R>* out, uint64_t m) {
+                    copy<R>(s_, out, in, n, n, n, mask_count(m), m);
+                };
+        }
+        coarse_->solve(eta, eta, psi, mask, fA, p_.coarse_poly > 0 ? &fM : nullptr, iters_scratch_.data(), nullptr);
     }
     int mx = 0;
     for (int q = 0; q < mask_count(mask); q++)
@@ -869,6 +988,7 @@ void Multigrid<R>::setup(const cplx<R>* tv0, RankStreams& rng) {
         DD_HIP(hipStreamSynchronize(s_));
     }
     check_sap_failures();
+    poly_build();
 }
 
 template <typename R>
@@ -917,6 +1037,7 @@ void Multigrid<R>::galerkin_level(int l) {
         to_half(s_, reinterpret_cast<const float*>(links(l + 1)), hlinks_[l + 1].as<_Float16>(),
                 hlinks_[l + 1].bytes() / sizeof(_Float16));
     DD_HIP(hipStreamSynchronize(s_));
+    if (l + 1 == levels() - 1) poly_build();
 }
 
 // Bootstrap adaptivity as the reference's documentation describes it (AMG_Documentation,
@@ -966,6 +1087,7 @@ void Multigrid<R>::bootstrap(int passes) {
             orthonormalize<R>(s_, W_[l].as<cplx<R>>(), G.Nx, G.Nt, G.dof, G.ax, G.at, G.nv);
             rebuild(l);
             DD_HIP(hipStreamSynchronize(s_));
+            poly_build();   // the coarsest operator changed
         }
 }
 

@@ -51,7 +51,8 @@ class Params(C.Structure):
                 ('ortho_outer', C.c_int), ('ortho_inner', C.c_int), ('ortho_coarse', C.c_int),
                 ('max_rhs', C.c_int), ('autotune', C.c_int),
                 ('sap_variant', C.c_int), ('coarse_basis', C.c_int), ('coarse_sap_links', C.c_int),
-                ('bootstrap', C.c_int), ('graph', C.c_int), ('sap_fused', C.c_int)]
+                ('bootstrap', C.c_int), ('graph', C.c_int), ('sap_fused', C.c_int),
+                ('coarse_poly', C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -100,6 +101,8 @@ _SIGS = {
     'dd_get_test_vectors': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_get_links': (C.c_int, [_ctx, C.c_int, _dp, C.c_int]),
     'dd_level_shape': (C.c_int, [_ctx, C.c_int, _ip, _ip, _ip]),
+    'dd_get_coarse_poly': (C.c_int, [_ctx, _ip, _dp]),
+    'dd_poly_roots': (C.c_int, [_dp, C.c_int, _dp]),
     'dd_get_tuning': (C.c_int, [_ctx, C.c_int, _ip, _dp]),
     'dd_coarsest_stats': (C.c_int, [_ctx, _dp]),
     'dd_graph_count': (C.c_int, [_ctx, C.POINTER(C.c_int)]),
@@ -467,6 +470,14 @@ class Context:
                              _cptr(rr)), allow=(DD_OK, DD_ERR_NOT_CONVERGED))
         return (xb if v.ndim == 4 else xb[0]), np.array(list(its)), rr
 
+    def coarse_poly_roots(self):
+        """Roots of the coarsest GMRES's polynomial preconditioner in their order of
+        application (empty without dd_params.coarse_poly)."""
+        d = C.c_int()
+        out = np.zeros(64, complex)
+        _chk(lib.dd_get_coarse_poly(self._h, C.byref(d), _cptr(out)))
+        return out[:d.value].copy()
+
     def links(self, level, precision=0):
         """(A, B[2], C[2]) of level >= 1, each (Nx, Nt, dof, dof); precision 0 FP64, 1 FP32,
         2 the binary16 copy of the FP32 links that the matrix-core coarse SAP reads."""
@@ -474,6 +485,19 @@ class Context:
         out = np.zeros((Nx, Nt, 5, d, d), complex)
         _chk(lib.dd_get_links(self._h, level, _cptr(out), precision))
         return out[:, :, 0], np.stack([out[:, :, 1], out[:, :, 2]]), np.stack([out[:, :, 3], out[:, :, 4]])
+
+
+def poly_roots(H):
+    """Harmonic Ritz values of the Arnoldi relation H ((d+1) x d) in Leja order, as the
+    library orders the coarsest polynomial preconditioner's roots (dd_poly_roots; host
+    code, no device needed)."""
+    H = np.ascontiguousarray(H, dtype=complex)
+    d = H.shape[1]
+    if H.shape != (d + 1, d):
+        raise ValueError('H must be (d+1) x d')
+    out = np.zeros(d, complex)
+    _chk(lib.dd_poly_roots(_cptr(H), d, _cptr(out)))
+    return out
 
 
 def solve_many_dev(contexts, nrhs, b_ptrs, x_ptrs):

@@ -102,6 +102,14 @@ typedef struct dd_params {
                                       blocks form their right-hand sides v - D x from the
                                       iterate; 0: separate launches with full-lattice
                                       residuals between the colours.                */
+    int coarse_poly;               /* degree d of a GMRES polynomial right
+                                      preconditioner of the coarsest GMRES: its roots
+                                      are the harmonic Ritz values of d Arnoldi steps
+                                      on the coarsest operator (a fixed start vector),
+                                      computed when that operator is built; the solve
+                                      then runs on D_c p(D_c), which needs far fewer
+                                      Arnoldi steps to the same tolerance.  0 (the
+                                      default): the reference's plain GMRES.        */
 } dd_params;
 
 typedef struct dd_solve_info {
@@ -303,6 +311,14 @@ int dd_coarsest_stats(dd_ctx* ctx, double* out);
 /* Number of V-cycle graphs the hierarchy has captured (dd_params.graph = 1); no
  * reference counterpart (diagnostics). */
 int dd_graph_count(dd_ctx* ctx, int* count);
+/* Roots of the coarsest GMRES's polynomial preconditioner (dd_params.coarse_poly > 0),
+ * in the order they are applied: roots[2i], roots[2i+1] = re, im of root i; *degree
+ * receives their number (0 without the preconditioner).  No reference counterpart. */
+int dd_get_coarse_poly(dd_ctx* ctx, int* degree, double* roots);
+/* Host computation behind it, for any Arnoldi relation: from H ((d+1) x d complex,
+ * row-major, H[2(i*d+j)] = re h_ij) the harmonic Ritz values in Leja order, as
+ * roots[2i], roots[2i+1].  Needs no device.  No reference counterpart. */
+int dd_poly_roots(const double* H, int d, double* roots);
 /* Shape of level l: Nx, Nt, dof. */
 int dd_level_shape(dd_ctx* ctx, int level, int* Nx, int* Nt, int* dof);
 /* ---- live kernel timing -----------------------------------------------------------------
