@@ -38,10 +38,11 @@ SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-
 BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
 SAP_FUSED = None      # None: the library default; 0 separate launches per colour, 1 one launch per iteration
 GRAPH = 0             # 1: the cycles replayed from captured HIP graphs (--graph)
-COARSE_POLY = None    # None: the library default (0); degree of the coarsest GMRES's polynomial preconditioner
+COARSE_POLY = 12      # degree of the coarsest GMRES's polynomial preconditioner in the FP32 (mixed)
+                      # hierarchies (--coarse-poly; 0: the reference's plain GMRES); FP64 rows keep plain GMRES
 
 
-def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
+def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18, coarse_poly=None):
     p = ddamg.params(3, agg=[(4, 4), (4, 4)], sap_block=[(4, 4), (4, 4)], nv=[12, 12], m0=m0, nu2=2,
                      fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
     if COARSE_BASIS is not None:
@@ -52,8 +53,7 @@ def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18):
     if SAP_FUSED is not None:
         p.sap_fused = SAP_FUSED
     p.graph = GRAPH
-    if COARSE_POLY is not None:
-        p.coarse_poly = COARSE_POLY
+    p.coarse_poly = (COARSE_POLY if precision == 1 else 0) if coarse_poly is None else coarse_poly
     return p
 
 
@@ -89,8 +89,11 @@ def main():
                     help='coarse SAP iterations in one launch (1) or per colour with separate residuals (0)')
     ap.add_argument('--graph', type=int, choices=[0, 1], default=0,
                     help='replay the cycles from captured HIP graphs (K-cycle FGMRES device-controlled)')
-    ap.add_argument('--coarse-poly', type=int, default=None,
-                    help='degree of the GMRES polynomial right preconditioner of the coarsest solve (0: plain GMRES)')
+    ap.add_argument('--coarse-poly', type=int, default=12,
+                    help='degree of the GMRES polynomial right preconditioner of the coarsest solve in the mixed '
+                         'hierarchies (0: the plain GMRES of the reference, which the FP64 rows always use)')
+    ap.add_argument('--skip-no-poly', action='store_true',
+                    help='skip the row that repeats the batches without the coarsest polynomial preconditioner')
     ap.add_argument('--setup-only', action='store_true',
                     help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
@@ -194,13 +197,27 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     dom_ms, dom_n, dom_bytes = ddamg.timer_read(dom_id)
     ddamg.timer_enable(dom_id, False)
-    peak = 8000.0
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': peak, 'unit': 'GB/s',
-                'frac': achieved / peak, 'traffic': measured_traffic(dom, N, nrhs),
-                'avg_launch_us': dom_ms * 1e3 / max(dom_n, 1), 'launches': dom_n,
-                'algorithmic_bytes_per_launch': dom_bytes / max(dom_n, 1)}
-    log(f'roofline {dom}: {achieved:.1f} GB/s over {dom_n} launches')
+    dom_flops = ddamg.timer_flops(dom_id)
+    hbm = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    if dom_flops > 0:
+        # a matrix-core kernel that counts its own products (the coarse SAP, FP32): against
+        # the FP32 matrix peak; its HBM rate alongside
+        peak = 157.3
+        achieved = dom_flops / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+        roofline = {'bound': 'mfma', 'kernel': dom, 'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                    'frac': achieved / peak, 'traffic': measured_traffic(dom, N, nrhs),
+                    'avg_launch_us': dom_ms * 1e3 / max(dom_n, 1), 'launches': dom_n,
+                    'algorithmic_flops_per_launch': dom_flops / max(dom_n, 1),
+                    'algorithmic_bytes_per_launch': dom_bytes / max(dom_n, 1), 'hbm_GB_per_s': hbm}
+        log(f'roofline {dom}: {achieved:.2f} TFLOP/s (FP32 matrix cores) over {dom_n} launches, {hbm:.1f} GB/s')
+    else:
+        peak = 8000.0
+        achieved = hbm
+        roofline = {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': peak, 'unit': 'GB/s',
+                    'frac': achieved / peak, 'traffic': measured_traffic(dom, N, nrhs),
+                    'avg_launch_us': dom_ms * 1e3 / max(dom_n, 1), 'launches': dom_n,
+                    'algorithmic_bytes_per_launch': dom_bytes / max(dom_n, 1)}
+        log(f'roofline {dom}: {achieved:.1f} GB/s over {dom_n} launches')
     x_mixed = x_dev.cpu().numpy().view(np.complex128).reshape(nrhs, N, N, 2)
     # true FP64 residual of every system of every timed batch
     r_dev = torch.zeros_like(b_dev)
@@ -237,6 +254,23 @@ def main():
                       'max_relres': info32.relres,
                       'ms_per_outer_iteration': t32 * 1e3 / max(int(its32.max()), 1)}
         log(f'mixed, FP32 coarsest basis: {t32 * 1e3:.1f} ms per solve, its {its32.min()}..{its32.max()}')
+    # the same batches without the coarsest polynomial preconditioner (the reference's plain
+    # coarsest GMRES inside the same mixed hierarchy)
+    no_poly = None
+    if not args.skip_mixed and not args.skip_no_poly and COARSE_POLY > 0:
+        ctx.setup(hierarchy_params(ddamg, N, 1, nrhs, m0, coarse_poly=0))
+        xs = [torch.zeros_like(v) for v in b_devs]
+        ctx.solve_dev(b_devs[0].data_ptr(), xs[0].data_ptr(), nrhs)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res0 = [ctx.solve_dev(b_devs[k].data_ptr(), xs[k].data_ptr(), nrhs) for k in range(args.steps)]
+        torch.cuda.synchronize()
+        tn = (time.perf_counter() - t0) / max(args.steps, 1)
+        per = [int(r[0].max()) for r in res0]
+        no_poly = {'ms_per_rhs': tn * 1e3 / nrhs, 'outer_iterations_per_batch': per,
+                   'ms_per_outer_iteration': tn * 1e3 * args.steps / max(sum(per), 1),
+                   'max_relres': max(r[1].relres for r in res0)}
+        log(f'mixed without the coarsest polynomial: {tn * 1e3:.1f} ms per solve, per batch {per}')
     ctx.close()
 
     # ---- plain FP64, one RHS -------------------------------------------------------------
@@ -284,11 +318,14 @@ def main():
                  'basis (products in FP32), c128 outer FGMRES and residual', 'data': 'synthetic (seeded Metropolis gauge field, Gaussian RHS)',
         'config': {'workload': f'DD-alphaAMG K-cycle solve, {N}^2 Schwinger Wilson-Dirac, beta=2, m0={m0}, '
                                f'3 levels, 4x4 aggregates, Nv=12, 4x4 SAP blocks, FGMRES(30), tol 1e-10, '
-                               f'{nrhs} RHS per solve', 'lattice': [N, N], 'nrhs': nrhs},
+                               f'{nrhs} RHS per solve; coarsest GMRES(400) to 0.1 with a degree-{COARSE_POLY} '
+                               f'GMRES polynomial right preconditioner (0: none)',
+                   'lattice': [N, N], 'nrhs': nrhs, 'coarse_poly': COARSE_POLY},
         'outer_iterations': [int(its.min()), int(its.max())],
         'outer_iterations_per_batch': outer_per_step,
         'ms_per_outer_iteration': t_mixed * 1e3 * args.steps / max(sum(outer_per_step), 1),
         'mixed_fp32_coarsest_basis': fp32_basis,
+        'mixed_no_coarse_poly': no_poly,
         'max_relres': float(relres.max()),
         'setup_s_mixed': setup_mixed,
         'autotuned_coarse_sap_variant': tune[0],
@@ -297,6 +334,8 @@ def main():
         'plain_fp64_iterations': None if its_plain is None else int(its_plain[0]),
         'setup_s_fp64': setup_plain,
         'speedup_vs_plain_fp64': None if t_plain is None else t_plain * 1e3 / ms_rhs,
+        'speedup_vs_plain_fp64_no_coarse_poly': None if (t_plain is None or no_poly is None)
+        else t_plain * 1e3 / no_poly['ms_per_rhs'],
         'mixed_vs_fp64_rel_error': err_vs_fp64,
         'cpu_baseline': cpu,
         'roofline': roofline,

@@ -865,6 +865,14 @@ int dd_timer_reset(void) {
     return DD_OK;
 }
 
+int dd_timer_flops(int kernel, double* flops) {
+    if (kernel < 0 || kernel >= dd::K_NUM || !flops) return DD_ERR_ARG;
+    return guard([&] {
+        *flops = dd::timer_read_flops(kernel);
+        return DD_OK;
+    });
+}
+
 int dd_timer_read(int kernel, double* total_ms, long long* launches, double* bytes) {
     if (kernel < 0 || kernel >= dd::K_NUM) return DD_ERR_ARG;
     return guard([&] {

@@ -26,6 +26,11 @@ bool timer_any();
 // total elapsed ms, number of timed launches and algorithmic bytes of kernel `id`
 // since the last reset (synchronises the events)
 double timer_read(int id, long long* count, double* bytes);
+// device word that the kernels of family `id` add their algorithmic floating-point
+// operations to while its timer is on (nullptr while it is off: the kernels then count
+// nothing), and its total since the last reset (synchronises the device)
+unsigned long long* timer_flops_dev(int id);
+double timer_read_flops(int id);
 
 // ---- device-side system masks ---------------------------------------------------------
 // While a DevMaskScope is alive on the calling host thread, every launcher below that takes
@@ -88,6 +93,7 @@ void sap_blocksc(hipStream_t s, const cplx<R>* L, const cplx<R>* r, cplx<R>* x, 
 // (a bounded wait timed out), the norm partial sums, the per-system gate of the
 // iterations and ||v||^2 per system (stride 2).
 struct SapFuse {
+    unsigned long long* flops = nullptr;   // matrix-core flops are added here (nullptr: not counted)
     unsigned* done = nullptr;           // [nbx*nbt][ntile]
     unsigned* ctr = nullptr;            // [2]
     unsigned* fail = nullptr;

@@ -32,16 +32,36 @@ struct TimerSlot {
     double bytes = 0;
 };
 TimerSlot g_timers[K_NUM];
+unsigned long long* g_flops = nullptr;   // [K_NUM] device counters, allocated on first enable
 }  // namespace
 
 void timer_enable(int id, bool on) {
-    if (id >= 0 && id < K_NUM) g_timers[id].on = on;
+    if (id < 0 || id >= K_NUM) return;
+    g_timers[id].on = on;
+    if (on && !g_flops) {
+        DD_HIP(hipMalloc(reinterpret_cast<void**>(&g_flops), sizeof(unsigned long long) * K_NUM));
+        DD_HIP(hipMemset(g_flops, 0, sizeof(unsigned long long) * K_NUM));
+    }
 }
 void timer_reset() {
     for (auto& t : g_timers) {
         t.used = 0;
         t.bytes = 0;
     }
+    if (g_flops) {
+        DD_HIP(hipDeviceSynchronize());
+        DD_HIP(hipMemset(g_flops, 0, sizeof(unsigned long long) * K_NUM));
+    }
+}
+unsigned long long* timer_flops_dev(int id) {
+    return (id >= 0 && id < K_NUM && g_timers[id].on && g_flops) ? g_flops + id : nullptr;
+}
+double timer_read_flops(int id) {
+    if (!g_flops || id < 0 || id >= K_NUM) return 0.0;
+    unsigned long long h = 0;
+    DD_HIP(hipDeviceSynchronize());
+    DD_HIP(hipMemcpy(&h, g_flops + id, sizeof(h), hipMemcpyDeviceToHost));
+    return (double)h;
 }
 void timer_begin(hipStream_t s, int id) {
     TimerSlot& t = g_timers[id];
@@ -69,6 +89,8 @@ void timer_release() {
         t.used = 0;
         t.bytes = 0;
     }
+    if (g_flops) (void)hipFree(g_flops);
+    g_flops = nullptr;
 }
 bool timer_any() {
     for (const auto& t : g_timers)

@@ -160,7 +160,9 @@ __device__ __forceinline__ void sapm_task(const LT* __restrict__ L, const float*
 
     // v <- D_B v (zero outside the block, no boundary sign).  The caller reaches a
     // work-group barrier (a reduction) before the next call re-stages.
+    int napp = 0;   // block-operator applications (flop count)
     auto apply = [&](Vec& v) {
+        ++napp;
 #pragma unroll
         for (int u = 0; u < SPW; u++)
             if (own[u]) {
@@ -429,6 +431,13 @@ __device__ __forceinline__ void sapm_task(const LT* __restrict__ L, const float*
                         xv[o] = xv[o] + sx[lidx(u, t, h)];
                     }
     }
+    if (fz.flops && threadIdx.x == 0) {
+        // matrix-core flops of the task: 128 DOF^2 per (site, hop) product of 16 systems;
+        // the block operator has the in-block hops, the fused right-hand side all five
+        const unsigned long long pairs = (unsigned long long)napp * (S + 2 * bx * (bt - 1) + 2 * bt * (bx - 1)) +
+                                         (FUSED ? 5ull * S : 0ull);
+        __hip_atomic_fetch_add(fz.flops, pairs * 128ull * DOF * DOF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     if (FUSED) {   // publish this block's iterate: every wave's stores, then one release
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -446,8 +455,10 @@ template <int DOF, int SPW, typename LT>
 __global__ __launch_bounds__(512) void k_sapm(const LT* __restrict__ L, const cplx<float>* __restrict__ rr,
                                               cplx<float>* __restrict__ xx, int Nx, int Nt, int bx, int bt,
                                               int colour, float m0, int restarts, float tol, uint64_t mask, size_t ld,
-                                              const int* __restrict__ gate, int nrhs, int ntile) {
-    const SapFuse fz{};
+                                              const int* __restrict__ gate, int nrhs, int ntile,
+                                              unsigned long long* flops) {
+    SapFuse fz{};
+    fz.flops = flops;
     sapm_task<DOF, SPW, LT, false>(L, nullptr, rr, xx, Nx, Nt, bx, bt, colour, m0, restarts, tol, mask, ld, gate,
                                    nrhs, ntile, blockIdx.x, fz);
 }
@@ -540,6 +551,9 @@ __global__ __launch_bounds__(512) void k_sapm_fused(const LT* __restrict__ L, co
         for (int w = 0; w < W; w++) t += red_s[w][threadIdx.x];
         fz.part[(size_t)sl * 64 + threadIdx.x] = t;
     }
+    if (fz.flops && threadIdx.x == 0 && ok_s)   // the slice's stencil: five products per site and tile
+        __hip_atomic_fetch_add(fz.flops, (unsigned long long)(n1 - n0) * ntile * 5ull * 128ull * DOF * DOF,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     __shared__ unsigned last_s;
@@ -569,7 +583,7 @@ static void launch_sapm_k(hipStream_t s, dim3 grid, int threads, size_t sh, cons
         attr = true;
     }
     k_sapm<DOF, SPW, LT><<<grid, threads, sh, s>>>(L, r, x, Nx, Nt, bx, bt, colour, (float)m0, restarts, (float)tol,
-                                                    mask, ld, gate, nrhs, ntile);
+                                                    mask, ld, gate, nrhs, ntile, timer_flops_dev(K_SAPC));
 }
 
 // One fused SAP iteration (k_sapm_fused) for the systems in mask; L the FP32 links, Lh
@@ -594,8 +608,10 @@ static void launch_sapm_fused(hipStream_t s, const cplx<float>* L, const _Float1
             attr = true;
         }
         if (sh + 4200 > 160 * 1024) throw BadShape("fused coarse SAP: block does not fit the LDS");
+        SapFuse fzc = fz;
+        fzc.flops = timer_flops_dev(K_SAPC);
         kern<<<grid, threads, sh, s>>>(Ls, Lf, v, x, Nx, Nt, bx, bt, (float)m0, restarts, (float)tol, mask, ld, nrhs,
-                                       ntile, ntask, nslice, fz);
+                                       ntile, ntask, nslice, fzc);
     };
     if (S > 8) {
         if (Lh) go(&k_sapm_fused<DOF, 2, _Float16>, 64 * ((S + 1) / 2), Lh);

@@ -117,6 +117,7 @@ _SIGS = {
     'dd_timer_enable': (C.c_int, [C.c_int, C.c_int]),
     'dd_timer_reset': (C.c_int, []),
     'dd_timer_read': (C.c_int, [C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
+    'dd_timer_flops': (C.c_int, [C.c_int, _dp]),
 }
 K_WILSON, K_COARSE, K_SAP0, K_SAPC, K_MULTIDOT, K_MULTIAXPY, K_RESTRICT, K_PROLONG, K_COARSEST, K_DOT = range(10)
 
@@ -135,6 +136,15 @@ def timer_read(kernel: int):
     t, n, b = C.c_double(), C.c_longlong(), C.c_double()
     _chk(lib.dd_timer_read(kernel, C.byref(t), C.byref(n), C.byref(b)))
     return t.value, n.value, b.value
+
+
+def timer_flops(kernel: int) -> float:
+    """Algorithmic floating-point operations the timed launches of `kernel` counted on
+    the device since the last reset (the coarse SAP's matrix-core products; 0 for the
+    other families)."""
+    f = C.c_double()
+    _chk(lib.dd_timer_flops(kernel, C.byref(f)))
+    return f.value
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.restype = _res

@@ -341,6 +341,11 @@ int dd_timer_reset(void);
  * launches (every operand counted once: links, test vectors and the vectors read or
  * written; re-reads from cache are not counted). */
 int dd_timer_read(int kernel, double* total_ms, long long* launches, double* bytes);
+/* Algorithmic floating-point operations of the timed launches of `kernel`, counted on the
+ * device by the kernels themselves: the matrix-core products of the coarse SAP (DD_K_SAPC:
+ * 128 dof^2 per site, hop and tile of 16 systems, for the block operators, the fused
+ * right-hand sides and the norm stencil); 0 for the other families.  Synchronises. */
+int dd_timer_flops(int kernel, double* flops);
 
 /* Last error message (thread-unsafe, per process). */
 const char* dd_last_error(void);
