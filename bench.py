@@ -300,7 +300,8 @@ def main():
         log('coarsest solver per plain solve: ' + json.dumps(coarsest_plain))
         ctx.close()
 
-    secondary, field256 = (None, None) if args.no_secondary else secondary_rows(ddamg, torch, args.sweeps, args.seed)
+    secondary, field256, ragged = ((None, None, None) if args.no_secondary
+                                   else secondary_rows(ddamg, torch, args.sweeps, args.seed))
 
     # ---- CPU baseline: the reference's own solve on the host cores --------------------------
     cpu = None
@@ -308,6 +309,8 @@ def main():
         cpu = cpu_reference(theta, B[0], N, m0, args.cpu_ranks)
         if field256 is not None:   # the 256^2 plain FP64 configuration on the same host cores
             cpu['cfg_256'] = cpu_reference(field256['theta'], field256['b0'], 256, -0.18, args.cpu_ranks)
+        if ragged is not None:   # the ragged batch's fields, right-hand side 0 of each
+            cpu['ragged'] = cpu_ragged(ragged)
 
     ms_rhs = t_mixed * 1e3 / nrhs
     line = {
@@ -437,15 +440,16 @@ def secondary_rows(ddamg, torch, sweeps, seed):
     # ragged batch: 8 fields, 4 RHS each, mixed precision
     sizes = [64, 96, 128, 128, 192, 256, 256, 256]
     m0s = rng.uniform(-0.18, -0.10, len(sizes))
-    ctxs, bs, xs = [], [], []
+    ctxs, bs, xs, ragged = [], [], [], []
     t0 = time.perf_counter()
     for i, (N, m0) in enumerate(zip(sizes, m0s)):
         c = field_ctx(N, seed + 100 + i)
         c.setup(hierarchy_params(ddamg, N, 1, 4, float(m0)))
-        bb, _ = rhs(4, N)
+        bb, Bh = rhs(4, N)
         ctxs.append(c)
         bs.append(bb)
         xs.append(torch.zeros_like(bb))
+        ragged.append({'N': N, 'm0': float(m0), 'theta': c.theta, 'b0': Bh[0]})
     t_setup = time.perf_counter() - t0
     args = ([4] * len(sizes), [v.data_ptr() for v in bs], [v.data_ptr() for v in xs])
     ddamg.solve_many_dev(ctxs, *args)   # warm-up
@@ -462,7 +466,7 @@ def secondary_rows(ddamg, torch, sweeps, seed):
         f'its {[i.iterations for i in infos]}')
     for c in ctxs:
         c.close()
-    return out, field256
+    return out, field256, ragged
 
 
 def sources_sha256(files):
@@ -524,6 +528,29 @@ def cpu_reference(theta, b, N, m0, ranks_s):
             'iterations': res['iterations'], 'relres': res['relres'], 'setup_s': res['setup_s'],
             'host_cpus_visible': os.cpu_count(), 'host_cpus_affinity': affinity,
             'host_cpu_share': os.environ.get('OMP_NUM_THREADS')}
+
+
+# MPI rank grids for the reference on the ragged sizes: the reference colours its SAP
+# blocks per rank, so every rank's level-0 and level-1 sub-lattices must hold an even
+# number of 4x4 blocks in each direction
+RAGGED_RANKS = {64: '2x2', 96: '3x3', 128: '4x4', 192: '3x3', 256: '4x4'}
+
+
+def cpu_ragged(fields):
+    """The reference's solve of right-hand side 0 of each ragged field, one after the other
+    (FP64, the same field, m0 and hierarchy parameters), each on its rank grid."""
+    rows = [cpu_reference(f['theta'], f['b0'], f['N'], f['m0'], RAGGED_RANKS[f['N']]) for f in fields]
+    if any('error' in r for r in rows):
+        return {'error': [r.get('error') for r in rows]}
+    tot = sum(r['value'] for r in rows)
+    return {'value': tot / len(rows), 'unit': 'ms/RHS', 'kind': 'reference',
+            'cores': max(r['cores'] for r in rows),
+            'sample': 'right-hand side 0 of each of the 8 ragged fields, solved one after the other by the '
+                      'reference under mpirun on the rank grids ' +
+                      ', '.join(f"{f['N']}^2: {RAGGED_RANKS[f['N']]}" for f in fields[:5]) +
+                      '; the value is the mean solve time per field (setup excluded)',
+            'ms_total': tot, 'per_field_ms': [r['value'] for r in rows],
+            'iterations': [r['iterations'] for r in rows]}
 
 
 if __name__ == '__main__':
