@@ -328,6 +328,110 @@ __device__ __forceinline__ void coarse_site_lds(const R* Ls, const cplx<R>* __re
     }
 }
 
+// coarse_site_lds in two phases, for a caller that waits for its neighbours in between:
+// PHASE 0 clears acc and accumulates the hops whose neighbour site lies inside the slice
+// [s0, s1) (rows the work-group wrote itself), PHASE 1 adds the other hops and runs the
+// epilogue.  The caller's slices are runs of at least two sites of one lattice row, so a
+// phase has at most three hops: they are gathered into three slots (shifted in, so that
+// every register index stays static), the last selected hop first.
+template <typename R, int DOF, int PHASE, typename Epi>
+__device__ __forceinline__ void coarse_site_lds_phase(const R* Ls, const cplx<R>* __restrict__ in,
+                                                      cplx<R>* __restrict__ out, int Nx, int Nt, R m0, uint64_t mask,
+                                                      size_t ld, int nrhs, int n, int tile, int lane, int s0, int s1,
+                                                      typename Mfma<R>::acc_t (&acc)[DOF / 8], const Epi& epi) {
+    constexpr int MT = DOF / 8, KS = DOF / 2;
+    static_assert(sizeof(R) == 4 && KS % 4 == 0, "FP32, K-steps in groups of four reals");
+    const int g = lane >> 4;
+    const int jj = tile * 16 + (lane & 15);
+    const bool on = jj < nrhs && ((mask >> jj) & 1ull);
+    const size_t sysoff = (size_t)(on ? jj : 0) * ld;
+    const R* vin = reinterpret_cast<const R*>(in + sysoff);
+    const int x = n / Nt, t = n - x * Nt;
+    const int nb[5] = {n, (t == Nt - 1) ? n - (Nt - 1) : n + 1, (x == Nx - 1) ? t : n + Nt,
+                       (t == 0) ? n + (Nt - 1) : n - 1, (x == 0) ? (Nx - 1) * Nt + t : n - Nt};
+    const R sg[5] = {R(1), (t == Nt - 1) ? R(-1) : R(1), R(1), (t == 0) ? R(-1) : R(1), R(1)};
+    // the phase's hops: neighbour site, boundary sign, hop index per slot
+    int cn0 = n, cn1 = n, cn2 = n, ch0 = 0, ch1 = 0, ch2 = 0, nh = 0;
+    R cg0 = R(0), cg1 = R(0), cg2 = R(0);
+#pragma unroll
+    for (int h = 0; h < 5; h++) {
+        const bool local = nb[h] >= s0 && nb[h] < s1;
+        if ((local ? 0 : 1) == PHASE) {
+            cn2 = cn1;
+            cn1 = cn0;
+            cn0 = nb[h];
+            cg2 = cg1;
+            cg1 = cg0;
+            cg0 = sg[h];
+            ch2 = ch1;
+            ch1 = ch0;
+            ch0 = h;
+            ++nh;
+        }
+    }
+    if constexpr (PHASE == 0) {
+#pragma unroll
+        for (int tt = 0; tt < MT; tt++) acc[tt] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+    }
+    R bv[3][KS];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (c >= nh) continue;
+        const int nbc = c == 0 ? cn0 : c == 1 ? cn1 : cn2;
+        const float4* vb = reinterpret_cast<const float4*>(vin + (size_t)nbc * DOF * 2 + KS * g);
+#pragma unroll
+        for (int s4 = 0; s4 < KS / 4; s4++) {
+            const float4 v4 = vb[s4];
+            bv[c][4 * s4] = v4.x;
+            bv[c][4 * s4 + 1] = v4.y;
+            bv[c][4 * s4 + 2] = v4.z;
+            bv[c][4 * s4 + 3] = v4.w;
+        }
+    }
+    const size_t base = sysoff + (size_t)n * DOF;
+    cplx<R> own[MT][2];
+    if constexpr (PHASE == 1) {
+#pragma unroll
+        for (int tt = 0; tt < MT; tt++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) own[tt][e] = in[base + 8 * tt + 2 * g + e];
+    }
+    const int i = lane & 15, p = i & 1;
+    const int lbase = 2 * ((KS / 2) * g * DOF + (i >> 1));
+    const int oe = lbase + p, oo = lbase + (1 - p);
+    const bool negodd = p == 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (c >= nh) continue;
+        const int hc = c == 0 ? ch0 : c == 1 ? ch1 : ch2;
+        const R gc = c == 0 ? cg0 : c == 1 ? cg1 : cg2;
+        const R* Lh = Ls + (size_t)hc * DOF * DOF * 2;
+#pragma unroll
+        for (int s2 = 0; s2 < KS; s2++) {
+            const R bb = on ? gc * bv[c][s2] : R(0);
+#pragma unroll
+            for (int tt = 0; tt < MT; tt++) {
+                const int cc = 2 * ((s2 >> 1) * DOF + 8 * tt);
+                R av = (s2 & 1) ? Lh[oo + cc] : Lh[oe + cc];
+                if ((s2 & 1) && negodd) av = -av;
+                acc[tt] = Mfma<R>::mac(av, bb, acc[tt]);
+            }
+        }
+    }
+    if constexpr (PHASE == 1) {
+        if (!on) return;
+        const R mass = m0 + R(2);
+#pragma unroll
+        for (int tt = 0; tt < MT; tt++) {
+            const int r0 = 8 * tt + 2 * g;
+            const cplx<R> o0 = mk<R>(mass * own[tt][0].x - acc[tt][0], mass * own[tt][0].y - acc[tt][1]);
+            const cplx<R> o1 = mk<R>(mass * own[tt][1].x - acc[tt][2], mass * own[tt][1].y - acc[tt][3]);
+            epi(out, base + r0, own[tt][0], o0);
+            epi(out, base + r0 + 1, own[tt][1], o1);
+        }
+    }
+}
+
 __device__ __forceinline__ cplx<double> cdivr(cplx<double> a, double d) { return mk<double>(a.x / d, a.y / d); }
 
 // Sums NV values (a power of two, 4 <= NV <= 32) over each group of LPS lanes (32 or 64)
@@ -571,6 +675,26 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
                     coarse_site<R, DOF, false, 1>(a.L, in, out, nullptr, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl,
                                                   lane, epi);
             }
+    };
+    // The polynomial stages' sweeps in two phases around the neighbour wait (one
+    // (site, tile) task per wave: FP32 slices of at most 4 sites with their links in LDS):
+    // phase 0 the hops inside the slice, phase 1 the others and the epilogue; pacc holds a
+    // wave's partial products in between.
+    typename Mfma<R>::acc_t pacc[DOF / 8];
+    bool split = false;
+    if constexpr (POLY && LPS == 32 && sizeof(R) == 4 && DOF <= 24)
+        split = a.lds_links && (s1 - s0) * ntile <= kNW && a.spw >= 2 && a.Nt % a.spw == 0 && a.Nx >= 2;
+    auto stencil_phase = [&](auto phase_, const cplx<R>* in, cplx<R>* out, uint64_t mk2, const auto& epi) {
+        if constexpr (POLY && LPS == 32 && sizeof(R) == 4 && DOF <= 24) {
+            constexpr int PH = decltype(phase_)::value;
+            const int ns = s1 - s0;
+            if (wv >= ns * ntile) return;
+            const int site = s0 + wv % ns, tl = wv / ns;
+            if (!((mk2 >> (16 * tl)) & 0xffffull)) return;
+            const R* Ls = lnk + (size_t)(site - s0) * 5 * DOF * DOF * 2;
+            coarse_site_lds_phase<R, DOF, PH>(Ls, in, out, a.Nx, a.Nt, a.m0, mk2, n, a.nsys, site, tl, lane, s0, s1,
+                                              pacc, epi);
+        }
     };
     auto load_rows = [&](cplx<R> (&r)[RPL], const cplx<R>* vec) {   // vec: system base + row0
 #pragma unroll
@@ -846,18 +970,25 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
     // after its neighbours have published the next stage, which follows their reads of
     // them).  false when a wait timed out.
     unsigned zst = 0;   // stages published in this launch (the same count in every work-group)
+    // With `split`, the caller has run phase 0 of the first sweep (on u_j) before its wait
+    // for the neighbours' update flags, and every later sweep's phase 0 runs between the
+    // publication of its input and the wait for the neighbours' copies of it.
     auto poly_stages = [&](uint64_t mk2) -> bool {
         const cplx<R>* zin = a.u;
         for (int i = 0; i < npoly; i++) {
             const bool fin = i == npoly - 1;
             cplx<R>* zout = fin ? a.w : zbuf(i + 1);
-            stencil_e(zin, zout, mk2, PolyEpi<R>{fin ? a.u : nullptr, nullptr, a.pc[i], czero<R>()});
+            const PolyEpi<R> pe{fin ? a.u : nullptr, nullptr, a.pc[i], czero<R>()};
+            if (split) stencil_phase(std::integral_constant<int, 1>{}, zin, zout, mk2, pe);
+            else stencil_e(zin, zout, mk2, pe);
             if (fin) break;
             ++zst;
             release_stores();
             __syncthreads();
             if (tid == 0)
                 __hip_atomic_store(&a.bar->zdone[wg * kFlagStride], zst, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // this work-group's rows of z_{i+1} are complete and visible to its waves
+            if (split) stencil_phase(std::integral_constant<int, 0>{}, zout, nullptr, mk2, pe);
             wait_nb(a.bar->zdone, zst);
             if (!si[2]) return false;
             zin = zout;
@@ -1145,7 +1276,14 @@ __global__ __launch_bounds__(kCT) void k_coarsest(CArgs<R, S> a) {
             const int nr = __builtin_popcountll(run);
             const bool hw = j < m;
             // the rows of u_j the stencil reads were written by the neighbours' update of
-            // the previous step (j = 0: before the last grid barrier)
+            // the previous step (j = 0: before the last grid barrier); the hops inside the
+            // slice read only this work-group's rows and may run before the wait
+            if constexpr (POLY) {   // (cleared every step, so no value is carried across the passes below)
+#pragma unroll
+                for (int tt = 0; tt < DOF / 8; tt++) pacc[tt] = typename Mfma<R>::acc_t{0, 0, 0, 0};
+            }
+            if (hw && npoly && split)
+                stencil_phase(std::integral_constant<int, 0>{}, a.u, nullptr, run, PolyEpi<R>{nullptr, nullptr, czero<R>(), czero<R>()});
             if (j > 0) {
                 wait_neighbours(gstep);
                 if (!si[2]) return;
