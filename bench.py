@@ -547,7 +547,7 @@ def cpu_ragged(fields):
             'cores': max(r['cores'] for r in rows),
             'sample': 'right-hand side 0 of each of the 8 ragged fields, solved one after the other by the '
                       'reference under mpirun on the rank grids ' +
-                      ', '.join(f"{f['N']}^2: {RAGGED_RANKS[f['N']]}" for f in fields[:5]) +
+                      ', '.join(f'{N}^2: {RAGGED_RANKS[N]}' for N in sorted({f['N'] for f in fields})) +
                       '; the value is the mean solve time per field (setup excluded)',
             'ms_total': tot, 'per_field_ms': [r['value'] for r in rows],
             'iterations': [r['iterations'] for r in rows]}
