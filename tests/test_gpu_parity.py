@@ -390,8 +390,8 @@ def test_vcycle_graph_equals_eager(golden, precision):
         assert rr < 1e-10
 
 
-@pytest.mark.parametrize('precision', [0, 1])
-def test_kcycle_graph_equals_eager(golden, precision):
+@pytest.mark.parametrize('precision,poly', [(0, 0), (1, 0), (1, 8)])
+def test_kcycle_graph_equals_eager(golden, precision, poly):
     """K-cycle preconditioner captured as a HIP graph (graph = 1, cycle = 1) and replayed once
     per outer FGMRES iteration.  Inside the capture the K-cycle's FGMRES(5) x 2 solves at
     levels 1 and 2 run device-controlled: the Givens rotations, breaks and restart tests of
@@ -401,7 +401,8 @@ def test_kcycle_graph_equals_eager(golden, precision):
     K-cycles' (host-controlled FGMRES) and the solutions agree to rounding (the rotations
     are the same formulas evaluated on the device; FP64 1e-10, mixed 1e-6); a second solve
     replays the cached graphs bit for bit; every system meets the outer tolerance with the
-    oracle's D_W."""
+    oracle's D_W.  The same with the coarsest polynomial preconditioner (poly = 8: the
+    persistent kernel's polynomial variant as a node of the captured graph)."""
     meta, a = golden('g64_solve')
     rng = np.random.default_rng(10)
     B = np.stack([a['rhs']] + [rng.standard_normal(a['rhs'].shape) + 1j * rng.standard_normal(a['rhs'].shape)
@@ -412,7 +413,7 @@ def test_kcycle_graph_equals_eager(golden, precision):
         with _ctx_from_golden(meta, a) as ctx:
             ctx.setup(ddamg.params(3, agg=[(4, 4), (2, 2)], sap_block=[(4, 4), (2, 2)], nv=[8, 8], m0=meta['m0'],
                                    rng_ranks=tuple(meta['ranks']), nu2=meta['nu2'], fgmres_m=meta['fgmres_m'],
-                                   cycle=1, precision=precision, max_rhs=4, graph=graph))
+                                   cycle=1, precision=precision, max_rhs=4, graph=graph, coarse_poly=poly))
             x, its, info = ctx.solve(B)
             if graph:
                 assert ctx.graph_count() >= 1
@@ -420,7 +421,8 @@ def test_kcycle_graph_equals_eager(golden, precision):
                 assert list(its2) == list(its) and np.array_equal(x2, x)
             out[graph] = (x, list(its))
     d = rel(out[1][0], out[0][0])
-    print(f'precision {precision}: eager iterations {out[0][1]}, graph {out[1][1]}, solution difference {d:.2e}')
+    print(f'precision {precision} poly {poly}: eager iterations {out[0][1]}, graph {out[1][1]}, '
+          f'solution difference {d:.2e}')
     assert out[1][1] == out[0][1]
     assert d < (1e-10 if precision == 0 else 1e-6)
     for r in range(4):
