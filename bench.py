@@ -2,9 +2,11 @@
 
 Metric: ms per right-hand side to reach 1e-10 relative residual (true FP64 D_W
 residual) at 512^2, beta=2, m0=-0.18, 3 levels, 4x4 aggregates, Nv=12, 4x4 SAP
-blocks, K-cycle, outer FGMRES(30).  The measured form is the mixed-precision batched
-solve (FP32 hierarchy inside the FP64 outer FGMRES, a batch of RHS per launch); the
-plain FP64 single-RHS solve of the same system is timed beside it.
+blocks, outer FGMRES(30).  The measured form is the mixed-precision batched solve (FP32
+hierarchy inside the FP64 outer FGMRES, a batch of RHS per launch) with the V-cycle and
+the coarsest GMRES's degree-12 polynomial preconditioner; the same batches are timed
+with the K-cycle and without the polynomial beside it, and the plain FP64 single-RHS
+solve of the same system with the reference's K-cycle (and the V-cycle).
 
 A step is one batched solve of --nrhs Gaussian right-hand sides already resident in
 HBM; every timed step solves its own batch (seeded), so the mean covers the spread of
@@ -38,13 +40,17 @@ SAP_LINKS = None      # None: the library default; 0 FP32, 1 binary16 (--coarse-
 BOOTSTRAP = 0         # bootstrap passes after the setup (--bootstrap; 0 = the reference's setup)
 SAP_FUSED = None      # None: the library default; 0 separate launches per colour, 1 one launch per iteration
 GRAPH = 0             # 1: the cycles replayed from captured HIP graphs (--graph)
+CYCLE = 0             # cycle of the mixed rows (--cycle): 0 V-cycle (FGMRES_AMG_v_cycle), 1 K-cycle
+                      # (FGMRES_AMG_k_cycle); the plain FP64 rows use the K-cycle unless asked
 COARSE_POLY = 12      # degree of the coarsest GMRES's polynomial preconditioner in the FP32 (mixed)
                       # hierarchies (--coarse-poly; 0: the reference's plain GMRES); FP64 rows keep plain GMRES
 
 
-def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18, coarse_poly=None):
+def hierarchy_params(ddamg, N, precision, nrhs, m0=-0.18, coarse_poly=None, cycle=None):
+    if cycle is None:
+        cycle = CYCLE if precision == 1 else 1
     p = ddamg.params(3, agg=[(4, 4), (4, 4)], sap_block=[(4, 4), (4, 4)], nv=[12, 12], m0=m0, nu2=2,
-                     fgmres_m=30, cycle=1, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
+                     fgmres_m=30, cycle=cycle, precision=precision, max_rhs=nrhs, rng_ranks=(4, 4))
     if COARSE_BASIS is not None:
         p.coarse_basis = COARSE_BASIS
     if SAP_LINKS is not None:
@@ -92,13 +98,17 @@ def main():
     ap.add_argument('--coarse-poly', type=int, default=12,
                     help='degree of the GMRES polynomial right preconditioner of the coarsest solve in the mixed '
                          'hierarchies (0: the plain GMRES of the reference, which the FP64 rows always use)')
+    ap.add_argument('--cycle', type=int, choices=[0, 1], default=0,
+                    help='cycle of the mixed rows: 0 V-cycle (default), 1 K-cycle (the plain FP64 rows run the '
+                         'K-cycle and, beside it, the V-cycle)')
     ap.add_argument('--skip-no-poly', action='store_true',
                     help='skip the row that repeats the batches without the coarsest polynomial preconditioner')
     ap.add_argument('--setup-only', action='store_true',
                     help='only generate the field and build the mixed and the FP64 hierarchies (setup profiles)')
     args = ap.parse_args()
-    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED, GRAPH, COARSE_POLY
+    global COARSE_BASIS, SAP_LINKS, BOOTSTRAP, SAP_FUSED, GRAPH, COARSE_POLY, CYCLE
     BOOTSTRAP = args.bootstrap
+    CYCLE = args.cycle
     COARSE_POLY = args.coarse_poly
     GRAPH = args.graph
     SAP_FUSED = args.sap_fused
@@ -254,27 +264,34 @@ def main():
                       'max_relres': info32.relres,
                       'ms_per_outer_iteration': t32 * 1e3 / max(int(its32.max()), 1)}
         log(f'mixed, FP32 coarsest basis: {t32 * 1e3:.1f} ms per solve, its {its32.min()}..{its32.max()}')
-    # the same batches without the coarsest polynomial preconditioner (the reference's plain
-    # coarsest GMRES inside the same mixed hierarchy)
-    no_poly = None
-    if not args.skip_mixed and not args.skip_no_poly and COARSE_POLY > 0:
-        ctx.setup(hierarchy_params(ddamg, N, 1, nrhs, m0, coarse_poly=0))
+    def repeat(p, what):
+        # the same batches (warm-up on batch 0, then every batch timed) with hierarchy p
+        ctx.setup(p)
         xs = [torch.zeros_like(v) for v in b_devs]
         ctx.solve_dev(b_devs[0].data_ptr(), xs[0].data_ptr(), nrhs)   # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res0 = [ctx.solve_dev(b_devs[k].data_ptr(), xs[k].data_ptr(), nrhs) for k in range(args.steps)]
+        res = [ctx.solve_dev(b_devs[k].data_ptr(), xs[k].data_ptr(), nrhs) for k in range(args.steps)]
         torch.cuda.synchronize()
         tn = (time.perf_counter() - t0) / max(args.steps, 1)
-        per = [int(r[0].max()) for r in res0]
-        no_poly = {'ms_per_rhs': tn * 1e3 / nrhs, 'outer_iterations_per_batch': per,
-                   'ms_per_outer_iteration': tn * 1e3 * args.steps / max(sum(per), 1),
-                   'max_relres': max(r[1].relres for r in res0)}
-        log(f'mixed without the coarsest polynomial: {tn * 1e3:.1f} ms per solve, per batch {per}')
+        per = [int(r[0].max()) for r in res]
+        log(f'mixed, {what}: {tn * 1e3:.1f} ms per solve, per batch {per}')
+        return {'ms_per_rhs': tn * 1e3 / nrhs, 'outer_iterations_per_batch': per,
+                'ms_per_outer_iteration': tn * 1e3 * args.steps / max(sum(per), 1),
+                'max_relres': max(r[1].relres for r in res)}
+    # the same batches without the coarsest polynomial preconditioner (the reference's plain
+    # coarsest GMRES inside the same mixed hierarchy), and with the other cycle
+    no_poly = other_cycle = None
+    if not args.skip_mixed and not args.skip_no_poly and COARSE_POLY > 0:
+        no_poly = repeat(hierarchy_params(ddamg, N, 1, nrhs, m0, coarse_poly=0),
+                         'without the coarsest polynomial')
+    if not args.skip_mixed and not args.skip_no_poly:
+        other_cycle = repeat(hierarchy_params(ddamg, N, 1, nrhs, m0, cycle=1 - CYCLE),
+                             'K-cycle' if CYCLE == 0 else 'V-cycle')
     ctx.close()
 
     # ---- plain FP64, one RHS -------------------------------------------------------------
-    t_plain, its_plain, err_vs_fp64, setup_plain, coarsest_plain = None, None, None, None, None
+    t_plain, its_plain, err_vs_fp64, setup_plain, coarsest_plain, plain_v = None, None, None, None, None, None
     if not args.skip_plain:
         ctx = ddamg.Context(N, N)
         ctx.set_gauge_theta(theta)
@@ -298,6 +315,18 @@ def main():
         nsolve = 3 if args.skip_mixed else 4
         coarsest_plain = {k: (v / nsolve if k not in ('persistent',) else v) for k, v in cst.items()}
         log('coarsest solver per plain solve: ' + json.dumps(coarsest_plain))
+        # the plain FP64 solve with the V-cycle as well (median of 3 after a warm-up)
+        ctx.setup(hierarchy_params(ddamg, N, 0, 1, m0, cycle=0))
+        ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
+        torch.cuda.synchronize()
+        vt = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            its_pv, info_pv = ctx.solve_dev(b_dev.data_ptr(), xp.data_ptr(), 1)
+            torch.cuda.synchronize()
+            vt.append(time.perf_counter() - t0)
+        plain_v = {'ms_per_rhs': float(np.median(vt)) * 1e3, 'iterations': int(its_pv[0]), 'relres': info_pv.relres}
+        log(f'plain FP64, V-cycle: {plain_v["ms_per_rhs"]:.1f} ms, its {its_pv[0]}, relres {info_pv.relres:.3e}')
         ctx.close()
 
     secondary, field256, ragged = ((None, None, None) if args.no_secondary
@@ -319,7 +348,7 @@ def main():
         'ms_per_step': t_mixed * 1e3, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'c64 hierarchy (FP32 SAP, coarse operators, test vectors) with a binary16-stored coarsest Krylov '
                  'basis (products in FP32), c128 outer FGMRES and residual', 'data': 'synthetic (seeded Metropolis gauge field, Gaussian RHS)',
-        'config': {'workload': f'DD-alphaAMG K-cycle solve, {N}^2 Schwinger Wilson-Dirac, beta=2, m0={m0}, '
+        'config': {'workload': f'DD-alphaAMG {"K" if CYCLE else "V"}-cycle solve, {N}^2 Schwinger Wilson-Dirac, beta=2, m0={m0}, '
                                f'3 levels, 4x4 aggregates, Nv=12, 4x4 SAP blocks, FGMRES(30), tol 1e-10, '
                                f'{nrhs} RHS per solve; coarsest GMRES(400) to 0.1 with a degree-{COARSE_POLY} '
                                f'GMRES polynomial right preconditioner (0: none)',
@@ -329,16 +358,19 @@ def main():
         'ms_per_outer_iteration': t_mixed * 1e3 * args.steps / max(sum(outer_per_step), 1),
         'mixed_fp32_coarsest_basis': fp32_basis,
         'mixed_no_coarse_poly': no_poly,
+        ('mixed_kcycle' if CYCLE == 0 else 'mixed_vcycle'): other_cycle,
         'max_relres': float(relres.max()),
         'setup_s_mixed': setup_mixed,
         'autotuned_coarse_sap_variant': tune[0],
         'plain_fp64_ms_per_rhs': None if t_plain is None else t_plain * 1e3,
         'plain_fp64_ms_samples': None if t_plain is None else [t * 1e3 for t in plain_times],
         'plain_fp64_iterations': None if its_plain is None else int(its_plain[0]),
+        'plain_fp64_vcycle': plain_v,
         'setup_s_fp64': setup_plain,
         'speedup_vs_plain_fp64': None if t_plain is None else t_plain * 1e3 / ms_rhs,
         'speedup_vs_plain_fp64_no_coarse_poly': None if (t_plain is None or no_poly is None)
         else t_plain * 1e3 / no_poly['ms_per_rhs'],
+        'speedup_vs_plain_fp64_vcycle': None if plain_v is None else plain_v['ms_per_rhs'] / ms_rhs,
         'mixed_vs_fp64_rel_error': err_vs_fp64,
         'cpu_baseline': cpu,
         'roofline': roofline,
@@ -406,7 +438,7 @@ def secondary_rows(ddamg, torch, sweeps, seed):
     field256 = {'theta': ctx.theta, 'b0': B[0]}
     x = torch.zeros_like(b)
     for prec, nr in ((0, 1), (1, 32)):
-        ctx.setup(hierarchy_params(ddamg, 256, prec, nr, -0.18))
+        ctx.setup(hierarchy_params(ddamg, 256, prec, nr, -0.18, cycle=1))
         ctx.solve_dev(b.data_ptr(), x.data_ptr(), nr)   # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
