@@ -14,8 +14,9 @@ the library's fixed start vector, and returns x0 + p(D_c) Q y.  Checked here:
   0.1 ||eta|| for every system, with fewer Arnoldi steps than without the preconditioner;
 - 3-level K-cycle solves (FP64 and mixed, 4 systems) reach 1e-10 in the FP64 residual and
   agree with the solves without the preconditioner to 1e-8;
-- at 512^2 on the fixture's field, the mixed 32-system batch with coarse_poly = 8 reaches
-  1e-10 for every system, and system 0 equals the FP64 solve to 1e-8;
+- at 512^2 on the fixture's field, the mixed 32-system batch (K-cycle with coarse_poly = 8,
+  V-cycle with coarse_poly = 12) reaches 1e-10 for every system, and system 0 equals the
+  FP64 solve to 1e-8;
 - at 512^2 (the shape whose polynomial sweeps run their in-slice hops before the
   neighbour wait) tight-tolerance coarsest solves reach the same FP64 residual and the
   same distance to a sparse LU solve of the downloaded coarsest operator as the plain
@@ -135,7 +136,9 @@ def test_kcycle_solve_with_poly(precision):
     assert d < 1e-8
 
 
-def test_512_mixed_batch_with_poly(golden):
+@pytest.mark.parametrize('cycle,poly', [(1, 8), (0, 12)])
+def test_512_mixed_batch_with_poly(golden, cycle, poly):
+    """K-cycle with degree 8, and the V-cycle with degree 12 (the benchmark's solve)."""
     meta, a = golden('g512_solve')
     N, m0 = meta['N'], meta['m0']
     lv = meta['levels']
@@ -151,14 +154,16 @@ def test_512_mixed_batch_with_poly(golden):
         assert hashlib.sha256(np.ascontiguousarray(th).tobytes()).hexdigest() == meta['theta_sha256']
         ctx.setup(ddamg.params(len(lv) + 1, precision=0, max_rhs=1, **prm))
         x64, its64, _ = ctx.solve(b0)
-        ctx.setup(ddamg.params(len(lv) + 1, precision=1, max_rhs=32, coarse_poly=8, **prm))
+        prm['cycle'] = cycle
+        ctx.setup(ddamg.params(len(lv) + 1, precision=1, max_rhs=32, coarse_poly=poly, **prm))
         assert ctx.coarsest_stats()['persistent'] == 1
         xm, itm, info = ctx.solve(B)
         Dx = ctx.apply_D(xm, m0)
         rr = np.linalg.norm((B - Dx).reshape(32, -1), axis=1) / np.linalg.norm(B.reshape(32, -1), axis=1)
         st = ctx.coarsest_stats()
         err = rel(xm[0], x64)
-        print(f'512^2 mixed, 32 RHS, coarse_poly 8: iterations {itm.min()}..{itm.max()}, max relres {rr.max():.2e}, '
+        print(f'512^2 mixed, 32 RHS, {"K" if cycle else "V"}-cycle, coarse_poly {poly}: iterations '
+              f'{itm.min()}..{itm.max()}, max relres {rr.max():.2e}, '
               f'system 0 vs FP64 {err:.2e}, {st["steps"] / max(st["solves"], 1):.1f} Arnoldi steps per coarsest solve')
         assert rr.max() < 1e-10 and info.relres < 1e-10
         assert err < 1e-8
