@@ -146,10 +146,11 @@ void dot_partial(hipStream_t s, const cplx<R>* a, const cplx<R>* b, size_t n, si
 // out[r*ostride] = sum of partials (complex, precision R)
 template <typename R>
 void dot_final(hipStream_t s, const RedBuf& rb, cplx<double>* out, int ostride, int nrhs, uint64_t mask);
-// h = sum partials -> H[r*hstride]; w -= h * v
+// h = sum of in's partials -> H[r*hstride]; w -= h * v; partials of <w, nxt> (nxt null:
+// of |w|^2) -> out, the bits of the update pass followed by dot_partial(w, nxt)
 template <typename R>
-void mgs_update(hipStream_t s, cplx<R>* w, const cplx<R>* v, size_t n, size_t ld, int nrhs, uint64_t mask,
-                const RedBuf& rb, cplx<double>* H, int hstride);
+void mgs_update_dot(hipStream_t s, cplx<R>* w, const cplx<R>* v, const cplx<R>* nxt, size_t n, size_t ld, int nrhs,
+                    uint64_t mask, const RedBuf& in, RedBuf& out, cplx<double>* H, int hstride);
 // h = sqrt(sum partials.re) -> H[r*hstride]; if h > 0: out = (1/h) w
 template <typename R>
 void norm_scale(hipStream_t s, cplx<R>* out, const cplx<R>* w, size_t n, size_t ld, int nrhs, uint64_t mask,

@@ -627,16 +627,29 @@ void dot_final(hipStream_t s, const RedBuf& rb, cplx<double>* out, int ostride, 
     DD_LAUNCH_CHECK();
 }
 
+static inline unsigned ew_blocks(size_t n) { return (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (n + 255) / 256)); }
+
+// One modified Gram-Schmidt projection and the next one's partial sums in a single pass:
+// h = sum of the partials in `pin` -> H[r*hstride]; w -= h v; then the partials of
+// <w, nxt> (nxt = w: |w|^2) -> pout.  The threads walk the elements exactly as
+// k_dot_partial does and sum them with the same block_sum, so the partials, and every h
+// and norm formed from them, have the bits of an update pass w -= h v followed by
+// k_dot_partial.  pin and pout must not overlap (every block reads all of pin before any
+// block's write).
 template <typename R>
-__global__ __launch_bounds__(256) void k_mgs_update(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ v, size_t n,
-                                                    size_t ld, uint64_t mask, const cplx<double>* __restrict__ partial,
-                                                    int nblk, cplx<double>* H, int hstride, const uint64_t* __restrict__ dm) {
+__global__ __launch_bounds__(256) void k_mgs_update_dot(cplx<R>* __restrict__ w, const cplx<R>* __restrict__ v,
+                                                        const cplx<R>* __restrict__ nxt, size_t n, size_t ld,
+                                                        uint64_t mask, const cplx<double>* __restrict__ pin,
+                                                        cplx<double>* __restrict__ pout, cplx<double>* H, int hstride,
+                                                        const uint64_t* __restrict__ dm) {
     if (dm) mask &= *dm;
     __shared__ cplx<double> hs;
+    __shared__ R scratch[32];
     const int r = blockIdx.y;
     if (!rhs_on(mask, r)) return;
+    const int nblk = gridDim.x;
     if (threadIdx.x < 64) {
-        const cplx<double> h = sum_partials(partial + (size_t)r * nblk, nblk);
+        const cplx<double> h = sum_partials(pin + (size_t)r * nblk, nblk);
         if (threadIdx.x == 0) {
             hs = h;
             if (blockIdx.x == 0) H[(size_t)r * hstride] = h;
@@ -646,17 +659,28 @@ __global__ __launch_bounds__(256) void k_mgs_update(cplx<R>* __restrict__ w, con
     const cplx<R> h = mk<R>((R)hs.x, (R)hs.y);
     cplx<R>* wr = w + r * ld;
     const cplx<R>* vr = v + r * ld;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        wr[i] = wr[i] - h * vr[i];
+    const cplx<R>* nr = nxt ? nxt + r * ld : nullptr;
+    cplx<R> s = czero<R>();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)nblk * blockDim.x) {
+        const cplx<R> wv = wr[i] - h * vr[i];
+        wr[i] = wv;
+        s = s + mulc(wv, nr ? nr[i] : wv);
+    }
+    s = block_sum(s, scratch);
+    if (threadIdx.x == 0) pout[(size_t)r * nblk + blockIdx.x] = mk<double>(s.x, s.y);
 }
 
-static inline unsigned ew_blocks(size_t n) { return (unsigned)std::min<size_t>(1024, std::max<size_t>(1, (n + 255) / 256)); }
-
 template <typename R>
-void mgs_update(hipStream_t s, cplx<R>* w, const cplx<R>* v, size_t n, size_t ld, int nrhs, uint64_t mask,
-                const RedBuf& rb, cplx<double>* H, int hstride) {
-    dim3 grid(ew_blocks(n), nrhs);
-    k_mgs_update<R><<<grid, 256, 0, s>>>(w, v, n, ld, mask, (const cplx<double>*)rb.partial, rb.nblk, H, hstride, dev_mask());
+void mgs_update_dot(hipStream_t s, cplx<R>* w, const cplx<R>* v, const cplx<R>* nxt, size_t n, size_t ld, int nrhs,
+                    uint64_t mask, const RedBuf& in, RedBuf& out, cplx<double>* H, int hstride) {
+    if (in.partial == out.partial) throw BadShape("mgs_update_dot: partial buffers must differ");
+    if (in.nblk != red_blocks(n)) throw BadShape("mgs_update_dot: input partials from another length");
+    timer_begin(s, K_DOT);
+    out.nblk = in.nblk;
+    dim3 grid(in.nblk, nrhs);
+    k_mgs_update_dot<R><<<grid, 256, 0, s>>>(w, v, nxt, n, ld, mask, (const cplx<double>*)in.partial,
+                                             (cplx<double>*)out.partial, H, hstride, dev_mask());
+    timer_end(s, K_DOT, nact(mask, nrhs) * (double)n * (nxt ? 4 : 3) * sizeof(cplx<R>));
     DD_LAUNCH_CHECK();
 }
 
@@ -1132,8 +1156,8 @@ void dtheta_contract(hipStream_t s, const cplx<double>* U, const cplx<double>* y
     template void dot_partial<R>(hipStream_t, const cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,        \
                                  RedBuf&);                                                                          \
     template void dot_final<R>(hipStream_t, const RedBuf&, cplx<double>*, int, int, uint64_t);                      \
-    template void mgs_update<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,               \
-                                const RedBuf&, cplx<double>*, int);                                                 \
+    template void mgs_update_dot<R>(hipStream_t, cplx<R>*, const cplx<R>*, const cplx<R>*, size_t, size_t, int,     \
+                                    uint64_t, const RedBuf&, RedBuf&, cplx<double>*, int);                          \
     template void norm_scale<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,               \
                                 const RedBuf&, cplx<double>*, int);                                                 \
     template void scale_inv<R>(hipStream_t, cplx<R>*, const cplx<R>*, size_t, size_t, int, uint64_t,                \

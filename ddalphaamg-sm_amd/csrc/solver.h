@@ -57,7 +57,8 @@ class Krylov {
     }
 
    private:
-    void ortho(int j, cplx<R>* w, uint64_t run, int nr);
+    // orthogonalises w against V_0..V_j (H column j); returns the partial sums of |w|^2
+    RedBuf ortho(int j, cplx<R>* w, uint64_t run, int nr);
     hipStream_t s_;
     size_t n_;
     int m_, restarts_, cap_, ortho_, m_max_;

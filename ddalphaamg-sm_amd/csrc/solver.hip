@@ -91,21 +91,29 @@ void Krylov<R>::alloc_ctl() {
 }
 
 template <typename R>
-void Krylov<R>::ortho(int j, cplx<R>* w, uint64_t run, int nr) {
+RedBuf Krylov<R>::ortho(int j, cplx<R>* w, uint64_t run, int nr) {
     const size_t vs = n_ * cap_;
     cplx<double>* Hj = H_.as<cplx<double>>() + (size_t)j * cap_ * (m_ + 1);
     const cplx<R>* V = V_.as<cplx<R>>();
     if (ortho_ == 0) {
-        for (int i = 0; i <= j; i++) {
-            dot_partial<R>(s_, w, V + i * vs, n_, n_, nr, run, rb_);
-            mgs_update<R>(s_, w, V + i * vs, n_, n_, nr, run, rb_, Hj + i, m_ + 1);
-        }
-    } else {
-        multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
-        multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, false);
-        multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
-        multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, true);
+        // modified Gram-Schmidt (src/fgmres.cpp:27-38): projection i updates w and forms
+        // the partial sums of projection i + 1 (of |w|^2 after the last) in the same pass;
+        // the partials alternate between two slots of part_ ([cap][nblk] each)
+        const size_t nblk = std::min<size_t>(kMaxRedBlocks, std::max<size_t>(1, (n_ + 2047) / 2048));
+        RedBuf pb[2] = {rb_, rb_};
+        pb[1].partial = reinterpret_cast<cplx<double>*>(rb_.partial) + (size_t)cap_ * nblk;   // m_ >= 1: 2 slots fit
+        dot_partial<R>(s_, w, V, n_, n_, nr, run, pb[0]);
+        for (int i = 0; i <= j; i++)
+            mgs_update_dot<R>(s_, w, V + i * vs, i < j ? V + (i + 1) * vs : nullptr, n_, n_, nr, run, pb[i & 1],
+                              pb[(i + 1) & 1], Hj + i, m_ + 1);
+        return pb[(j + 1) & 1];
     }
+    multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
+    multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, false);
+    multidot<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_);
+    multi_axpy<R>(s_, w, V, vs, j + 1, n_, n_, nr, run, rb_, Hj, m_ + 1, true);
+    dot_partial<R>(s_, w, w, n_, n_, nr, run, rb_);
+    return rb_;
 }
 
 template <typename R>
@@ -169,10 +177,9 @@ uint64_t Krylov<R>::solve(const cplx<R>* phi, const cplx<R>* x0, cplx<R>* x, uin
             cplx<R>* Zj = Z + j * vs;
             if (flex_) (*M)(Vj, Zj, run);
             A(Zj, w, run);
-            ortho(j, w, run, nr);
+            const RedBuf wn = ortho(j, w, run, nr);
             cplx<double>* Hj = H_.as<cplx<double>>() + (size_t)j * cap_ * (m_ + 1);
-            dot_partial<R>(s_, w, w, n, n, nr, run, rb_);
-            norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, run, rb_, Hj + (j + 1), m_ + 1);
+            norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, run, wn, Hj + (j + 1), m_ + 1);
             DD_HIP(hipMemcpyAsync(Hh, Hj, sizeof(cplx<double>) * (m_ + 1) * nr, hipMemcpyDeviceToHost, s_));
             DD_HIP(hipStreamSynchronize(s_));
             for (int q = 0; q < nr; q++) {
@@ -426,9 +433,8 @@ void Krylov<R>::solve_dc(const cplx<R>* phi, cplx<R>* x, uint64_t mask, const Fu
                 cplx<R>* Zj = Z + j * vs;
                 if (flex_) (*M)(Vj, Zj, mask);
                 A(Zj, w, mask);
-                ortho(j, w, mask, nr);
-                dot_partial<R>(s_, w, w, n, n, nr, mask, rb_);
-                norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, mask, rb_, Hj + (j + 1), m_ + 1);
+                const RedBuf wn = ortho(j, w, mask, nr);
+                norm_scale<R>(s_, V + (j + 1) * vs, w, n, n, nr, mask, wn, Hj + (j + 1), m_ + 1);
             }
             k_kc_givens<<<1, 64, 0, s_>>>(v, Hj, m_ + 1, j, tol_);
             DD_HIP(hipGetLastError());
