@@ -10,7 +10,9 @@
   right-hand sides reaches 1e-10 in FP64 residual for every system, and its system 0
   is within 1e-8 of the FP64 solve.
 - 32 right-hand sides on one 256^2 field in mixed precision: every residual < 1e-10
-  and sampled systems equal their separate single-system solves to 1e-8.
+  and sampled systems equal their separate single-system solves to 1e-8 (the K-cycle
+  with the plain coarsest GMRES, and the V-cycle with the degree-12 coarsest
+  polynomial, as in the benchmark's rows; also for the ragged batch below).
 - The ragged batch of 8 fields {64,96,128,128,192,256,256,256}^2 with m0 drawn from
   U[-0.18, -0.10] and 4 right-hand sides each, mixed precision, solved at the same time
   (dd_solve_many_dev): every residual < 1e-10, and each lattice's result equals its own
@@ -43,13 +45,18 @@ def gaussian(N, seed, nrhs=None, dof=2):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-def hierarchy(meta, precision, nrhs, cycle='K', m0=None):
+def hierarchy(meta, precision, nrhs, cycle='K', m0=None, coarse_poly=0):
     lv = meta['levels']
     return ddamg.params(len(lv) + 1, agg=[tuple(l['agg']) for l in lv],
                         sap_block=[tuple(l['sap_block']) for l in lv], nv=[l['nv'] for l in lv],
                         m0=meta['m0'] if m0 is None else m0, nu2=meta['nu2'], fgmres_m=meta['fgmres_m'],
                         cycle=1 if cycle == 'K' else 0, precision=precision, max_rhs=nrhs,
-                        rng_ranks=tuple(meta['ranks']))
+                        rng_ranks=tuple(meta['ranks']), coarse_poly=coarse_poly)
+
+
+# the mixed-precision forms of the batched tests: the reference's K-cycle with its plain
+# coarsest GMRES, and the benchmark's V-cycle with the degree-12 coarsest polynomial
+MIXED_FORMS = [('K', 0), ('V', 12)]
 
 
 def field_ctx(meta):
@@ -119,16 +126,18 @@ def test_512_fp64_and_mixed_batch(golden):
         assert err < 1e-8
 
 
-def test_256_mixed_32rhs_equals_separate_solves(golden):
+@pytest.mark.parametrize('cycle,poly', MIXED_FORMS)
+def test_256_mixed_32rhs_equals_separate_solves(golden, cycle, poly):
     meta, _ = golden('g256_solve')
     N, m0 = meta['N'], meta['m0']
     ctx, U = field_ctx(meta)
     B = gaussian(N, 9002, nrhs=32)
     with ctx:
-        ctx.setup(hierarchy(meta, 1, 32))
+        ctx.setup(hierarchy(meta, 1, 32, cycle, coarse_poly=poly))
         xb, itb, info = ctx.solve(B)
         rr = true_relres(ctx, U, xb, B, m0, oracle_systems=(0, 17))
-        print(f'256^2 mixed, 32 RHS: iterations {itb.min()}..{itb.max()}, max relres {rr.max():.3e}')
+        print(f'256^2 mixed, 32 RHS, {cycle}-cycle, coarse_poly {poly}: iterations {itb.min()}..{itb.max()}, '
+              f'max relres {rr.max():.3e}')
         assert rr.max() < 1e-10
         for r in (0, 13, 31):
             x1, it1, _ = ctx.solve(B[r])
@@ -137,7 +146,8 @@ def test_256_mixed_32rhs_equals_separate_solves(golden):
             assert d < 1e-8
 
 
-def test_ragged_batch_of_eight_fields():
+@pytest.mark.parametrize('cycle,poly', MIXED_FORMS)
+def test_ragged_batch_of_eight_fields(cycle, poly):
     torch = pytest.importorskip('torch')
     sizes = [64, 96, 128, 128, 192, 256, 256, 256]
     rng = np.random.default_rng(4242)
@@ -148,7 +158,7 @@ def test_ragged_batch_of_eight_fields():
         for i, (N, m0) in enumerate(zip(sizes, m0s)):
             c = ddamg.Context(N, N)
             th = c.generate_gauge(2.0, 2000, 300 + i)
-            c.setup(hierarchy(meta, 1, 4, m0=float(m0)))
+            c.setup(hierarchy(meta, 1, 4, cycle, m0=float(m0), coarse_poly=poly))
             ctxs.append(c)
             Us.append(np.exp(1j * th))
             Bs.append(gaussian(N, 400 + i, nrhs=4))
@@ -161,7 +171,8 @@ def test_ragged_batch_of_eight_fields():
             rr = true_relres(c, Us[i], x, Bs[i], float(m0))
             xs, its, _ = c.solve(Bs[i])
             d = rel(x, xs)
-            print(f'ragged {N}^2 m0={m0:.4f}: iterations {infos[i].iterations}, max relres {rr.max():.3e}, '
+            print(f'ragged {N}^2 m0={m0:.4f} ({cycle}-cycle, coarse_poly {poly}): iterations {infos[i].iterations}, '
+                  f'max relres {rr.max():.3e}, '
                   f'vs own solve {d:.3e}')
             assert rr.max() < 1e-10 and infos[i].relres < 1e-10
             assert d < 1e-10
