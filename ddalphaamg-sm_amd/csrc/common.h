@@ -66,6 +66,13 @@ __host__ __device__ __forceinline__ cplx<R> cdiv(cplx<R> a, cplx<R> b) {
     R d = b.x * b.x + b.y * b.y;
     return mk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
+// a / b with one division (by |b|^2) and two multiplications: the block solves'
+// back-substitution, where the quotient's last-bit rounding does not matter
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> cdiv_recip(cplx<R> a, cplx<R> b) {
+    const R id = R(1) / (b.x * b.x + b.y * b.y);
+    return mk((a.x * b.x + a.y * b.y) * id, (a.y * b.x - a.x * b.y) * id);
+}
 template <typename R>
 __host__ __device__ __forceinline__ cplx<R> czero() { return mk<R>(0, 0); }
 

@@ -72,9 +72,9 @@ __device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int re
             }
             const cplx<R> hjj = H[Tri<M>::at(j, j)];
             const cplx<R> hs = mk<R>(hn, 0);
-            const R den = sqrt(norm2(hjj) + norm2(hs));
-            sn[j] = mk<R>(hs.x / den, hs.y / den);
-            cn[j] = mk<R>(hjj.x / den, hjj.y / den);
+            const R iden = R(1) / sqrt(norm2(hjj) + norm2(hs));   // one division per rotation
+            sn[j] = mk<R>(hs.x * iden, hs.y * iden);
+            cn[j] = mk<R>(hjj.x * iden, hjj.y * iden);
             H[Tri<M>::at(j, j)] = cmul(cn[j], hjj) + cmul(sn[j], hs);
             g[j + 1] = -(sn[j] * g[j]);
             g[j] = cmul(cn[j], g[j]);
@@ -92,7 +92,7 @@ __device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int re
 #pragma unroll
                 for (int jj = i + 1; jj < M; jj++)
                     if (jj < maxIt) e = e - H[Tri<M>::at(i, jj)] * eta[jj];
-                eta[i] = cdiv(e, H[Tri<M>::at(i, i)]);
+                eta[i] = cdiv_recip(e, H[Tri<M>::at(i, i)]);
             }
         }
 #pragma unroll
@@ -180,7 +180,9 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
     const bool hasTp = lt < bt - 1, hasTm = lt > 0, hasXp = lx < bx - 1, hasXm = lx > 0;
     const int sTp = hasTp ? lane + 1 : lane, sTm = hasTm ? lane - 1 : lane;
     const int sXp = hasXp ? lane + bt : lane, sXm = hasXm ? lane - bt : lane;
-    const cplx<R> U0 = U[2 * n], U1 = U[2 * n + 1];
+    // links of the hops that leave the block are zero, so the neighbour values shifted in
+    // from outside the block (finite: other sites of the wave, or 0) drop out unmasked
+    const cplx<R> U0 = hasTp ? U[2 * n] : czero<R>(), U1 = hasXp ? U[2 * n + 1] : czero<R>();
     const cplx<R> U0m = hasTm ? conj(U[2 * (n - 1)]) : czero<R>();
     const cplx<R> U1m = hasXm ? conj(U[2 * (n - Nt) + 1]) : czero<R>();
     const R mass = m0 + R(2);
@@ -207,10 +209,6 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
             xm0 = shfl_c(v.a, sXm);
             xm1 = shfl_c(v.b, sXm);
         }
-        if (!hasTp) tp0 = tp1 = czero<R>();
-        if (!hasXp) xp0 = xp1 = czero<R>();
-        if (!hasTm) tm0 = tm1 = czero<R>();
-        if (!hasXm) xm0 = xm1 = czero<R>();
         const cplx<R> a0 =
             U0 * (tp0 - tp1) + U1 * (xp0 + times_i(xp1)) + U0m * (tm0 + tm1) + U1m * (xm0 - times_i(xm1));
         const cplx<R> a1 =
