@@ -112,14 +112,33 @@ __device__ __forceinline__ double row_shift(double v) {
     return __builtin_bit_cast(double, w);
 }
 
+// value of the lane K places before this one in its 16-lane row (DPP row_ror:K)
+template <int K>
+__device__ __forceinline__ float row_rotate(float v) {
+    static_assert(K > 0 && K < 16, "row rotation");
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + K, 0xf, 0xf, true));
+}
+template <int K>
+__device__ __forceinline__ double row_rotate(double v) {
+    static_assert(K > 0 && K < 16, "row rotation");
+    lanes_detail::W2 w = __builtin_bit_cast(lanes_detail::W2, v);
+    w.lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)w.lo, 0x120 + K, 0xf, 0xf, true);
+    w.hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)w.hi, 0x120 + K, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, w);
+}
+
 // sum over each group of 16 lanes, partners at lane offsets 8, 4, 2, 1 in that order (the
-// order of wave_sum(v, 16)); every lane of the group gets the same bits
+// order of wave_sum(v, 16)); every lane of the group gets the same bits.  The partners
+// come from row rotations by 8, 4, 2, 1: after the step with rotation K the values are
+// K-periodic in the row, so the partner K lanes back holds the value of lane ^ K, and
+// every lane adds (own, partner) exactly as the xor butterfly does, one DPP move a step.
 template <typename R>
 __device__ __forceinline__ R seg16_sum(R v, int lane) {
-    v += lane_xor<8>(v, lane);
-    v += lane_xor<4>(v, lane);
-    v += lane_xor<2>(v, lane);
-    v += lane_xor<1>(v, lane);
+    (void)lane;
+    v += row_rotate<8>(v);
+    v += row_rotate<4>(v);
+    v += row_rotate<2>(v);
+    v += row_rotate<1>(v);
     return v;
 }
 
