@@ -189,30 +189,25 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
     (void)base;
     // D_B restricted to the block, zero outside, no boundary sign
     auto A = [&](const Spin2<R>& v) -> Spin2<R> {
-        cplx<R> tp0, tp1, xp0, xp1, tm0, tm1, xm0, xm1;
+        // (1 -+ sigma_mu) of each hop formed on the own site before the lane move, one
+        // complex value per hop: t+ a - b, x+ a + i b, t- a + b, x- a - i b (spin 0's
+        // rows; spin 1's rows of the same projectors are -1, -i, 1 and i times them)
+        const cplx<R> pt = v.a - v.b, px = v.a + times_i(v.b), mt = v.a + v.b, mx = v.a - times_i(v.b);
+        cplx<R> dtp, dxp, dtm, dxm;
         if constexpr (B44) {
-            tp0 = row_shift_c(v.a, 1);
-            tp1 = row_shift_c(v.b, 1);
-            xp0 = row_shift_c(v.a, 4);
-            xp1 = row_shift_c(v.b, 4);
-            tm0 = row_shift_c(v.a, -1);
-            tm1 = row_shift_c(v.b, -1);
-            xm0 = row_shift_c(v.a, -4);
-            xm1 = row_shift_c(v.b, -4);
+            dtp = row_shift_c(pt, 1);
+            dxp = row_shift_c(px, 4);
+            dtm = row_shift_c(mt, -1);
+            dxm = row_shift_c(mx, -4);
         } else {
-            tp0 = shfl_c(v.a, sTp);
-            tp1 = shfl_c(v.b, sTp);
-            xp0 = shfl_c(v.a, sXp);
-            xp1 = shfl_c(v.b, sXp);
-            tm0 = shfl_c(v.a, sTm);
-            tm1 = shfl_c(v.b, sTm);
-            xm0 = shfl_c(v.a, sXm);
-            xm1 = shfl_c(v.b, sXm);
+            dtp = shfl_c(pt, sTp);
+            dxp = shfl_c(px, sXp);
+            dtm = shfl_c(mt, sTm);
+            dxm = shfl_c(mx, sXm);
         }
-        const cplx<R> a0 =
-            U0 * (tp0 - tp1) + U1 * (xp0 + times_i(xp1)) + U0m * (tm0 + tm1) + U1m * (xm0 - times_i(xm1));
-        const cplx<R> a1 =
-            U0 * (tp1 - tp0) + U1 * (xp1 - times_i(xp0)) + U0m * (tm0 + tm1) + U1m * (times_i(xm0) + xm1);
+        const cplx<R> Tp = U0 * dtp, Xp = U1 * dxp, Tm = U0m * dtm, Xm = U1m * dxm;
+        const cplx<R> a0 = Tp + Xp + Tm + Xm;
+        const cplx<R> a1 = Tm - Tp + times_i(Xm) - times_i(Xp);
         return {mass * v.a - R(0.5) * a0, mass * v.b - R(0.5) * a1};
     };
     auto red = [&](cplx<R> v) -> cplx<R> {
