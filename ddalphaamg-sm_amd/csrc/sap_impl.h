@@ -43,11 +43,12 @@ __device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int re
 #pragma unroll
     for (int i = 0; i <= M; i++) V[i] = E::zero();
     cplx<R> H[Tri<M>::size];
-    cplx<R> g[M + 1], cn[M], sn[M];
+    cplx<R> g[M + 1], cn[M];
+    R sn[M];   // the rotations' sines are real: h_{j+1,j} / den with h_{j+1,j} a norm
 #pragma unroll
     for (int i = 0; i < M; i++) {
         cn[i] = czero<R>();
-        sn[i] = czero<R>();
+        sn[i] = R(0);
     }
     for (int k = 0; k < restarts; k++) {
         V[0] = (R(1) / err) * r;
@@ -62,20 +63,19 @@ __device__ __forceinline__ E block_gmres(const E& phi, Op&& A, Red&& red, int re
                 w = w - h * V[i];
             }
             const R hn = sqrt(red(w.norm2()).x);
-            if (j + 1 <= M && hn > R(0)) V[j + 1] = (R(1) / hn) * w;
+            if (j + 1 < M && hn > R(0)) V[j + 1] = (R(1) / hn) * w;   // V_M is never read
             // Givens rotations of column j (src/fgmres.cpp:99-115)
 #pragma unroll
             for (int i = 0; i < j; i++) {
                 const cplx<R> a = H[Tri<M>::at(i, j)], bb = H[Tri<M>::at(i + 1, j)];
-                H[Tri<M>::at(i, j)] = cmul(cn[i], a) + cmul(sn[i], bb);
+                H[Tri<M>::at(i, j)] = cmul(cn[i], a) + sn[i] * bb;
                 H[Tri<M>::at(i + 1, j)] = cn[i] * bb - sn[i] * a;
             }
             const cplx<R> hjj = H[Tri<M>::at(j, j)];
-            const cplx<R> hs = mk<R>(hn, 0);
-            const R iden = R(1) / sqrt(norm2(hjj) + norm2(hs));   // one division per rotation
-            sn[j] = mk<R>(hs.x * iden, hs.y * iden);
+            const R iden = R(1) / sqrt(norm2(hjj) + hn * hn);   // one division per rotation
+            sn[j] = hn * iden;
             cn[j] = mk<R>(hjj.x * iden, hjj.y * iden);
-            H[Tri<M>::at(j, j)] = cmul(cn[j], hjj) + cmul(sn[j], hs);
+            H[Tri<M>::at(j, j)] = cmul(cn[j], hjj) + mk<R>(sn[j] * hn, R(0));
             g[j + 1] = -(sn[j] * g[j]);
             g[j] = cmul(cn[j], g[j]);
             if (sqrt(norm2(g[j + 1])) < tol * nphi) {
