@@ -121,7 +121,9 @@ class Multigrid {
     // [V][5][2][2] (A = 0), as Level::makeDirac builds them (src/level.cpp:286-331)
     void dirac_links_level0(cplx<R>* out);
     void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
-    void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
+    // psi_zero: the caller guarantees psi == 0 on entry; without pre-smoothing the
+    // residual before the restriction is then eta itself and is not formed
+    void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero = false);
     void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     // psi = V-cycle(eta) from level 0 with zero initial guess, replayed from a HIP graph
     // captured on the first call for these buffers and this mask.  false (nothing done)

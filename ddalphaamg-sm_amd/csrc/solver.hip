@@ -857,7 +857,7 @@ bool Multigrid<R>::capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, boo
         try {
             zero<R>(s_, psi, n, n, mask_count(mask), mask);
             if (kcyc) kcycle(0, eta, psi, mask);
-            else vcycle(0, eta, psi, mask);
+            else vcycle(0, eta, psi, mask, true);
         } catch (...) {
             ok = false;
         }
@@ -924,7 +924,7 @@ void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
 }
 
 template <typename R>
-void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero) {
     const int L = levels();
     if (l == L - 1) {
         coarsest(eta, psi, mask);
@@ -934,11 +934,15 @@ void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
     cplx<R>* r = res_[l].as<cplx<R>>();
     cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
     cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
-    apply(l, psi, r, eta, mask);
-    restrict_(l, r, e1, mask);
+    if (p_.nu1 == 0 && psi_zero) {
+        restrict_(l, eta, e1, mask);   // eta - D 0 = eta, bit for bit
+    } else {
+        apply(l, psi, r, eta, mask);
+        restrict_(l, r, e1, mask);
+    }
     const size_t n1 = g_[l + 1].n();
     zero<R>(s_, p1, n1, n1, mask_count(mask), mask);
-    vcycle(l + 1, e1, p1, mask);
+    vcycle(l + 1, e1, p1, mask, true);
     prolong(l, p1, psi, psi, mask);
     if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
 }
@@ -1086,7 +1090,7 @@ void Multigrid<R>::bootstrap(int passes) {
                 apply(l, w + i0 * n, b + i0 * n, w + i0 * n, mk);   // b = w - D w
                 zero<R>(s_, x + i0 * n, n, n, c, mk);
                 if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk);
-                else vcycle(l, b + i0 * n, x + i0 * n, mk);
+                else vcycle(l, b + i0 * n, x + i0 * n, mk, true);
             }
             DD_HIP(hipMemcpyAsync(w, x, sizeof(cplx<R>) * G.nv * n, hipMemcpyDeviceToDevice, s_));
             transpose_tv<R>(s_, x, W_[l].as<cplx<R>>(), G.V(), G.nv, G.dof, true);
@@ -1350,7 +1354,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             }
             zero<double>(s, io, n, n, nr, m);
             if (prm.cycle == 1) mg64->kcycle(0, in, io, m);
-            else mg64->vcycle(0, in, io, m);
+            else mg64->vcycle(0, in, io, m, true);
         } else {
             cplx<float>* a = x32_.as<cplx<float>>();
             cplx<float>* y = y32_.as<cplx<float>>();
@@ -1360,7 +1364,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             } else {
                 zero<float>(s, y, n, n, nr, m);
                 if (prm.cycle == 1) mg32->kcycle(0, a, y, m);
-                else mg32->vcycle(0, a, y, m);
+                else mg32->vcycle(0, a, y, m, true);
             }
             convert<double, float>(s, io, y, n, n, n, nr, m);
         }
