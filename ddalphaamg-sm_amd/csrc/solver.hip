@@ -856,7 +856,7 @@ bool Multigrid<R>::capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, boo
         bool ok = true;
         try {
             zero<R>(s_, psi, n, n, mask_count(mask), mask);
-            if (kcyc) kcycle(0, eta, psi, mask);
+            if (kcyc) kcycle(0, eta, psi, mask, true);
             else vcycle(0, eta, psi, mask, true);
         } catch (...) {
             ok = false;
@@ -897,7 +897,7 @@ void Multigrid<R>::check_sap_failures() {
 }
 
 template <typename R>
-void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask) {
+void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero) {
     const int L = levels();
     if (l == L - 1) {
         coarsest(eta, psi, mask);
@@ -907,8 +907,12 @@ void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
     cplx<R>* r = res_[l].as<cplx<R>>();
     cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
     cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
-    apply(l, psi, r, eta, mask);
-    restrict_(l, r, e1, mask);
+    if (p_.nu1 == 0 && psi_zero) {
+        restrict_(l, eta, e1, mask);   // eta - D 0 = eta, bit for bit
+    } else {
+        apply(l, psi, r, eta, mask);
+        restrict_(l, r, e1, mask);
+    }
     typename Krylov<R>::Func fA = [this, l](const cplx<R>* in, cplx<R>* out, uint64_t m) {
         apply(l + 1, in, out, nullptr, m);
     };
@@ -1089,7 +1093,7 @@ void Multigrid<R>::bootstrap(int passes) {
                 const uint64_t mk = full_mask(c);
                 apply(l, w + i0 * n, b + i0 * n, w + i0 * n, mk);   // b = w - D w
                 zero<R>(s_, x + i0 * n, n, n, c, mk);
-                if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk);
+                if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk, true);
                 else vcycle(l, b + i0 * n, x + i0 * n, mk, true);
             }
             DD_HIP(hipMemcpyAsync(w, x, sizeof(cplx<R>) * G.nv * n, hipMemcpyDeviceToDevice, s_));
@@ -1353,7 +1357,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
                 }
             }
             zero<double>(s, io, n, n, nr, m);
-            if (prm.cycle == 1) mg64->kcycle(0, in, io, m);
+            if (prm.cycle == 1) mg64->kcycle(0, in, io, m, true);
             else mg64->vcycle(0, in, io, m, true);
         } else {
             cplx<float>* a = x32_.as<cplx<float>>();
@@ -1363,7 +1367,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
                 graphs_used = true;
             } else {
                 zero<float>(s, y, n, n, nr, m);
-                if (prm.cycle == 1) mg32->kcycle(0, a, y, m);
+                if (prm.cycle == 1) mg32->kcycle(0, a, y, m, true);
                 else mg32->vcycle(0, a, y, m, true);
             }
             convert<double, float>(s, io, y, n, n, n, nr, m);
