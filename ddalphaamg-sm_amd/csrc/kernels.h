@@ -76,9 +76,12 @@ void coarse_apply(hipStream_t s, const cplx<R>* L, const cplx<R>* in, cplx<R>* o
 
 // ---- SAP block solves (one colour, x += D_B^-1 r on the blocks of that colour) -------
 template <typename R>
+// form_residual: r is the right-hand side v, and each block forms its right-hand side
+// v - D x at its own sites from the current x (the full-lattice operator); valid only
+// where no two blocks of one colour are neighbours (even block counts in both directions)
 void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
                  int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
-                 const int* gate);
+                 const int* gate, bool form_residual = false);
 // variant: kernel variant (systems per work-group, load batching), -1 = default;
 // sapc_variants<R>() lists the compiled ones (autotuned in solver.hip).  Lh: binary16
 // copy of L (to_half), read by variant 33 (the matrix-core kernel on binary16 links);

@@ -3,7 +3,7 @@
 
 namespace dd {
 template void sap_blocks0<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
-                               double, int, int, double, int, uint64_t, size_t, const int*);
+                               double, int, int, double, int, uint64_t, size_t, const int*, bool);
 template void sap_blocksc<float>(hipStream_t, const cplx<float>*, const cplx<float>*, cplx<float>*, int, int, int, int, int,
                                int, double, int, int, double, int, uint64_t, size_t, const int*, int,
                                  const _Float16*);

@@ -3,7 +3,7 @@
 
 namespace dd {
 template void sap_blocks0<double>(hipStream_t, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int, int, int, int,
-                               double, int, int, double, int, uint64_t, size_t, const int*);
+                               double, int, int, double, int, uint64_t, size_t, const int*, bool);
 template void sap_blocksc<double>(hipStream_t, const cplx<double>*, const cplx<double>*, cplx<double>*, int, int, int, int, int,
                                int, double, int, int, double, int, uint64_t, size_t, const int*, int,
                                  const _Float16*);

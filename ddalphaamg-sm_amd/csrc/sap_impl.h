@@ -4,6 +4,7 @@
 // SAP_C::GMRES_D_B hooks) src/fgmres.cpp:5-127 and include/sap.h:14-83, and the
 // block operator Level::SAP_level_l::D_local src/level.cpp:607-653.
 #pragma once
+#include <type_traits>
 #include <vector>
 
 #include "kernels.h"
@@ -157,7 +158,11 @@ __device__ __forceinline__ cplx<R> row_shift_c(cplx<R> v, int d) {   // d: +1, -
 // B44: blocks of 4 x 4 sites, one per 16-lane row: the neighbours inside the block are DPP
 // row shifts by 1 (t) and 4 (x) and the block sums a DPP butterfly over the row, in the
 // order of wave_sum (the same bits); otherwise LDS permutes for any block shape.
-template <typename R, int M, bool B44>
+// FR: rr is the SAP's right-hand side v and the block right-hand side v - D x is formed
+// here at the block's sites (the operator of k_wilson: periodic in x, antiperiodic in t);
+// the neighbours it reads belong to blocks of the other colour, which no work-group of
+// this launch changes.
+template <typename R, int M, bool B44, bool FR>
 __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, const cplx<R>* __restrict__ rr,
                                              cplx<R>* __restrict__ xx, int Nx, int Nt, int bx, int bt, int colour,
                                              R m0, int restarts, R tol, uint64_t mask, size_t ld,
@@ -215,7 +220,23 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
         else return wave_sum(v, S);
     };
     const cplx<R>* rv = rr + rh * ld;
-    const Spin2<R> phi = {rv[2 * n], rv[2 * n + 1]};
+    Spin2<R> phi = {rv[2 * n], rv[2 * n + 1]};
+    if constexpr (FR) {   // phi = v - D x at this site
+        const cplx<R>* xs = xx + rh * ld;
+        const int ntp = (T == Nt - 1) ? n - (Nt - 1) : n + 1, ntm = (T == 0) ? n + (Nt - 1) : n - 1;
+        const int nxp = (X == Nx - 1) ? T : n + Nt, nxm = (X == 0) ? (Nx - 1) * Nt + T : n - Nt;
+        const R sp = (T == Nt - 1) ? R(-1) : R(1), sm = (T == 0) ? R(-1) : R(1);
+        const cplx<R> W0 = U[2 * n], W1 = U[2 * n + 1], W0m = conj(U[2 * ntm]), W1m = conj(U[2 * nxm + 1]);
+        const cplx<R> p0 = xs[2 * n], p1 = xs[2 * n + 1];
+        const cplx<R> tp0 = xs[2 * ntp], tp1 = xs[2 * ntp + 1], xp0 = xs[2 * nxp], xp1 = xs[2 * nxp + 1];
+        const cplx<R> tm0 = xs[2 * ntm], tm1 = xs[2 * ntm + 1], xm0 = xs[2 * nxm], xm1 = xs[2 * nxm + 1];
+        const cplx<R> a0 = W0 * (sp * (tp0 - tp1)) + W1 * (xp0 + times_i(xp1)) + W0m * (sm * (tm0 + tm1)) +
+                           W1m * (xm0 - times_i(xm1));
+        const cplx<R> a1 = W0 * (sp * (tp1 - tp0)) + W1 * (xp1 - times_i(xp0)) + W0m * (sm * (tm0 + tm1)) +
+                           W1m * (times_i(xm0) + xm1);
+        phi.a = phi.a - (mass * p0 - R(0.5) * a0);
+        phi.b = phi.b - (mass * p1 - R(0.5) * a1);
+    }
     const Spin2<R> e = block_gmres<R, M>(phi, A, red, restarts, tol);
     cplx<R>* xv = xx + rh * ld;
     xv[2 * n] = xv[2 * n] + e.a;
@@ -225,20 +246,30 @@ __global__ __launch_bounds__(64) void k_sap0(const cplx<R>* __restrict__ U, cons
 template <typename R>
 void sap_blocks0(hipStream_t s, const cplx<R>* U, const cplx<R>* r, cplx<R>* x, int Nx, int Nt, int bx, int bt,
                  int colour, double m0, int m, int restarts, double tol, int nrhs, uint64_t mask, size_t ld,
-                 const int* gate) {
+                 const int* gate, bool form_residual) {
     timer_begin(s, K_SAP0);
     const int S = bx * bt;
     if (S > 64 || (S & (S - 1)) != 0) throw BadShape("level-0 SAP block must have a power-of-two site count <= 64");
     if (m != kSapM) throw BadShape("SAP block GMRES restart length must be 5");
     const int nblk = (Nx / bx) * (Nt / bt) / 2;
+    if (form_residual && (((Nx / bx) & 1) || ((Nt / bt) & 1)))
+        throw BadShape("level-0 SAP: forming the residual in the block kernel needs even block counts");
     dim3 grid(cdivu(nblk, 64 / S), nrhs);
-    if (bx == 4 && bt == 4)
-        k_sap0<R, kSapM, true><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld,
-                                                   gate, nblk);
-    else
-        k_sap0<R, kSapM, false><<<grid, 64, 0, s>>>(U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld,
-                                                    gate, nblk);
-    timer_end(s, K_SAP0, 0.5 * Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + 3.0 * nact(mask, nrhs)));
+    auto go = [&](auto b44, auto fr) {
+        k_sap0<R, kSapM, decltype(b44)::value, decltype(fr)::value><<<grid, 64, 0, s>>>(
+            U, r, x, Nx, Nt, bx, bt, colour, (R)m0, restarts, (R)tol, mask, ld, gate, nblk);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (bx == 4 && bt == 4) {
+        if (form_residual) go(T_{}, T_{});
+        else go(T_{}, F_{});
+    } else {
+        if (form_residual) go(F_{}, T_{});
+        else go(F_{}, F_{});
+    }
+    // links once, x (and with form_residual its neighbours and v) and the residual per system
+    timer_end(s, K_SAP0, 0.5 * Nx * Nt * 2 * sizeof(cplx<R>) * (1.0 + (form_residual ? 4.0 : 3.0) * nact(mask, nrhs)));
     DD_LAUNCH_CHECK();
 }
 

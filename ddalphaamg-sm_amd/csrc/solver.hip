@@ -660,17 +660,21 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
         if (ran) return;
         fuse_ok_[l] = 0;   // this level's variant has no fused form: separate launches from now on
     }
-    apply(l, x, r, v, mask);
+    // level 0 with a proper red-black colouring (even block counts): each colour's blocks
+    // form v - D x at their own sites, so the full residual is needed only after the black
+    // blocks, for the stopping test (src/sap.cpp:115-156 forms it after both colours)
+    const bool fr = l == 0 && (G.Nx / G.bx) % 2 == 0 && (G.Nt / G.bt) % 2 == 0;
+    if (!fr) apply(l, x, r, v, mask);
     for (int i = 0; i < nu; i++) {
         for (int colour = 0; colour < 2; colour++) {
             if (l == 0)
-                sap_blocks0<R>(s_, U_, r, x, G.Nx, G.Nt, G.bx, G.bt, colour, p_.m0, p_.sap_m, p_.sap_restarts,
-                               p_.sap_tol, nr, mask, n, gate);
+                sap_blocks0<R>(s_, U_, fr ? v : r, x, G.Nx, G.Nt, G.bx, G.bt, colour, p_.m0, p_.sap_m,
+                               p_.sap_restarts, p_.sap_tol, nr, mask, n, gate, fr);
             else
                 sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, colour, p_.m0, p_.sap_m,
                                p_.sap_restarts, p_.sap_tol, nr, mask, n, gate,
                                l < (int)sapc_var_.size() ? sapc_var_[l] : -1, hlinks(l));
-            apply(l, x, r, v, mask);
+            if (!fr || colour == 1) apply(l, x, r, v, mask);
         }
         dot_partial<R>(s_, r, r, n, n, nr, mask, rb_[l]);
         dot_final<R>(s_, rb_[l], nrm + 1, 2, nr, mask);
