@@ -111,7 +111,7 @@ struct SapFuse {
 // right-hand side v - D x formed from the iterate (src/sap.cpp:103-156).  Returns false,
 // launching nothing, where it does not apply (FP64, another variant, odd numbers of
 // blocks in a direction, blocks above 16 sites or dof above 24).  nslice: work-groups of
-// the norm.
+// the norm; 0 launches no norm stage and leaves the gate unchanged.
 template <typename R>
 bool sap_fused_iteration(hipStream_t s, const cplx<R>* L, const _Float16* Lh, const cplx<R>* v, cplx<R>* x, int Nx,
                          int Nt, int dof, int bx, int bt, double m0, int m, int restarts, double tol, int nrhs,

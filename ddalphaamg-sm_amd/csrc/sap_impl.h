@@ -687,10 +687,11 @@ bool sap_fused_iteration(hipStream_t s, const cplx<R>* L, const _Float16* Lh, co
         default:
             break;
     }
-    // both colours' block solves (links once per colour) and the norm pass (links and
-    // the iterate once)
+    // both colours' block solves (links once per colour) and, with nslice > 0, the norm
+    // pass (links and the iterate once)
     const double lb = variant == 33 ? 2.0 * sizeof(_Float16) : (double)sizeof(cplx<R>);
-    timer_end(s, K_SAPC, done ? Nx * Nt * dof * (5.0 * dof * (lb + sizeof(cplx<R>)) +
+    const double lnorm = nslice > 0 ? (double)sizeof(cplx<R>) : 0.0;
+    timer_end(s, K_SAPC, done ? Nx * Nt * dof * (5.0 * dof * (lb + lnorm) +
                                                3.0 * nact(mask, nrhs) * sizeof(cplx<R>))
                               : 0.0);
     DD_LAUNCH_CHECK();

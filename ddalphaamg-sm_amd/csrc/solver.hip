@@ -653,8 +653,10 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
             k_fill_int<<<1, 64, 0, s_>>>(reinterpret_cast<int*>(fz.ctr), 0, 2);
             DD_HIP(hipGetLastError());
             fz.epoch = ++fuse_epoch_[l];
+            // the last iteration's residual norm would feed no stopping test: no norm stage
             ran = sap_fused_iteration<R>(s_, links(l), hlinks(l), v, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, p_.m0, p_.sap_m,
-                                         p_.sap_restarts, p_.sap_tol, nr, mask, n, var, fz, fuse_slices_);
+                                         p_.sap_restarts, p_.sap_tol, nr, mask, n, var, fz,
+                                         i + 1 < nu ? fuse_slices_ : 0);
             if (!ran && i > 0) throw BadShape("fused SAP became unavailable between iterations");
         }
         if (ran) return;
@@ -663,9 +665,12 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
     // level 0 with a proper red-black colouring (even block counts): each colour's blocks
     // form v - D x at their own sites, so the full residual is needed only after the black
     // blocks, for the stopping test (src/sap.cpp:115-156 forms it after both colours)
+    // The residual after the last iteration's black blocks would feed only a stopping test
+    // that no iteration follows, so it is not formed (nor its norm).
     const bool fr = l == 0 && (G.Nx / G.bx) % 2 == 0 && (G.Nt / G.bt) % 2 == 0;
     if (!fr) apply(l, x, r, v, mask);
     for (int i = 0; i < nu; i++) {
+        const bool last = i + 1 == nu;
         for (int colour = 0; colour < 2; colour++) {
             if (l == 0)
                 sap_blocks0<R>(s_, U_, fr ? v : r, x, G.Nx, G.Nt, G.bx, G.bt, colour, p_.m0, p_.sap_m,
@@ -674,8 +679,9 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
                 sap_blocksc<R>(s_, links(l), r, x, G.Nx, G.Nt, G.dof, G.bx, G.bt, colour, p_.m0, p_.sap_m,
                                p_.sap_restarts, p_.sap_tol, nr, mask, n, gate,
                                l < (int)sapc_var_.size() ? sapc_var_[l] : -1, hlinks(l));
-            if (!fr || colour == 1) apply(l, x, r, v, mask);
+            if (colour == 0 ? !fr : !last) apply(l, x, r, v, mask);
         }
+        if (last) break;
         dot_partial<R>(s_, r, r, n, n, nr, mask, rb_[l]);
         dot_final<R>(s_, rb_[l], nrm + 1, 2, nr, mask);
         gate_update(s_, gate, nrm + 1, nrm + 0, 2, tol, nr, mask);

@@ -9,7 +9,8 @@ every block solve makes exactly 1 + 5 x (5 + 1) = 31 applications, so the count 
 SAP iteration is fixed by the geometry alone:
 
 - fused (one launch per iteration): blocks x (31 x in-block pairs + 5 x sites per block)
-  + lattice sites x 5 (the norm stencil);
+  + lattice sites x 5 (the norm stencil), the norm stencil in every iteration but the
+  last (whose residual norm would feed no stopping test);
 - separate launches per colour: blocks x 31 x in-block pairs;
 
 with 64 in-block (site, hop) pairs on a 4x4 block (16 self, 24 in t, 24 in x).
@@ -47,10 +48,10 @@ def test_sap_flop_count(fused):
     inblock = S + 2 * bx * (bt - 1) + 2 * bt * (bx - 1)
     assert inblock == 64
     if fused:
-        pairs = blocks * (31 * inblock + 5 * S) + Nx * Nt * 5
+        pairs = nu * blocks * (31 * inblock + 5 * S) + (nu - 1) * Nx * Nt * 5
     else:
-        pairs = blocks * 31 * inblock
-    expected = nu * pairs * 128 * dof * dof
+        pairs = nu * blocks * 31 * inblock
+    expected = pairs * 128 * dof * dof
     print(f'fused {fused}: {launches} launches, {flops:.0f} flops counted, {expected} expected')
     assert launches == (nu if fused else 2 * nu)
     assert flops == expected
