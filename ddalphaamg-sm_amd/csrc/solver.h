@@ -120,11 +120,12 @@ class Multigrid {
     // level-0 link matrices of the Wilson-Dirac operator in the coarse-link layout
     // [V][5][2][2] (A = 0), as Level::makeDirac builds them (src/level.cpp:286-331)
     void dirac_links_level0(cplx<R>* out);
-    void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero = false);
-    // psi_zero (both cycles): the caller guarantees psi == 0 on entry; without
-    // pre-smoothing the residual before the restriction is then eta itself and is not
-    // formed
-    void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero = false);
+    void kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool zero_guess = false);
+    // zero_guess (both cycles): the initial guess is zero and psi need not be set; psi is
+    // zeroed only where it is read before being written (the coarsest solve, the
+    // pre-smoothing), and without pre-smoothing the residual before the restriction is
+    // eta itself and the prolongation writes psi = P p1
+    void vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool zero_guess = false);
     void coarsest(const cplx<R>* eta, cplx<R>* psi, uint64_t mask);
     // psi = V-cycle(eta) from level 0 with zero initial guess, replayed from a HIP graph
     // captured on the first call for these buffers and this mask.  false (nothing done)

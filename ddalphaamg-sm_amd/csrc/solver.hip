@@ -865,8 +865,7 @@ bool Multigrid<R>::capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, boo
         dc_ = kcyc;
         bool ok = true;
         try {
-            zero<R>(s_, psi, n, n, mask_count(mask), mask);
-            if (kcyc) kcycle(0, eta, psi, mask, true);
+            if (kcyc) kcycle(0, eta, psi, mask, true);   // zero initial guess
             else vcycle(0, eta, psi, mask, true);
         } catch (...) {
             ok = false;
@@ -907,8 +906,13 @@ void Multigrid<R>::check_sap_failures() {
 }
 
 template <typename R>
-void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero) {
+void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool zero_guess) {
     const int L = levels();
+    // a zero initial guess is stored only where it is read before psi is written: by the
+    // coarsest solve and by the pre-smoothing; otherwise the prolongation writes psi
+    const bool fresh = zero_guess && p_.nu1 == 0;
+    if (zero_guess && (!fresh || l == L - 1))
+        zero<R>(s_, psi, g_[l].n(), g_[l].n(), mask_count(mask), mask);
     if (l == L - 1) {
         coarsest(eta, psi, mask);
         return;
@@ -917,7 +921,7 @@ void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
     cplx<R>* r = res_[l].as<cplx<R>>();
     cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
     cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
-    if (p_.nu1 == 0 && psi_zero) {
+    if (fresh) {
         restrict_(l, eta, e1, mask);   // eta - D 0 = eta, bit for bit
     } else {
         apply(l, psi, r, eta, mask);
@@ -933,13 +937,17 @@ void Multigrid<R>::kcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
         std::vector<int> its(64);
         kry_[l + 1]->solve(e1, e1, p1, mask, fA, &fM, its.data(), nullptr);
     }
-    prolong(l, p1, psi, psi, mask);
+    prolong(l, p1, fresh ? nullptr : psi, psi, mask);   // psi = P p1 (+ psi)
     if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
 }
 
 template <typename R>
-void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool psi_zero) {
+void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask, bool zero_guess) {
     const int L = levels();
+    // as in kcycle: a zero initial guess is stored only where it is read first
+    const bool fresh = zero_guess && p_.nu1 == 0;
+    if (zero_guess && (!fresh || l == L - 1))
+        zero<R>(s_, psi, g_[l].n(), g_[l].n(), mask_count(mask), mask);
     if (l == L - 1) {
         coarsest(eta, psi, mask);
         return;
@@ -948,16 +956,14 @@ void Multigrid<R>::vcycle(int l, const cplx<R>* eta, cplx<R>* psi, uint64_t mask
     cplx<R>* r = res_[l].as<cplx<R>>();
     cplx<R>* e1 = eta_[l + 1].template as<cplx<R>>();
     cplx<R>* p1 = psi_[l + 1].template as<cplx<R>>();
-    if (p_.nu1 == 0 && psi_zero) {
+    if (fresh) {
         restrict_(l, eta, e1, mask);   // eta - D 0 = eta, bit for bit
     } else {
         apply(l, psi, r, eta, mask);
         restrict_(l, r, e1, mask);
     }
-    const size_t n1 = g_[l + 1].n();
-    zero<R>(s_, p1, n1, n1, mask_count(mask), mask);
     vcycle(l + 1, e1, p1, mask, true);
-    prolong(l, p1, psi, psi, mask);
+    prolong(l, p1, fresh ? nullptr : psi, psi, mask);   // psi = P p1 (+ psi)
     if (p_.nu2 > 0) sap(l, eta, psi, p_.nu2, 1e-10, mask);
 }
 
@@ -1102,8 +1108,7 @@ void Multigrid<R>::bootstrap(int passes) {
                 const int c = std::min(cb, G.nv - i0);
                 const uint64_t mk = full_mask(c);
                 apply(l, w + i0 * n, b + i0 * n, w + i0 * n, mk);   // b = w - D w
-                zero<R>(s_, x + i0 * n, n, n, c, mk);
-                if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk, true);
+                if (p_.cycle == 1) kcycle(l, b + i0 * n, x + i0 * n, mk, true);   // zero initial guess
                 else vcycle(l, b + i0 * n, x + i0 * n, mk, true);
             }
             DD_HIP(hipMemcpyAsync(w, x, sizeof(cplx<R>) * G.nv * n, hipMemcpyDeviceToDevice, s_));
@@ -1366,8 +1371,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
                     return;
                 }
             }
-            zero<double>(s, io, n, n, nr, m);
-            if (prm.cycle == 1) mg64->kcycle(0, in, io, m, true);
+            if (prm.cycle == 1) mg64->kcycle(0, in, io, m, true);   // zero initial guess
             else mg64->vcycle(0, in, io, m, true);
         } else {
             cplx<float>* a = x32_.as<cplx<float>>();
@@ -1376,8 +1380,7 @@ int Solver::solve(int nrhs, const cplx<double>* b, cplx<double>* x, int* iters, 
             if (graph && (kc ? mg32->kcycle_graph(a, y, m) : mg32->vcycle_graph(a, y, m))) {
                 graphs_used = true;
             } else {
-                zero<float>(s, y, n, n, nr, m);
-                if (prm.cycle == 1) mg32->kcycle(0, a, y, m, true);
+                if (prm.cycle == 1) mg32->kcycle(0, a, y, m, true);   // zero initial guess
                 else mg32->vcycle(0, a, y, m, true);
             }
             convert<double, float>(s, io, y, n, n, n, nr, m);
