@@ -662,11 +662,11 @@ void Multigrid<R>::sap(int l, const cplx<R>* v, cplx<R>* x, int nu, double tol, 
         if (ran) return;
         fuse_ok_[l] = 0;   // this level's variant has no fused form: separate launches from now on
     }
-    // level 0 with a proper red-black colouring (even block counts): each colour's blocks
+    // Level 0 with a proper red-black colouring (even block counts): each colour's blocks
     // form v - D x at their own sites, so the full residual is needed only after the black
-    // blocks, for the stopping test (src/sap.cpp:115-156 forms it after both colours)
-    // The residual after the last iteration's black blocks would feed only a stopping test
-    // that no iteration follows, so it is not formed (nor its norm).
+    // blocks, for the stopping test (src/sap.cpp:115-156 forms it after both colours).
+    // After the last iteration's black blocks it would feed only a stopping test that no
+    // iteration follows, so it is not formed there (nor its norm).
     const bool fr = l == 0 && (G.Nx / G.bx) % 2 == 0 && (G.Nt / G.bt) % 2 == 0;
     if (!fr) apply(l, x, r, v, mask);
     for (int i = 0; i < nu; i++) {
@@ -857,7 +857,6 @@ bool Multigrid<R>::capture_(const cplx<R>* eta, cplx<R>* psi, uint64_t mask, boo
     const auto key = std::make_tuple(static_cast<const void*>(eta), static_cast<const void*>(psi), mask, kcyc);
     auto it = graphs_.find(key);
     if (it == graphs_.end()) {
-        const size_t n = g_[0].n();
         hipGraph_t g = nullptr;
         hipGraphExec_t ex = nullptr;
         DD_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
