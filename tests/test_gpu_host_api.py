@@ -137,23 +137,57 @@ def test_reference_main_on_the_library(golden, tmp_path):
     assert (tmp_path / 'metadata_64x64_Levels2_Kcycle.dat').exists()
 
 
+@pytest.mark.parametrize('nu1', [0, 1])
 @pytest.mark.parametrize('kcycle', [True, False])
-def test_cycle_matches_oracle(golden, kcycle):
-    """One preconditioner application (K- or V-cycle from level 0, zero initial guess)
-    equals the oracle's AlgebraicMG::k_cycle / v_cycle on the same field."""
+def test_cycle_matches_oracle(golden, kcycle, nu1):
+    """One preconditioner application (K- or V-cycle from level 0, zero initial guess,
+    with and without pre-smoothing) equals the oracle's AlgebraicMG::k_cycle / v_cycle on
+    the same field."""
     meta, a = golden('g16_setup3')
     U = np.exp(1j * a['theta'])
     lv = meta['levels']
     levels = [orc.LevelSpec(tuple(l['agg']), tuple(l['sap_block']), l['nv']) for l in lv]
-    prm_o = orc.AmgParams(levels=levels, m0=meta['m0'], rng_ranks=tuple(meta['ranks']))
+    prm_o = orc.AmgParams(levels=levels, m0=meta['m0'], rng_ranks=tuple(meta['ranks']), nu1=nu1)
     amg = orc.AMG(U, prm_o)
     amg.setup()
     ref = (amg.k_cycle if kcycle else amg.v_cycle)(0, a['rhs'], np.zeros_like(a['rhs']))
     prm = ddamg.params(3, agg=[tuple(l['agg']) for l in lv], sap_block=[tuple(l['sap_block']) for l in lv],
                        nv=[l['nv'] for l in lv], m0=meta['m0'], rng_ranks=tuple(meta['ranks']),
-                       ortho_coarse=0)
+                       ortho_coarse=0, nu1=nu1)
     with ddamg.Context(16, 16) as ctx:
         ctx.set_gauge(U)
         ctx.setup(prm)
         out = ctx.cycle(0, a['rhs'], np.zeros_like(a['rhs']), kcycle=kcycle)
     assert rel(out, ref) < 1e-9
+
+
+@pytest.mark.parametrize('cycle', ['K', 'V'])
+def test_presmoothing_solve_matches_oracle(golden, cycle):
+    """With one pre-smoothing SAP iteration (nu1 = 1) the solve, whose cycles start from a
+    zero initial guess inside the FGMRES preconditioner (so the library forms the zero
+    guess, the pre-smoothing and the residual itself), gives the oracle's iteration count
+    and solution in FP64, and the mixed-precision solve reaches the same FP64 residual
+    bound (src/fgmres.cpp:77-90)."""
+    meta, a = golden('g16_setup3')
+    U = np.exp(1j * a['theta'])
+    lv = meta['levels']
+    levels = [orc.LevelSpec(tuple(l['agg']), tuple(l['sap_block']), l['nv']) for l in lv]
+    amg = orc.AMG(U, orc.AmgParams(levels=levels, m0=meta['m0'], rng_ranks=tuple(meta['ranks']), nu1=1))
+    amg.setup()
+    res = amg.solve(a['rhs'], cycle)
+    prm = ddamg.params(3, agg=[tuple(l['agg']) for l in lv], sap_block=[tuple(l['sap_block']) for l in lv],
+                       nv=[l['nv'] for l in lv], m0=meta['m0'], rng_ranks=tuple(meta['ranks']),
+                       ortho_coarse=0, nu1=1, cycle=1 if cycle == 'K' else 0)
+    with ddamg.Context(16, 16) as ctx:
+        ctx.set_gauge(U)
+        ctx.setup(prm)
+        x, its, info = ctx.solve(a['rhs'])
+        prm.precision = 1
+        ctx.setup(prm)
+        xm, itm, infom = ctx.solve(a['rhs'])
+    rr = lambda y: np.linalg.norm(a['rhs'] - orc.D_phi(U, y, meta['m0'])) / np.linalg.norm(a['rhs'])
+    print(f'nu1 = 1, {cycle}-cycle: {its[0]} iterations (oracle {res.iterations[0]}), relres {rr(x):.2e}, '
+          f'vs oracle x {rel(x, res.x[0]):.2e}; mixed {itm[0]} iterations, relres {rr(xm):.2e}')
+    assert int(its[0]) == int(res.iterations[0])
+    assert rel(x, res.x[0]) < 1e-9 and rr(x) < 1e-10
+    assert rr(xm) < 1e-10 and infom.relres < 1e-10
