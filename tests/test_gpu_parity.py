@@ -504,3 +504,36 @@ def test_fused_sap_iteration_equals_separate_launches(links16):
     one_step0 = [r for r in range(20) if rel(res[0][1][r], res[0][2][r]) < 1e-6]
     print('systems stopped after one iteration:', one_step)
     assert one_step == one_step0 and 5 in one_step
+
+
+@pytest.mark.parametrize('nu', [1, 2])
+@pytest.mark.parametrize('shape', [(16, 16), (20, 24), (24, 20)])
+def test_level0_sap_block_counts(shape, nu):
+    """Level-0 SAP against the oracle's SAP (src/sap.cpp:81-165) on the library's own
+    level-0 operator, for 3 systems: with even block counts (the blocks form v - D x at
+    their own sites) and with an odd count in x or in t (full residuals between the
+    colours), after 1 and 2 iterations (no residual after a call's last iteration).  FP64
+    to 1e-11; the FP32 hierarchy to FP32 accuracy."""
+    Nx, Nt = shape
+    rng = np.random.default_rng(Nx * 100 + Nt + nu)
+    U = np.exp(1j * rng.uniform(-np.pi, np.pi, (Nx, Nt, 2)) * 0.5)
+    v = rng.standard_normal((3, Nx, Nt, 2)) + 1j * rng.standard_normal((3, Nx, Nt, 2))
+    x0 = 0.1 * (rng.standard_normal((3, Nx, Nt, 2)) + 1j * rng.standard_normal((3, Nx, Nt, 2)))
+    m0 = -0.2
+    refs = None
+    for precision in (0, 1):
+        prm = ddamg.params(2, agg=[(4, 4)], sap_block=[(4, 4)], nv=[8], m0=m0, precision=precision, max_rhs=3,
+                           rng_ranks=(1, 1))
+        with ddamg.Context(Nx, Nt) as ctx:
+            ctx.set_gauge(U)
+            ctx.setup(prm)
+            if refs is None:   # the oracle on the FP64 hierarchy's level-0 operator
+                A, B, Cm = ctx.links(0)
+                op = orc.Stencil(A, B, Cm, m0)
+                blocks = orc.SapBlocks(Nx, Nt, 4, 4)
+                refs = [orc.sap(op, blocks, v[r], x0[r], nu, 1e-10)[0] for r in range(3)]
+            x = ctx.sap(0, v, x0, nu, precision=precision)
+            for r in range(3):
+                err = rel(x[r], refs[r])
+                print(f'{Nx}x{Nt}, nu {nu}, precision {precision}, system {r}: {err:.2e}')
+                assert err < (1e-11 if precision == 0 else 2e-5)
