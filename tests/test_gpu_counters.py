@@ -22,9 +22,10 @@ pytestmark = pytest.mark.gpu
 ddamg = pytest.importorskip('ddamg')
 
 
+@pytest.mark.parametrize('nu', [1, 2])
 @pytest.mark.parametrize('fused', [1, 0])
-def test_sap_flop_count(fused):
-    N, nrhs, nu = 64, 16, 2
+def test_sap_flop_count(fused, nu):
+    N, nrhs = 64, 16
     rng = np.random.default_rng(41)
     U = np.exp(1j * rng.uniform(-np.pi, np.pi, (N, N, 2)) * 0.5)
     with ddamg.Context(N, N) as ctx:
